@@ -1,0 +1,146 @@
+/*
+ * bve.h - C ABI of the MI355X Biot-Savart / RK4 library (libbve.so).
+ *
+ * Drop-in boundary for the velocity path of the barotropic-vorticity
+ * particle-panel solver.  Every entry point takes plain pointers and sizes
+ * (host memory unless stated otherwise), returns BVE_OK or an error code,
+ * and leaves a message for bve_last_error().  Calls are synchronous at the
+ * boundary: host buffers are read or written before the call returns.
+ *
+ * Array conventions (those of the reference's Fortran types,
+ * ParticlesEdgesPanels2.f90:45-89):
+ *   positions  double[3*n], point j at [3j, 3j+1, 3j+2]: the column-major x(3,N)
+ *   scalars    double[n]
+ *   active panels  = leaf panels in GatherPanels order (ParticlesEdgesPanels2.f90:1556-1619)
+ *   passive panels = divided (parent) panels in the same order
+ *
+ * A context is one instance of the reference module's SAVE state: it is not
+ * re-entrant; use one per host thread or per ensemble member.
+ */
+#ifndef BVE_H
+#define BVE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bve_ctx bve_ctx;
+
+enum {
+    BVE_OK = 0,
+    BVE_ERR_ARGUMENT = 1, /* bad size, null pointer or out-of-range value */
+    BVE_ERR_STATE = 2,    /* call out of order (e.g. step before panels are loaded) */
+    BVE_ERR_DEVICE = 3,   /* a HIP call failed */
+    BVE_ERR_MEMORY = 4    /* device allocation failed */
+};
+
+enum { BVE_PRECISION_FP64 = 0, BVE_PRECISION_MIXED = 1 };
+
+/* ---- lifetime -------------------------------------------------------- */
+
+/* New context on HIP device `device`.  No counterpart in the reference
+ * (its state is module SAVE data, PanelVelocityParallel.f90:25-89). */
+int bve_create(bve_ctx **ctx, int device);
+int bve_destroy(bve_ctx *ctx);
+/* Message of the last failed call on ctx (or of bve_create when ctx is NULL). */
+const char *bve_last_error(const bve_ctx *ctx);
+
+/* ---- parameters ------------------------------------------------------ */
+
+/* Rotation rate Omega in dzeta/dt = -2 Omega u_z.  Replaces SetOmega,
+ * TracerAndVorticityDistributions.f90:136-139 (default 2 PI per day, :34-35). */
+int bve_set_omega(bve_ctx *ctx, double omega);
+/* Regularisation of the passive-panel sum, denominator (1 - x.y) + smooth^2.
+ * Replaces SetSmooth / GetSmooth, PanelVelocityParallel.f90:96-105 (default 0.05, :27). */
+int bve_set_smooth(bve_ctx *ctx, double smooth);
+int bve_get_smooth(const bve_ctx *ctx, double *smooth);
+/* Arithmetic of the pair sum: BVE_PRECISION_FP64 (default) or
+ * BVE_PRECISION_MIXED (float32 far field, float64 for pairs closer than
+ * nearfield_radius, float64 target reduction).  No reference counterpart. */
+int bve_set_precision(bve_ctx *ctx, int precision, double nearfield_radius);
+/* 1 = project every position back onto the unit sphere after each step
+ * (RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702, applied on the
+ * device); 0 (default) = the reference's BVERK4, which never does. */
+int bve_set_renormalize(bve_ctx *ctx, int on);
+/* Launch geometry of the pair kernel: targets held per lane (1, 2, 4),
+ * sources per LDS tile (256, 512) and the workgroup count to aim for; 0
+ * keeps the built-in choice.  Used by the autotuner. */
+int bve_set_plan(bve_ctx *ctx, int targets_per_lane, int tile, int target_workgroups);
+
+/* ---- panel RK4 path (drop-in for PanelVelocityParallelModule) --------- */
+
+/* InitRK4 + GatherPanels payload (PanelVelocityParallel.f90:236-318): the
+ * active panels' positions, relative vorticity and areas and the passive
+ * panels' positions are copied to the device and cached there until
+ * bve_reset, exactly as the module caches its private panel copies. */
+int bve_load_panels(bve_ctx *ctx, int n_active, const double *x_active,
+                    const double *relvort_active, const double *area_active,
+                    int n_passive, const double *x_passive);
+
+/* BVERK4(aParticles, aPanels, dt, procRank, numProcs),
+ * PanelVelocityParallel.f90:435-658: one classical RK4 step of the cached
+ * panels and of the n_particles particles (read from x_particles every
+ * call and overwritten with the new positions).  If absvort is given, the
+ * particles' relative vorticity absVort - 2 Omega z is written to relvort
+ * (:651).  total_ke (scalar) and ke (n_active) receive totalKE and
+ * kineticEnergy of stage 1 (:495-501); any of them may be NULL. */
+int bve_rk4_step(bve_ctx *ctx, double dt, int n_particles, double *x_particles,
+                 const double *absvort, double *relvort, double *total_ke, double *ke);
+
+/* ScatterPanels payload (ParticlesEdgesPanels2.f90:1622-1637): current
+ * active positions and relative vorticity and passive positions.  Any
+ * pointer may be NULL. */
+int bve_store_panels(bve_ctx *ctx, double *x_active, double *relvort_active, double *x_passive);
+
+/* ResetRK4, PanelVelocityParallel.f90:321-380: drop the cached panels and
+ * particles (call after a remesh). */
+int bve_reset(bve_ctx *ctx);
+
+/* ---- device-resident stepping (no host transfer per step) ------------ */
+
+/* Upload particles once (and their absolute vorticity, may be NULL). */
+int bve_load_particles(bve_ctx *ctx, int n_particles, const double *x_particles,
+                       const double *absvort);
+/* n_steps RK4 steps on the resident state; asynchronous on the context's
+ * stream until bve_synchronize or a store call. */
+int bve_advance(bve_ctx *ctx, double dt, int n_steps);
+int bve_store_particles(bve_ctx *ctx, double *x_particles, double *relvort);
+int bve_get_kinetic_energy(bve_ctx *ctx, double *total_ke, double *ke);
+int bve_synchronize(bve_ctx *ctx);
+
+/* ---- single evaluation (BVEActiveRHS / BVEPassiveRHS / BVESmoothVelocity) */
+
+/* u (3*n_targets) at the targets from the n_sources sources with weight
+ * vort*area:  targets [0, self_end) skip the source of the same index
+ * (BVEActiveRHS, PanelVelocityParallel.f90:1287-1334, self_end <= n_sources),
+ * targets [smooth_begin, smooth_end) use the smoothed denominator
+ * (BVESmoothVelocity, :1376-1412), all others the singular one
+ * (BVEPassiveRHS, :1337-1373).  dvort (self_end entries, may be NULL)
+ * receives -2 Omega u_z (:1333). */
+int bve_velocity(bve_ctx *ctx, int n_targets, const double *x_targets, int self_end,
+                 int smooth_begin, int smooth_end, int n_sources, const double *x_sources,
+                 const double *vort, const double *area, double *u, double *dvort);
+
+/* ---- Voronoi path (drop-in for VoronoiVelocity{Serial,Parallel}Module) */
+
+/* BVERK4(aPanels, dt[, procRank, numProcs]) of VoronoiVelocityParallel.f90:304-483
+ * on n generators in SoA form (x, y, z, relvort in/out, area in), including
+ * NormalizeGrid (:480); the host re-triangulates afterwards.  total_ke, ke
+ * as for the panel step (may be NULL). */
+int bve_voronoi_rk4_step(bve_ctx *ctx, double dt, int n, double *x, double *y, double *z,
+                         double *relvort, const double *area, double *total_ke, double *ke);
+
+/* ---- measurement ------------------------------------------------------ */
+
+/* 1 = bracket every pair-kernel launch with HIP events on the context's
+ * stream; bve_get_profile returns the launches and their summed duration
+ * since profiling was switched on. */
+int bve_set_profiling(bve_ctx *ctx, int on);
+int bve_get_profile(bve_ctx *ctx, long long *launches, double *total_ms);
+/* Pairs evaluated by one RK4 step of the loaded problem: 4 (N_a (N_a - 1) + (N_p + N_pp) N_a). */
+int bve_pairs_per_step(const bve_ctx *ctx, double *pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BVE_H */

@@ -1,0 +1,276 @@
+/*
+ * CPU oracle for the Biot-Savart / RK4 path of the barotropic-vorticity
+ * particle-panel solver.  TEST INFRASTRUCTURE ONLY: the test suite, the
+ * smoke check and the CPU-baseline leg of the benchmark call it as the
+ * checker.  The product library never links or calls it.
+ *
+ * Each function restates one reference routine, in the reference's loop
+ * order and rounding points, so that compiled with -O2 -ffp-contract=off it
+ * reproduces the reference's float64 results (the golden vectors in
+ * tests/golden/ were written by the reference compiled with flang -O2 on
+ * x86-64, which has no FMA contraction).
+ *
+ * Layout: positions are AoS x[3*i+c], the column-major x(3,N) of the
+ * reference.  Indices are 0-based; ranges are half-open [i0, i1).
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   bvo_active_rhs, bvo_passive_rhs, bvo_smooth_velocity, bvo_rk4_step:
+ *       golden vectors from the compiled reference (bitwise or <= 1e-15),
+ *       the level-0 cube by hand, tangency u.x = 0, active-block momentum
+ *       sum_i w_i u_i = 0, and a __float128 brute force (bvo_velocity_q).
+ *   bvo_voronoi_velocity: equals bvo_active_rhs on SoA input (same sum,
+ *       the reference's sequential-subtraction denominator), tangency.
+ *   bvo_rk4_step: RK4 polynomial on a solid-body flow, zeta + 2 Omega z
+ *       invariant, golden vectors over 12 steps.
+ *   bvo_renormalize: |x| = 1 within 1 ulp.
+ *   bvo_velocity_q: the level-0 cube in closed form, and bvo_active_rhs.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* PI of NumberKinds.f90:12, rounded to float64 */
+static const double BVO_PI = 3.141592653589793238462643383279502797479068098137295573004504331874296718662975536062731407582759857177734375;
+
+/* BVEActiveRHS, PanelVelocityParallel.f90:1287-1334.
+ * Targets j in [i0, i1) of the n active panels, sources all active panels
+ * except k == j (the loop split at :1319 / :1325).  dX is zeroed for all n
+ * (the reference zeroes the whole array, :1316-1317), then divided by
+ * (-4 PI) and dVort = -2 Omega dX(3,:) (:1332-1333). */
+void bvo_active_rhs(int n, const double *x, const double *vort, const double *area,
+                    int i0, int i1, double omega, double *dx, double *dvort)
+{
+    memset(dx, 0, sizeof(double) * 3 * (size_t)n);
+    memset(dvort, 0, sizeof(double) * (size_t)n);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int j = i0; j < i1; ++j) {
+        const double xj0 = x[3 * j], xj1 = x[3 * j + 1], xj2 = x[3 * j + 2];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            const double xk0 = x[3 * k], xk1 = x[3 * k + 1], xk2 = x[3 * k + 2];
+            const double denom = 1.0 - ((xk0 * xj0 + xk1 * xj1) + xk2 * xj2);
+            a0 = a0 + ((xj1 * xk2 - xj2 * xk1) * vort[k]) * area[k] / denom;
+            a1 = a1 + ((xj2 * xk0 - xj0 * xk2) * vort[k]) * area[k] / denom;
+            a2 = a2 + ((xj0 * xk1 - xj1 * xk0) * vort[k]) * area[k] / denom;
+        }
+        dx[3 * j] = a0;
+        dx[3 * j + 1] = a1;
+        dx[3 * j + 2] = a2;
+    }
+    const double c = -4.0 * BVO_PI;
+    for (int j = 0; j < n; ++j) {
+        dx[3 * j] /= c;
+        dx[3 * j + 1] /= c;
+        dx[3 * j + 2] /= c;
+        dvort[j] = -(2.0 * omega * dx[3 * j + 2]);
+    }
+}
+
+/* BVEPassiveRHS (PanelVelocityParallel.f90:1337-1373) for eps2 == 0 and
+ * BVESmoothVelocity (:1376-1412) for eps2 = smooth*smooth: the sum at m
+ * targets X against all n active sources, no exclusion.  The smoothed
+ * denominator is (1 - x.y) + smooth^2 (:1405). */
+static void bvo_target_sum(int m, const double *X, int n, const double *x,
+                           const double *vort, const double *area, int i0, int i1,
+                           int smoothed, double eps2, double *dx)
+{
+    memset(dx, 0, sizeof(double) * 3 * (size_t)m);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int j = i0; j < i1; ++j) {
+        const double X0 = X[3 * j], X1 = X[3 * j + 1], X2 = X[3 * j + 2];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double xk0 = x[3 * k], xk1 = x[3 * k + 1], xk2 = x[3 * k + 2];
+            double denom = 1.0 - ((xk0 * X0 + xk1 * X1) + xk2 * X2);
+            if (smoothed) denom = denom + eps2;
+            a0 = a0 + ((X1 * xk2 - X2 * xk1) * vort[k]) * area[k] / denom;
+            a1 = a1 + ((X2 * xk0 - X0 * xk2) * vort[k]) * area[k] / denom;
+            a2 = a2 + ((X0 * xk1 - X1 * xk0) * vort[k]) * area[k] / denom;
+        }
+        dx[3 * j] = a0;
+        dx[3 * j + 1] = a1;
+        dx[3 * j + 2] = a2;
+    }
+    const double c = -4.0 * BVO_PI;
+    for (int j = 0; j < 3 * m; ++j) dx[j] /= c;
+}
+
+void bvo_passive_rhs(int m, const double *X, int n, const double *x, const double *vort,
+                     const double *area, int i0, int i1, double *dx)
+{
+    bvo_target_sum(m, X, n, x, vort, area, i0, i1, 0, 0.0, dx);
+}
+
+void bvo_smooth_velocity(int m, const double *X, int n, const double *x, const double *vort,
+                         const double *area, int i0, int i1, double smooth, double *dx)
+{
+    bvo_target_sum(m, X, n, x, vort, area, i0, i1, 1, smooth * smooth, dx);
+}
+
+/* BVEVelocity (Voronoi), VoronoiVelocityParallel.f90:487-515: SoA input,
+ * denominator 1 - x_j x_k - y_j y_k - z_j z_k subtracted left to right. */
+void bvo_voronoi_velocity(int n, const double *x, const double *y, const double *z,
+                          const double *vort, const double *area, int i0, int i1,
+                          double omega, double *dx, double *dy, double *dz, double *dvort)
+{
+    memset(dx, 0, sizeof(double) * (size_t)n);
+    memset(dy, 0, sizeof(double) * (size_t)n);
+    memset(dz, 0, sizeof(double) * (size_t)n);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int j = i0; j < i1; ++j) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            const double denom = 1.0 - x[j] * x[k] - y[j] * y[k] - z[j] * z[k];
+            a0 = a0 + ((y[j] * z[k] - z[j] * y[k]) * vort[k]) * area[k] / denom;
+            a1 = a1 + ((z[j] * x[k] - x[j] * z[k]) * vort[k]) * area[k] / denom;
+            a2 = a2 + ((x[j] * y[k] - y[j] * x[k]) * vort[k]) * area[k] / denom;
+        }
+        dx[j] = a0;
+        dy[j] = a1;
+        dz[j] = a2;
+    }
+    const double c = -4.0 * BVO_PI;
+    for (int j = 0; j < n; ++j) {
+        dx[j] /= c;
+        dy[j] /= c;
+        dz[j] /= c;
+        dvort[j] = -(2.0 * omega * dz[j]);
+    }
+}
+
+/* One RK4 step of BVERK4, PanelVelocityParallel.f90:435-658, on one rank.
+ *
+ * In/out: active x (3*na), active relvort (na); passive x (3*npp);
+ * particles x (3*np), particle relvort (np) recomputed from absvort after
+ * the step (:651).  area (na) is fixed for the step (:461).
+ * Out: ke (na) = |u|^2 at stage 1 (:495-497), *total_ke = 1/2 sum ke*area
+ * (:498).  Stage inputs x + a_s k_{s-1} (:458-462, :513-516, :555-558,
+ * :597-600), k_s = dt*u_s (:503-506), update x + k1/6 + k2/3 + k3/3 + k4/6
+ * evaluated left to right (:637-644). */
+void bvo_rk4_step(int na, double *ax, double *avort, const double *area,
+                  int npp, double *px, int np, double *qx, double *qrel,
+                  const double *qabs, double dt, double omega, double smooth,
+                  double *ke, double *total_ke)
+{
+    const size_t a3 = 3 * (size_t)na, p3 = 3 * (size_t)npp, q3 = 3 * (size_t)np;
+    double *ain = malloc(sizeof(double) * a3), *vin = malloc(sizeof(double) * na);
+    double *pin = malloc(sizeof(double) * p3), *qin = malloc(sizeof(double) * q3);
+    double *ka[4], *kv[4], *kp[4], *kq[4];
+    for (int s = 0; s < 4; ++s) {
+        ka[s] = malloc(sizeof(double) * a3);
+        kv[s] = malloc(sizeof(double) * na);
+        kp[s] = malloc(sizeof(double) * p3);
+        kq[s] = malloc(sizeof(double) * q3);
+    }
+    static const double stage_a[4] = {0.0, 0.5, 0.5, 1.0};
+    for (int s = 0; s < 4; ++s) {
+        if (s == 0) {
+            memcpy(ain, ax, sizeof(double) * a3);
+            memcpy(vin, avort, sizeof(double) * na);
+            memcpy(pin, px, sizeof(double) * p3);
+            memcpy(qin, qx, sizeof(double) * q3);
+        } else if (s < 3) {
+            for (size_t i = 0; i < a3; ++i) ain[i] = ax[i] + stage_a[s] * ka[s - 1][i];
+            for (int i = 0; i < na; ++i) vin[i] = avort[i] + stage_a[s] * kv[s - 1][i];
+            for (size_t i = 0; i < p3; ++i) pin[i] = px[i] + stage_a[s] * kp[s - 1][i];
+            for (size_t i = 0; i < q3; ++i) qin[i] = qx[i] + stage_a[s] * kq[s - 1][i];
+        } else {
+            for (size_t i = 0; i < a3; ++i) ain[i] = ax[i] + ka[2][i];
+            for (int i = 0; i < na; ++i) vin[i] = avort[i] + kv[2][i];
+            for (size_t i = 0; i < p3; ++i) pin[i] = px[i] + kp[2][i];
+            for (size_t i = 0; i < q3; ++i) qin[i] = qx[i] + kq[2][i];
+        }
+        bvo_active_rhs(na, ain, vin, area, 0, na, omega, ka[s], kv[s]);
+        bvo_passive_rhs(np, qin, na, ain, vin, area, 0, np, kq[s]);
+        bvo_smooth_velocity(npp, pin, na, ain, vin, area, 0, npp, smooth, kp[s]);
+        if (s == 0) {
+            double tot = 0.0;
+            for (int j = 0; j < na; ++j) {
+                const double *u = ka[0] + 3 * j;
+                double e = (u[0] * u[0] + u[1] * u[1]) + u[2] * u[2];
+                if (ke) ke[j] = e;
+                tot = tot + e * area[j];
+            }
+            if (total_ke) *total_ke = 0.5 * tot;
+        }
+        for (size_t i = 0; i < a3; ++i) ka[s][i] = dt * ka[s][i];
+        for (size_t i = 0; i < p3; ++i) kp[s][i] = dt * kp[s][i];
+        for (size_t i = 0; i < q3; ++i) kq[s][i] = dt * kq[s][i];
+        for (int i = 0; i < na; ++i) kv[s][i] = dt * kv[s][i];
+    }
+    for (size_t i = 0; i < q3; ++i)
+        qx[i] = qx[i] + kq[0][i] / 6.0 + kq[1][i] / 3.0 + kq[2][i] / 3.0 + kq[3][i] / 6.0;
+    for (size_t i = 0; i < a3; ++i)
+        ax[i] = ax[i] + ka[0][i] / 6.0 + ka[1][i] / 3.0 + ka[2][i] / 3.0 + ka[3][i] / 6.0;
+    for (size_t i = 0; i < p3; ++i)
+        px[i] = px[i] + kp[0][i] / 6.0 + kp[1][i] / 3.0 + kp[2][i] / 3.0 + kp[3][i] / 6.0;
+    for (int i = 0; i < na; ++i)
+        avort[i] = avort[i] + kv[0][i] / 6.0 + kv[1][i] / 3.0 + kv[2][i] / 3.0 + kv[3][i] / 6.0;
+    if (qrel && qabs)
+        for (int i = 0; i < np; ++i) qrel[i] = qabs[i] - 2.0 * omega * qx[3 * i + 2];
+    free(ain); free(vin); free(pin); free(qin);
+    for (int s = 0; s < 4; ++s) { free(ka[s]); free(kv[s]); free(kp[s]); free(kq[s]); }
+}
+
+/* RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702: x / sqrt(x.x). */
+void bvo_renormalize(int n, double *x)
+{
+    for (int j = 0; j < n; ++j) {
+        double *p = x + 3 * j;
+        const double r = sqrt((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]);
+        p[0] /= r; p[1] /= r; p[2] /= r;
+    }
+}
+
+/* High-precision brute force of the same sum (not the reference's rounding):
+ * every quantity in __float128.  Targets X (m), sources x (n); self index
+ * for target j is self_off + j when self_off >= 0 (active block), and the
+ * denominator is (1 - x.X) + eps2.  Output in float64 after the final
+ * division by -4 PI.  For checking float64 kernels on small inputs. */
+void bvo_velocity_q(int m, const double *X, int n, const double *x, const double *w,
+                    int self_off, double eps2, double *u)
+{
+    const __float128 pi_q = 3.14159265358979323846264338327950288Q;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int j = 0; j < m; ++j) {
+        const __float128 X0 = X[3 * j], X1 = X[3 * j + 1], X2 = X[3 * j + 2];
+        __float128 a0 = 0, a1 = 0, a2 = 0;
+        for (int k = 0; k < n; ++k) {
+            if (self_off >= 0 && k == self_off + j) continue;
+            const __float128 xk0 = x[3 * k], xk1 = x[3 * k + 1], xk2 = x[3 * k + 2];
+            const __float128 d = (1 - (xk0 * X0 + xk1 * X1 + xk2 * X2)) + (__float128)eps2;
+            const __float128 s = (__float128)w[k] / d;
+            a0 += s * (X1 * xk2 - X2 * xk1);
+            a1 += s * (X2 * xk0 - X0 * xk2);
+            a2 += s * (X0 * xk1 - X1 * xk0);
+        }
+        u[3 * j] = (double)(a0 / (-4 * pi_q));
+        u[3 * j + 1] = (double)(a1 / (-4 * pi_q));
+        u[3 * j + 2] = (double)(a2 / (-4 * pi_q));
+    }
+}
+
+/* Threads the OpenMP loops above use (1 when built without OpenMP). */
+int bvo_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void bvo_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -1,0 +1,249 @@
+"""GPU parity: libbve (HIP, gfx950) against the oracle and the reference's
+golden vectors.  Tolerance for float64: 1e-12 normwise relative
+(max_i |u_i - u_i^ref| / max_i |u_i^ref|) on velocities; positions after a
+step, which move by dt*u, are compared with the same bound relative to
+the displacement."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden_manifest, load_golden, normwise_rel
+from thesiscode_amd import _lib
+from thesiscode_amd import grid as gridmod
+from thesiscode_amd import initial_conditions as ic
+from thesiscode_amd.panel_velocity import PanelVelocity, Particles, Panels
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+CASES = sorted(golden_manifest())
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = _lib.Context(0)
+    yield c
+    c.close()
+
+
+def golden_state(name):
+    g = load_golden(name)
+    return g, dict(ax=g["s0_active_x"], av=g["s0_active_relvort"], ar=g["s0_active_area"],
+                   px=g["s0_passive_x"], qx=g["s0_particles_x"], qa=g["s0_particles_absvort"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_single_evaluation_matches_oracle(ctx, name, manifest):
+    """bve_velocity for the three target classes vs BVEActiveRHS,
+    BVEPassiveRHS and BVESmoothVelocity on the golden initial states."""
+    c = manifest[name]
+    print("case", name, flush=True)
+    g, s = golden_state(name)
+    ctx.set_smooth(c["smooth"])
+    ctx.set_omega(c["omega"])
+    na, npp = len(s["av"]), len(s["px"])
+    ref_a, ref_dv = oracle.active_rhs(s["ax"], s["av"], s["ar"], c["omega"])
+    ref_p = oracle.passive_rhs(s["qx"], s["ax"], s["av"], s["ar"])
+    ref_s = oracle.smooth_velocity(s["px"], s["ax"], s["av"], s["ar"], c["smooth"])
+    # all three classes in one launch: [active | passive | particles]
+    t = np.concatenate([s["ax"], s["px"], s["qx"]])
+    u, dv = ctx.velocity(t, s["ax"], s["av"], s["ar"], self_end=na, smooth_range=(na, na + npp),
+                         want_dvort=True)
+    assert normwise_rel(u[:na], ref_a) < TOL
+    assert normwise_rel(u[na:na + npp], ref_s) < TOL
+    assert normwise_rel(u[na + npp:], ref_p) < TOL
+    assert np.abs(dv - ref_dv).max() <= TOL * np.abs(ref_dv).max()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_rk4_steps_match_reference(ctx, name, manifest):
+    """bve_load_panels + bve_rk4_step + bve_store_panels (the BVERK4 drop-in)
+    against the compiled reference's states after step 1 and the last step."""
+    c = manifest[name]
+    print("case", name, flush=True)
+    g, s = golden_state(name)
+    ctx.reset()
+    ctx.set_smooth(c["smooth"])
+    ctx.set_omega(c["omega"])
+    ctx.load_panels(s["ax"], s["av"], s["ar"], s["px"])
+    qx = s["qx"]
+    kes = []
+    for step in range(c["nsteps"]):
+        qx, rel, tot, ke = ctx.rk4_step(c["dt"], qx, s["qa"])
+        kes.append(tot)
+        if step == 0:
+            ax, av, px = ctx.store_panels()
+            disp = np.linalg.norm(g["s1_active_x"] - s["ax"], axis=1).max()
+            assert np.abs(ax - g["s1_active_x"]).max() < TOL * disp
+            assert np.abs(px - g["s1_passive_x"]).max() < TOL * disp
+            assert np.abs(qx - g["s1_particles_x"]).max() < TOL * disp
+            dzeta = np.abs(g["s1_active_relvort"] - s["av"]).max()
+            assert np.abs(av - g["s1_active_relvort"]).max() < TOL * max(dzeta, 1e-300)
+            assert np.abs(rel - g["s1_particles_relvort"]).max() < 1e-13 * np.abs(g["s1_particles_relvort"]).max()
+            assert np.abs(ke - g["ke_step1"]).max() < 2 * TOL * np.abs(g["ke_step1"]).max()
+    assert np.allclose(kes, g["total_ke"], rtol=2 * TOL, atol=0)
+    if c["nsteps"] > 1:
+        ax, av, px = ctx.store_panels()
+        disp = np.linalg.norm(g["sn_active_x"] - s["ax"], axis=1).max()
+        assert np.abs(ax - g["sn_active_x"]).max() < 10 * TOL * disp
+        assert np.abs(qx - g["sn_particles_x"]).max() < 10 * TOL * disp
+        assert np.abs(px - g["sn_passive_x"]).max() < 10 * TOL * disp
+
+
+def test_dropin_mirror_matches_reference():
+    """PanelVelocity.BVERK4 (the module mirror, gather/scatter on the host)
+    driven like RHWavePanelsMPI.f90:247-320 on the golden 12-step case."""
+    name = "rh4_L4_12steps"
+    c = golden_manifest()[name]
+    g, s = golden_state(name)
+    na, npp = len(s["av"]), len(s["px"])
+    # tree order: passive (divided) panels first, then the leaves
+    panels = Panels(x=np.concatenate([s["px"], s["ax"]]), area=np.concatenate([np.zeros(npp), s["ar"]]),
+                    hasChildren=np.concatenate([np.ones(npp, bool), np.zeros(na, bool)]),
+                    relVort=np.concatenate([np.zeros(npp), s["av"]]), absVort=np.zeros(na + npp),
+                    tracer=np.zeros((na + npp, 3)))
+    parts = Particles(x=s["qx"].copy(), relVort=g["s0_particles_relvort"].copy(), absVort=s["qa"])
+    pv = PanelVelocity(omega=c["omega"], smooth=c["smooth"])
+    pv.InitializeMPIRK4(parts, panels, 0, 1)
+    tots = []
+    for _ in range(c["nsteps"]):
+        pv.BVERK4(parts, panels, c["dt"], 0, 1)
+        tots.append(pv.totalKE)
+    pv.ResetRK4()
+    pv.close()
+    disp = np.linalg.norm(g["sn_active_x"] - s["ax"], axis=1).max()
+    assert np.abs(panels.x[npp:] - g["sn_active_x"]).max() < 10 * TOL * disp
+    assert np.abs(panels.x[:npp] - g["sn_passive_x"]).max() < 10 * TOL * disp
+    assert np.abs(parts.x - g["sn_particles_x"]).max() < 10 * TOL * disp
+    assert np.allclose(tots, g["total_ke"], rtol=2 * TOL, atol=0)
+    assert np.allclose(panels.tracer[npp:, 1], pv.kineticEnergy)
+
+
+def test_reruns_are_bitwise_reproducible(ctx):
+    """Fixed-order chunk reduction: the same step twice gives identical bits."""
+    g = gridmod.cubed_sphere(5)
+    z = ic.rh4_wave(g.active_x)
+    outs = []
+    for _ in range(2):
+        ctx.reset()
+        ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+        qx, _, tot, ke = ctx.rk4_step(0.01, g.particles_x)
+        outs.append((qx, ctx.store_panels(), tot, ke))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert all(np.array_equal(a, b) for a, b in zip(outs[0][1], outs[1][1]))
+    assert outs[0][2] == outs[1][2] and np.array_equal(outs[0][3], outs[1][3])
+
+
+@pytest.mark.parametrize("plan", [(1, 256, 0), (2, 512, 64), (4, 256, 4096), (4, 512, 0)])
+def test_launch_plans_agree(ctx, plan):
+    """Every supported launch geometry gives the oracle's velocities."""
+    g = gridmod.cubed_sphere(4)
+    z = ic.rh4_wave(g.active_x)
+    ref, _ = oracle.active_rhs(g.active_x, z, g.active_area, 0.0)
+    ctx.set_plan(*plan)
+    try:
+        u = ctx.velocity(g.active_x, g.active_x, z, g.active_area, self_end=g.n_active)
+    finally:
+        ctx.set_plan(0, 0, 0)
+    assert normwise_rel(u, ref) < TOL
+
+
+def test_level7_invariants(ctx):
+    """Full benchmark size (98304 active panels): tangency u.x = 0 and the
+    active-block momentum sum_i w_i u_i = 0, which no subset of a wrong sum
+    satisfies; plus a sampled comparison with the oracle."""
+    g = gridmod.cubed_sphere(7)
+    za, _, _ = ic.gauss_vortex(g)
+    x, a = g.active_x, g.active_area
+    u = ctx.velocity(x, x, za, a, self_end=g.n_active)
+    umax = np.linalg.norm(u, axis=1).max()
+    assert np.abs(np.sum(u * x, axis=1)).max() < 1e-13 * umax
+    w = za * a
+    assert np.abs((w[:, None] * u).sum(axis=0)).max() < 1e-12 * umax * np.abs(w).sum()
+    # sampled targets against the reference-order oracle (self excluded by index)
+    idx = np.random.default_rng(7).choice(g.n_active, 96, replace=False)
+    sub_ref = np.zeros((len(idx), 3))
+    for k, i in enumerate(idx):
+        m = np.ones(g.n_active, bool)
+        m[i] = False
+        sub_ref[k] = oracle.passive_rhs(x[i:i + 1], x[m], za[m], a[m])[0]
+    assert normwise_rel(u[idx], sub_ref) < TOL
+
+
+def test_voronoi_step_matches_oracle(ctx):
+    """bve_voronoi_rk4_step: the Voronoi BVERK4 with NormalizeGrid, on a
+    generator set (cubed-sphere centres) vs the oracle's RK4 + renormalise."""
+    g = gridmod.cubed_sphere(4)
+    z = ic.rh4_wave(g.active_x)
+    x = g.active_x
+    r = oracle.rk4_step(x, z, g.active_area, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0),
+                        0.01, 2 * np.pi, 0.05)
+    ref = oracle.renormalize(r["active_x"])
+    ctx.set_omega(2 * np.pi)
+    vx, vy, vz, vr, tot, ke = ctx.voronoi_rk4_step(0.01, x[:, 0], x[:, 1], x[:, 2], z, g.active_area)
+    got = np.stack([vx, vy, vz], 1)
+    disp = np.linalg.norm(r["active_x"] - x, axis=1).max()
+    assert np.abs(got - ref).max() < TOL * disp
+    assert np.abs(vr - r["active_relvort"]).max() < TOL * np.abs(r["active_relvort"] - z).max()
+    assert tot == pytest.approx(r["total_ke"], rel=2 * TOL)
+
+
+def test_resident_advance_matches_stepwise(ctx):
+    """bve_advance over n steps (state resident in HBM) equals n calls of the
+    host-transfer step."""
+    g = gridmod.cubed_sphere(4)
+    z = ic.rh4_wave(g.active_x)
+    ctx.reset()
+    ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+    qx = g.particles_x
+    for _ in range(3):
+        qx, _, _, _ = ctx.rk4_step(0.01, qx)
+    a1 = ctx.store_panels()
+    ctx.reset()
+    ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+    ctx.load_particles(g.particles_x)
+    ctx.advance(0.01, 3)
+    qx2, _ = ctx.store_particles()
+    a2 = ctx.store_panels()
+    assert np.array_equal(qx, qx2)
+    assert all(np.array_equal(u, v) for u, v in zip(a1, a2))
+
+
+def test_renormalize_option(ctx):
+    g = gridmod.cubed_sphere(3)
+    z = ic.rh4_wave(g.active_x)
+    ctx.reset()
+    ctx.set_renormalize(True)
+    try:
+        ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+        qx, _, _, _ = ctx.rk4_step(0.05, g.particles_x)
+        ax, _, px = ctx.store_panels()
+    finally:
+        ctx.set_renormalize(False)
+    for p in (qx, ax, px):
+        assert np.abs(np.linalg.norm(p, axis=1) - 1).max() < 4e-16
+
+
+def test_small_and_empty_classes(ctx):
+    """The level-0 cube (6 panels, no passive panels, no particles) and a
+    single-panel problem run and match the oracle."""
+    x = np.array([[1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float)
+    w = np.array([0.3, 1.7, -0.4, 0.9, 2.2, -1.1])
+    ctx.reset()
+    ctx.load_panels(x, w, np.ones(6), np.zeros((0, 3)))
+    ctx.rk4_step(0.01, np.zeros((0, 3)))
+    ax, av, _ = ctx.store_panels()
+    r = oracle.rk4_step(x, w, np.ones(6), np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), 0.01,
+                        2 * np.pi, 0.05)
+    assert np.abs(ax - r["active_x"]).max() < 1e-15
+    u = ctx.velocity(x[:1], x[:1], w[:1], np.ones(1), self_end=1)
+    assert not u.any()
+
+
+def test_errors_are_reported(ctx):
+    ctx.reset()
+    with pytest.raises(_lib.BVEError, match="no panels loaded"):
+        ctx.rk4_step(0.01, np.zeros((3, 3)))
+    with pytest.raises(_lib.BVEError, match="unsupported plan"):
+        ctx.set_plan(3, 256, 0)
+    with pytest.raises(_lib.BVEError, match="bad target classes"):
+        ctx.velocity(np.zeros((4, 3)), np.zeros((2, 3)), np.ones(2), np.ones(2), self_end=3)

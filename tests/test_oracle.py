@@ -1,0 +1,165 @@
+"""The CPU oracle pinned against the reference's golden vectors and against
+what the mathematics fixes (closed forms, invariants, float128 brute force)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden_manifest, load_golden, normwise_rel
+from thesiscode_amd import grid as gridmod
+from thesiscode_amd import initial_conditions as ic
+
+CASES = sorted(golden_manifest())
+
+
+def run_oracle_case(name, c):
+    g = load_golden(name)
+    st = dict(ax=g["s0_active_x"], av=g["s0_active_relvort"], ar=g["s0_active_area"],
+              px=g["s0_passive_x"], qx=g["s0_particles_x"], qa=g["s0_particles_absvort"])
+    out = []
+    for _ in range(c["nsteps"]):
+        r = oracle.rk4_step(st["ax"], st["av"], st["ar"], st["px"], st["qx"], st["qa"],
+                            c["dt"], c["omega"], c["smooth"])
+        st.update(ax=r["active_x"], av=r["active_relvort"], px=r["passive_x"], qx=r["particles_x"])
+        out.append(r)
+    return g, out
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_reference_bitwise(name, manifest):
+    """bvo_rk4_step restates BVERK4 in the reference's rounding order: the
+    compiled reference's outputs are reproduced bit for bit."""
+    c = manifest[name]
+    g, out = run_oracle_case(name, c)
+    r1 = out[0]
+    for key, gk in [("active_x", "s1_active_x"), ("active_relvort", "s1_active_relvort"),
+                    ("passive_x", "s1_passive_x"), ("particles_x", "s1_particles_x"),
+                    ("particles_relvort", "s1_particles_relvort"), ("ke", "ke_step1")]:
+        assert np.array_equal(r1[key], g[gk]), (name, key)
+    assert np.array_equal([r["total_ke"] for r in out], g["total_ke"])
+    if c["nsteps"] > 1:
+        rn = out[-1]
+        for key in ("active_x", "active_relvort", "passive_x", "particles_x"):
+            assert np.array_equal(rn[key], g["sn_" + key]), (name, key)
+
+
+def test_oracle_detects_planted_mistake():
+    """A dropped term (one source skipped) must break the golden comparison:
+    the golden check is sensitive at the level of a single pair."""
+    g = load_golden("rh4ref_L4_1step")
+    x, v, a = g["s0_active_x"], g["s0_active_relvort"], g["s0_active_area"]
+    dx, _ = oracle.active_rhs(x, v, a, 2 * np.pi)
+    v2 = v.copy()
+    v2[17] = 0.0                     # plant: drop source 17
+    dx2, _ = oracle.active_rhs(x, v2, a, 2 * np.pi)
+    assert normwise_rel(dx2, dx) > 1e-6
+
+
+def test_level0_cube_closed_form():
+    """Six face centres +-e_k: u(e_1) = -(1/4pi) [(w_{+2} - w_{-2}) e_3 + (w_{-3} - w_{+3}) e_2],
+    derived by hand from e_1 x e_2 = e_3, e_1 x e_3 = -e_2, d = 1 for orthogonal
+    pairs and a zero cross product for the antipode."""
+    x = np.array([[1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float)
+    w = np.array([0.3, 1.7, -0.4, 0.9, 2.2, -1.1])
+    u, _ = oracle.active_rhs(x, w, np.ones(6), 0.0)
+    wp2, wm2, wp3, wm3 = w[1], w[3], w[4], w[5]
+    expect = -1.0 / (4 * np.pi) * np.array([0.0, wm3 - wp3, wp2 - wm2])
+    assert np.allclose(u[0], expect, rtol=0, atol=1e-15)
+    uq = oracle.velocity_q(x, x, w, self_off=0)
+    assert np.allclose(uq[0], expect, rtol=0, atol=1e-16)
+
+
+@pytest.fixture(scope="module")
+def l4_state():
+    g = gridmod.cubed_sphere(4)
+    z = ic.rh4_wave(g.active_x)
+    return g, z
+
+
+def test_oracle_vs_float128(l4_state):
+    """Reference-order float64 vs a float128 evaluation of the same sum."""
+    g, z = l4_state
+    x, a = g.active_x, g.active_area
+    u, _ = oracle.active_rhs(x, z, a, 0.0)
+    uq = oracle.velocity_q(x, x, z * a, self_off=0)
+    assert normwise_rel(u, uq) < 1e-13
+    up = oracle.passive_rhs(g.particles_x, x, z, a)
+    upq = oracle.velocity_q(g.particles_x, x, z * a)
+    assert normwise_rel(up, upq) < 1e-13
+    us = oracle.smooth_velocity(g.passive_x, x, z, a, 0.05)
+    usq = oracle.velocity_q(g.passive_x, x, z * a, eps2=0.05 ** 2)
+    assert normwise_rel(us, usq) < 1e-13
+
+
+def test_tangency_and_momentum(l4_state):
+    """u_i . x_i = 0 (x_i x x_j is orthogonal to x_i), and in the active block
+    sum_i w_i u_i = 0: the pair terms w_i w_j (x_i x x_j)/d_ij are antisymmetric."""
+    g, z = l4_state
+    x, a = g.active_x, g.active_area
+    u, _ = oracle.active_rhs(x, z, a, 0.0)
+    umax = np.abs(u).max()
+    assert np.abs(np.sum(u * x, axis=1)).max() < 1e-14 * umax
+    w = z * a
+    assert np.abs((w[:, None] * u).sum(axis=0)).max() < 1e-13 * umax * np.abs(w).sum()
+
+
+def test_voronoi_sum_equals_active_sum(l4_state):
+    """BVEVelocity (SoA, sequential-subtraction denominator) computes the
+    same sum as BVEActiveRHS up to rounding."""
+    g, z = l4_state
+    x, a = g.active_x, g.active_area
+    u, dv = oracle.active_rhs(x, z, a, 2 * np.pi)
+    vx, vy, vz, vdv = oracle.voronoi_velocity(x[:, 0], x[:, 1], x[:, 2], z, a, 2 * np.pi)
+    assert normwise_rel(np.stack([vx, vy, vz], 1), u) < 1e-14
+    assert np.allclose(vdv, dv, rtol=0, atol=1e-14 * np.abs(dv).max())
+
+
+def test_index_range_is_a_shard(l4_state):
+    """A target sub-range [i0, i1) (what LoadBalance gives one rank) equals
+    the same rows of the full evaluation, and leaves the rest zero."""
+    g, z = l4_state
+    x, a = g.active_x, g.active_area
+    full, _ = oracle.active_rhs(x, z, a, 0.0)
+    part, _ = oracle.active_rhs(x, z, a, 0.0, 300, 700)
+    assert np.array_equal(part[300:700], full[300:700])
+    assert not part[:300].any() and not part[700:].any()
+
+
+def test_absolute_vorticity_is_carried(l4_state):
+    """RK4 with dzeta = -2 Omega u_z keeps zeta + 2 Omega z constant for every
+    active panel (a linear invariant of the scheme) up to rounding."""
+    g, z = l4_state
+    omega = 2 * np.pi
+    r = oracle.rk4_step(g.active_x, z, g.active_area, g.passive_x, g.particles_x,
+                        ic.rh4_wave(g.particles_x) + 2 * omega * g.particles_x[:, 2], 0.01, omega, 0.05)
+    before = z + 2 * omega * g.active_x[:, 2]
+    after = r["active_relvort"] + 2 * omega * r["active_x"][:, 2]
+    assert np.abs(after - before).max() < 1e-13
+
+
+def test_solid_body_rotation_converges():
+    """zeta = 2 w z with Omega = 0 is solid-body rotation about z: after time t
+    every point is rotated by w t (the pattern of solidBody.f90:298-327).  The
+    max-norm error of the midpoint rule with the singular kernel at least
+    halves per level (h halves per level)."""
+    wrot, dt, nsteps = 1.0, 0.05, 4
+    errs = []
+    for level in (2, 3, 4):
+        g = gridmod.cubed_sphere(level)
+        x = g.active_x
+        st = dict(ax=x, av=2 * wrot * x[:, 2])
+        for _ in range(nsteps):
+            r = oracle.rk4_step(st["ax"], st["av"], g.active_area, g.passive_x[:0], g.particles_x[:0],
+                                np.zeros(0), dt, 0.0, 0.05)
+            st = dict(ax=r["active_x"], av=r["active_relvort"])
+        th = wrot * dt * nsteps
+        rot = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+        exact = x @ rot.T
+        errs.append(np.abs(st["ax"] - exact).max())
+    assert errs[0] / errs[1] > 2.0 and errs[1] / errs[2] > 2.0 and errs[2] < 1e-3, errs
+
+
+def test_renormalize():
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(1000, 3))
+    y = oracle.renormalize(x)
+    assert np.abs(np.linalg.norm(y, axis=1) - 1).max() < 3e-16

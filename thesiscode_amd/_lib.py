@@ -1,0 +1,238 @@
+"""ctypes binding of libbve.so (the HIP library) and a thin numpy wrapper.
+
+The library must have been built (thesiscode_amd.build.build()); loading
+fails loudly if it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbve.so")
+
+BVE_OK = 0
+BVE_PRECISION_FP64 = 0
+BVE_PRECISION_MIXED = 1
+
+# every symbol include/bve.h declares, with its ctypes signature
+_d, _i, _p, _ll = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "bve_create": (_i, [ctypes.POINTER(_p), _i]),
+    "bve_destroy": (_i, [_p]),
+    "bve_last_error": (ctypes.c_char_p, [_p]),
+    "bve_set_omega": (_i, [_p, _d]),
+    "bve_set_smooth": (_i, [_p, _d]),
+    "bve_get_smooth": (_i, [_p, _dp]),
+    "bve_set_precision": (_i, [_p, _i, _d]),
+    "bve_set_renormalize": (_i, [_p, _i]),
+    "bve_set_plan": (_i, [_p, _i, _i, _i]),
+    "bve_load_panels": (_i, [_p, _i, _p, _p, _p, _i, _p]),
+    "bve_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
+    "bve_store_panels": (_i, [_p, _p, _p, _p]),
+    "bve_reset": (_i, [_p]),
+    "bve_load_particles": (_i, [_p, _i, _p, _p]),
+    "bve_advance": (_i, [_p, _d, _i]),
+    "bve_store_particles": (_i, [_p, _p, _p]),
+    "bve_get_kinetic_energy": (_i, [_p, _p, _p]),
+    "bve_synchronize": (_i, [_p]),
+    "bve_velocity": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p]),
+    "bve_voronoi_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p, _p, _p]),
+    "bve_set_profiling": (_i, [_p, _i]),
+    "bve_get_profile": (_i, [_p, ctypes.POINTER(_ll), _dp]),
+    "bve_pairs_per_step": (_i, [_p, _dp]),
+}
+
+_lib = None
+
+
+class BVEError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libbve.so and declare its signatures (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BVEError(f"{LIB_PATH} is not built: run thesiscode_amd.build.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+class Context:
+    """One libbve context (device state of one solver instance)."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = _p()
+        rc = L.bve_create(ctypes.byref(h), device)
+        if rc != BVE_OK:
+            raise BVEError(f"bve_create failed ({rc}): {L.bve_last_error(None).decode()}")
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != BVE_OK:
+            raise BVEError(f"{what} failed ({rc}): {self._L.bve_last_error(self._h).decode()}")
+
+    # parameters
+    def set_omega(self, omega):
+        self._check(self._L.bve_set_omega(self._h, omega), "bve_set_omega")
+
+    def set_smooth(self, smooth):
+        self._check(self._L.bve_set_smooth(self._h, smooth), "bve_set_smooth")
+
+    def get_smooth(self):
+        v = ctypes.c_double()
+        self._check(self._L.bve_get_smooth(self._h, ctypes.byref(v)), "bve_get_smooth")
+        return v.value
+
+    def set_precision(self, precision, nearfield_radius=0.02):
+        self._check(self._L.bve_set_precision(self._h, precision, nearfield_radius),
+                    "bve_set_precision")
+
+    def set_renormalize(self, on):
+        self._check(self._L.bve_set_renormalize(self._h, int(bool(on))), "bve_set_renormalize")
+
+    def set_plan(self, targets_per_lane=0, tile=0, target_workgroups=0):
+        self._check(self._L.bve_set_plan(self._h, targets_per_lane, tile, target_workgroups),
+                    "bve_set_plan")
+
+    # panel path
+    def load_panels(self, active_x, active_relvort, active_area, passive_x):
+        na = len(active_relvort)
+        npp = len(passive_x)
+        ax = _f64(active_x, 3 * na)
+        av = _f64(active_relvort, na)
+        ar = _f64(active_area, na)
+        px = _f64(passive_x, 3 * npp) if npp else None
+        self._check(self._L.bve_load_panels(self._h, na, _ptr(ax), _ptr(av), _ptr(ar), npp, _ptr(px)),
+                    "bve_load_panels")
+        self.n_active, self.n_passive = na, npp
+
+    def rk4_step(self, dt, particles_x, absvort=None, want_ke=True):
+        """One step; returns (new particle x, particle relvort or None, totalKE, ke)."""
+        n = len(particles_x)
+        qx = _f64(particles_x, 3 * n).copy()
+        qa = _f64(absvort, n) if absvort is not None else None
+        rel = np.zeros(n) if absvort is not None else None
+        tot = np.zeros(1)
+        ke = np.zeros(self.n_active) if want_ke else None
+        self._check(self._L.bve_rk4_step(self._h, dt, n, _ptr(qx), _ptr(qa), _ptr(rel), _ptr(tot),
+                                         _ptr(ke)), "bve_rk4_step")
+        return qx.reshape(n, 3), rel, float(tot[0]), ke
+
+    def store_panels(self):
+        ax = np.zeros((self.n_active, 3))
+        av = np.zeros(self.n_active)
+        px = np.zeros((self.n_passive, 3))
+        self._check(self._L.bve_store_panels(self._h, _ptr(ax), _ptr(av), _ptr(px)),
+                    "bve_store_panels")
+        return ax, av, px
+
+    def reset(self):
+        self._check(self._L.bve_reset(self._h), "bve_reset")
+
+    # resident stepping
+    def load_particles(self, particles_x, absvort=None):
+        n = len(particles_x)
+        qx = _f64(particles_x, 3 * n)
+        qa = _f64(absvort, n) if absvort is not None else None
+        self._check(self._L.bve_load_particles(self._h, n, _ptr(qx), _ptr(qa)), "bve_load_particles")
+        self.n_particles = n
+
+    def advance(self, dt, n_steps):
+        self._check(self._L.bve_advance(self._h, dt, n_steps), "bve_advance")
+
+    def store_particles(self, want_relvort=False):
+        qx = np.zeros((self.n_particles, 3))
+        rel = np.zeros(self.n_particles) if want_relvort else None
+        self._check(self._L.bve_store_particles(self._h, _ptr(qx), _ptr(rel)), "bve_store_particles")
+        return qx, rel
+
+    def kinetic_energy(self):
+        tot = np.zeros(1)
+        ke = np.zeros(self.n_active)
+        self._check(self._L.bve_get_kinetic_energy(self._h, _ptr(tot), _ptr(ke)),
+                    "bve_get_kinetic_energy")
+        return float(tot[0]), ke
+
+    def synchronize(self):
+        self._check(self._L.bve_synchronize(self._h), "bve_synchronize")
+
+    # single evaluation
+    def velocity(self, targets_x, sources_x, vort, area, self_end=0, smooth_range=(0, 0),
+                 want_dvort=False):
+        nt, ns = len(targets_x), len(vort)
+        tx = _f64(targets_x, 3 * nt)
+        sx = _f64(sources_x, 3 * ns)
+        v = _f64(vort, ns)
+        a = _f64(area, ns)
+        u = np.zeros((nt, 3))
+        dv = np.zeros(self_end) if want_dvort else None
+        self._check(self._L.bve_velocity(self._h, nt, _ptr(tx), self_end, smooth_range[0],
+                                         smooth_range[1], ns, _ptr(sx), _ptr(v), _ptr(a), _ptr(u),
+                                         _ptr(dv)), "bve_velocity")
+        return (u, dv) if want_dvort else u
+
+    # Voronoi path
+    def voronoi_rk4_step(self, dt, x, y, z, relvort, area):
+        n = len(x)
+        xs = [_f64(a, n).copy() for a in (x, y, z, relvort)]
+        ar = _f64(area, n)
+        tot = np.zeros(1)
+        ke = np.zeros(n)
+        self._check(self._L.bve_voronoi_rk4_step(self._h, dt, n, *(_ptr(a) for a in xs), _ptr(ar),
+                                                 _ptr(tot), _ptr(ke)), "bve_voronoi_rk4_step")
+        return xs[0], xs[1], xs[2], xs[3], float(tot[0]), ke
+
+    # measurement
+    def set_profiling(self, on):
+        self._check(self._L.bve_set_profiling(self._h, int(bool(on))), "bve_set_profiling")
+
+    def profile(self):
+        n = ctypes.c_longlong()
+        ms = ctypes.c_double()
+        self._check(self._L.bve_get_profile(self._h, ctypes.byref(n), ctypes.byref(ms)),
+                    "bve_get_profile")
+        return n.value, ms.value
+
+    def pairs_per_step(self):
+        v = ctypes.c_double()
+        self._check(self._L.bve_pairs_per_step(self._h, ctypes.byref(v)), "bve_pairs_per_step")
+        return v.value

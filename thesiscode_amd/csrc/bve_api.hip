@@ -1,0 +1,771 @@
+// C ABI of libbve: contexts, device state of the panel and Voronoi RK4
+// paths, single velocity evaluations and launch-level profiling.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bve_common.h"
+#include "bve_kernels.h"
+#include "../../include/bve.h"
+
+using namespace bve;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <class P>
+hipError_t dev_alloc(P **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void **)p, count * sizeof(P));
+}
+
+template <class P>
+void dev_free(P *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct bve_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+
+    // parameters
+    double omega = 2.0 * kPi;
+    double smooth = 0.05;
+    int precision = BVE_PRECISION_FP64;
+    double nearfield = 0.02;
+    int renormalize = 0;
+    int plan_t = 0, plan_tile = 0, plan_wg = 0;
+
+    // problem sizes
+    bool panels_loaded = false;
+    int n_active = 0, n_passive = 0, n_particles = 0;
+    int n_targets = 0, t_stride = 0, n_src_pad = 0;
+
+    // device state (targets [active | passive | particles], SoA rows)
+    double *x0 = nullptr, *xs = nullptr, *acc = nullptr;
+    double *zeta0 = nullptr, *zacc = nullptr, *area = nullptr;
+    double *absvort = nullptr, *relvort = nullptr;
+    Source *src = nullptr;
+    double *partial = nullptr;
+    size_t partial_cap = 0;   // doubles
+    double *ke = nullptr, *ke_part = nullptr, *ke_total = nullptr;
+    int ke_part_cap = 0;
+    double *staging = nullptr;   // AoS staging for uploads/downloads
+    size_t staging_cap = 0;      // doubles
+    bool particles_have_absvort = false;
+    PairPlan plan{};
+
+    // profiling
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, event_pool;
+    long long prof_launches = 0;
+    double prof_ms = 0.0;
+
+    int fail(int code, const char *fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        error = buf;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char *what)
+    {
+        return fail(e == hipErrorOutOfMemory ? BVE_ERR_MEMORY : BVE_ERR_DEVICE, "%s: %s", what,
+                    hipGetErrorString(e));
+    }
+};
+
+#define HIP_TRY(ctx, call)                                       \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return (ctx)->hip_fail(e_, #call); \
+    } while (0)
+
+namespace {
+
+int choose_targets_per_lane(int n_targets)
+{
+    if (n_targets >= 65536) return 4;
+    if (n_targets >= 16384) return 2;
+    return 1;
+}
+
+PairPlan make_plan(int n_targets, int n_src_pad, int T, int tile, int target_wg)
+{
+    PairPlan p{};
+    p.targets_per_lane = T;
+    p.tile = tile;
+    p.n_tiles = std::max(1, ceil_div(n_targets, kBlock * T));
+    const int want_chunks = std::max(1, ceil_div(target_wg, p.n_tiles));
+    p.chunk = round_up(std::max(tile, ceil_div(n_src_pad, want_chunks)), tile);
+    p.n_chunks = std::max(1, ceil_div(n_src_pad, p.chunk));
+    p.n_work = p.n_tiles * p.n_chunks;
+    p.grid = round_up(p.n_work, 8);
+    return p;
+}
+
+int ensure_staging(bve_ctx *c, size_t doubles)
+{
+    if (doubles <= c->staging_cap) return BVE_OK;
+    dev_free(c->staging);
+    size_t cap = std::max(doubles, (size_t)4096);
+    HIP_TRY(c, dev_alloc(&c->staging, cap));
+    c->staging_cap = cap;
+    return BVE_OK;
+}
+
+int ensure_partial(bve_ctx *c, size_t doubles)
+{
+    if (doubles <= c->partial_cap) return BVE_OK;
+    dev_free(c->partial);
+    HIP_TRY(c, dev_alloc(&c->partial, doubles));
+    c->partial_cap = doubles;
+    return BVE_OK;
+}
+
+void free_state(bve_ctx *c)
+{
+    dev_free(c->x0);
+    dev_free(c->xs);
+    dev_free(c->acc);
+    dev_free(c->zeta0);
+    dev_free(c->zacc);
+    dev_free(c->area);
+    dev_free(c->absvort);
+    dev_free(c->relvort);
+    dev_free(c->src);
+    dev_free(c->ke);
+    dev_free(c->ke_part);
+    dev_free(c->ke_total);
+    c->ke_part_cap = 0;
+    c->panels_loaded = false;
+    c->n_active = c->n_passive = c->n_particles = 0;
+    c->n_targets = c->t_stride = c->n_src_pad = 0;
+    c->particles_have_absvort = false;
+}
+
+int update_plan(bve_ctx *c)
+{
+    const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_targets);
+    const int tile = c->plan_tile ? c->plan_tile : 256;
+    const int wg = c->plan_wg ? c->plan_wg : 2048;
+    c->plan = make_plan(c->n_targets, c->n_src_pad, T, tile, wg);
+    if (c->plan.n_tiles * kBlock * T > c->t_stride)
+        return c->fail(BVE_ERR_STATE, "internal: target tiles exceed the padded target arrays");
+    return ensure_partial(c, (size_t)c->plan.n_chunks * 3 * c->t_stride);
+}
+
+// Resizes the target arrays for n_particles particles, keeping the panel part.
+int ensure_particles(bve_ctx *c, int n_particles)
+{
+    if (n_particles == c->n_particles && c->x0) return BVE_OK;
+    const int n_panel = c->n_active + c->n_passive;
+    const int nt = n_panel + n_particles;
+    const int stride = round_up(std::max(nt, 1), kTargetAlign);
+    if (stride != c->t_stride) {
+        double *x0 = nullptr, *xs = nullptr, *acc = nullptr;
+        HIP_TRY(c, dev_alloc(&x0, 3 * (size_t)stride));
+        HIP_TRY(c, dev_alloc(&xs, 3 * (size_t)stride));
+        HIP_TRY(c, dev_alloc(&acc, 3 * (size_t)stride));
+        HIP_TRY(c, hipMemsetAsync(x0, 0, 3 * sizeof(double) * stride, c->stream));
+        HIP_TRY(c, hipMemsetAsync(xs, 0, 3 * sizeof(double) * stride, c->stream));
+        HIP_TRY(c, hipMemsetAsync(acc, 0, 3 * sizeof(double) * stride, c->stream));
+        if (c->x0 && n_panel > 0)
+            for (int r = 0; r < 3; ++r)
+                HIP_TRY(c, hipMemcpyAsync(x0 + (size_t)r * stride, c->x0 + (size_t)r * c->t_stride,
+                                          sizeof(double) * n_panel, hipMemcpyDeviceToDevice,
+                                          c->stream));
+        dev_free(c->x0);
+        dev_free(c->xs);
+        dev_free(c->acc);
+        c->x0 = x0;
+        c->xs = xs;
+        c->acc = acc;
+        c->t_stride = stride;
+    }
+    if (n_particles != c->n_particles || !c->absvort) {
+        dev_free(c->absvort);
+        dev_free(c->relvort);
+        HIP_TRY(c, dev_alloc(&c->absvort, std::max(n_particles, 1)));
+        HIP_TRY(c, dev_alloc(&c->relvort, std::max(n_particles, 1)));
+    }
+    c->n_particles = n_particles;
+    c->n_targets = nt;
+    const int need = ke_partial_count(nt);
+    if (need > c->ke_part_cap) {
+        dev_free(c->ke_part);
+        HIP_TRY(c, dev_alloc(&c->ke_part, need));
+        c->ke_part_cap = need;
+    }
+    return update_plan(c);
+}
+
+StepState step_state(bve_ctx *c)
+{
+    StepState st{};
+    st.n_targets = c->n_targets;
+    st.t_stride = c->t_stride;
+    st.n_active = c->n_active;
+    st.particle_begin = c->n_active + c->n_passive;
+    st.particle_end = c->n_targets;
+    st.x0 = c->x0;
+    st.xs = c->xs;
+    st.acc = c->acc;
+    st.zeta0 = c->zeta0;
+    st.zacc = c->zacc;
+    st.area = c->area;
+    st.src = c->src;
+    st.partial = c->partial;
+    st.ke = c->ke;
+    st.ke_part = c->ke_part;
+    st.ke_total = c->ke_total;
+    st.particle_absvort = c->particles_have_absvort ? c->absvort : nullptr;
+    st.particle_relvort = c->relvort;
+    return st;
+}
+
+std::pair<hipEvent_t, hipEvent_t> take_events(bve_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        auto ev = c->event_pool.back();
+        c->event_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    return {a, b};
+}
+
+// Adds the durations of completed profiled launches (call after a sync).
+void collect_profile(bve_ctx *c)
+{
+    for (auto &ev : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+            c->prof_ms += ms;
+            c->prof_launches += 1;
+        }
+        c->event_pool.push_back(ev);
+    }
+    c->pending.clear();
+}
+
+int pair_pass(bve_ctx *c, const PairArgs &a)
+{
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (c->profiling) {
+        ev = take_events(c);
+        HIP_TRY(c, hipEventRecord(ev.first, c->stream));
+    }
+    HIP_TRY(c, launch_pair_sum_f64(a, c->stream));
+    if (c->profiling) {
+        HIP_TRY(c, hipEventRecord(ev.second, c->stream));
+        c->pending.push_back(ev);
+    }
+    return BVE_OK;
+}
+
+int run_steps(bve_ctx *c, double dt, int n_steps)
+{
+    StepState st = step_state(c);
+    PairArgs pa{};
+    pa.tx = c->xs;
+    pa.ty = c->xs + c->t_stride;
+    pa.tz = c->xs + 2 * (size_t)c->t_stride;
+    pa.t_stride = c->t_stride;
+    pa.cls.self_end = c->n_active;
+    pa.cls.smooth_begin = c->n_active;
+    pa.cls.smooth_end = c->n_active + c->n_passive;
+    pa.cls.eps2 = c->smooth * c->smooth;
+    pa.src = c->src;
+    pa.n_src_pad = c->n_src_pad;
+    pa.plan = c->plan;
+    pa.partial = c->partial;
+    static const double a_next[4] = {0.5, 0.5, 1.0, 0.0};
+    static const double divisor[4] = {6.0, 3.0, 3.0, 6.0};
+    HIP_TRY(c, launch_prepare_step(st, c->stream));
+    for (int n = 0; n < n_steps; ++n) {
+        for (int s = 0; s < 4; ++s) {
+            int rc = pair_pass(c, pa);
+            if (rc) return rc;
+            StageParams sp{};
+            sp.stage = s;
+            sp.n_chunks = c->plan.n_chunks;
+            sp.dt = dt;
+            sp.omega = c->omega;
+            sp.a_next = a_next[s];
+            sp.divisor = divisor[s];
+            sp.renormalize = c->renormalize;
+            HIP_TRY(c, launch_stage_update(st, sp, c->stream));
+        }
+    }
+    return BVE_OK;
+}
+
+int upload_aos(bve_ctx *c, const double *host, int n, double *dst_rows, int stride, int off)
+{
+    if (n <= 0) return BVE_OK;
+    int rc = ensure_staging(c, 3 * (size_t)n);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(c->staging, host, 3 * sizeof(double) * n, hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, launch_aos_to_soa(c->staging, n, dst_rows, stride, off, c->stream));
+    return BVE_OK;
+}
+
+int download_aos(bve_ctx *c, const double *src_rows, int stride, int off, int n, double *host)
+{
+    if (n <= 0 || !host) return BVE_OK;
+    int rc = ensure_staging(c, 3 * (size_t)n);
+    if (rc) return rc;
+    HIP_TRY(c, launch_soa_to_aos(src_rows, stride, off, n, c->staging, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(host, c->staging, 3 * sizeof(double) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BVE_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+int bve_create(bve_ctx **out, int device)
+{
+    if (!out) {
+        g_create_error = "bve_create: null output pointer";
+        return BVE_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        g_create_error = std::string("bve_create: no HIP device: ") + hipGetErrorString(e);
+        return BVE_ERR_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_error = "bve_create: device index out of range";
+        return BVE_ERR_ARGUMENT;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        g_create_error = std::string("bve_create: hipSetDevice: ") + hipGetErrorString(e);
+        return BVE_ERR_DEVICE;
+    }
+    bve_ctx *c = new bve_ctx;
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_error = std::string("bve_create: hipStreamCreate: ") + hipGetErrorString(e);
+        delete c;
+        return BVE_ERR_DEVICE;
+    }
+    *out = c;
+    return BVE_OK;
+}
+
+int bve_destroy(bve_ctx *c)
+{
+    if (!c) return BVE_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_state(c);
+    dev_free(c->partial);
+    dev_free(c->staging);
+    for (auto &ev : c->pending) c->event_pool.push_back(ev);
+    for (auto &ev : c->event_pool) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BVE_OK;
+}
+
+const char *bve_last_error(const bve_ctx *c)
+{
+    return c ? c->error.c_str() : g_create_error.c_str();
+}
+
+int bve_set_omega(bve_ctx *c, double omega)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    c->omega = omega;
+    return BVE_OK;
+}
+
+int bve_set_smooth(bve_ctx *c, double smooth)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    c->smooth = smooth;
+    return BVE_OK;
+}
+
+int bve_get_smooth(const bve_ctx *c, double *smooth)
+{
+    if (!c || !smooth) return BVE_ERR_ARGUMENT;
+    *smooth = c->smooth;
+    return BVE_OK;
+}
+
+int bve_set_precision(bve_ctx *c, int precision, double nearfield_radius)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (precision != BVE_PRECISION_FP64 && precision != BVE_PRECISION_MIXED)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: unknown precision %d", precision);
+    if (precision == BVE_PRECISION_MIXED)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: mixed precision is not built yet");
+    if (!(nearfield_radius >= 0.0))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: nearfield_radius must be >= 0");
+    c->precision = precision;
+    c->nearfield = nearfield_radius;
+    return BVE_OK;
+}
+
+int bve_set_renormalize(bve_ctx *c, int on)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    c->renormalize = on ? 1 : 0;
+    return BVE_OK;
+}
+
+int bve_set_plan(bve_ctx *c, int targets_per_lane, int tile, int target_workgroups)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    const int T = targets_per_lane ? targets_per_lane : 1;
+    const int TL = tile ? tile : 256;
+    if (!plan_supported(T, TL) || target_workgroups < 0)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_set_plan: unsupported plan (T=%d, tile=%d, wg=%d)",
+                       targets_per_lane, tile, target_workgroups);
+    c->plan_t = targets_per_lane;
+    c->plan_tile = tile;
+    c->plan_wg = target_workgroups;
+    if (c->panels_loaded && c->x0) {
+        HIP_TRY(c, hipSetDevice(c->device));
+        return update_plan(c);
+    }
+    return BVE_OK;
+}
+
+int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const double *relvort,
+                    const double *area, int n_passive, const double *x_passive)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (n_active < 1 || n_passive < 0 || !x_active || !relvort || !area ||
+        (n_passive > 0 && !x_passive))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_load_panels: bad sizes or null arrays "
+                       "(n_active=%d, n_passive=%d)", n_active, n_passive);
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_state(c);
+    c->n_active = n_active;
+    c->n_passive = n_passive;
+    c->n_src_pad = round_up(n_active, kSourceAlign);
+    HIP_TRY(c, dev_alloc(&c->zeta0, n_active));
+    HIP_TRY(c, dev_alloc(&c->zacc, n_active));
+    HIP_TRY(c, dev_alloc(&c->area, n_active));
+    HIP_TRY(c, dev_alloc(&c->ke, n_active));
+    HIP_TRY(c, dev_alloc(&c->ke_total, 1));
+    HIP_TRY(c, dev_alloc(&c->src, c->n_src_pad));
+    HIP_TRY(c, hipMemsetAsync(c->src, 0, sizeof(Source) * c->n_src_pad, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->zeta0, relvort, sizeof(double) * n_active,
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->area, area, sizeof(double) * n_active, hipMemcpyHostToDevice,
+                              c->stream));
+    c->n_particles = -1;   // forces the target arrays to be built
+    int rc = ensure_particles(c, 0);
+    if (rc) return rc;
+    rc = upload_aos(c, x_active, n_active, c->x0, c->t_stride, 0);
+    if (rc) return rc;
+    rc = upload_aos(c, x_passive, n_passive, c->x0, c->t_stride, n_active);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->panels_loaded = true;
+    return BVE_OK;
+}
+
+int bve_load_particles(bve_ctx *c, int n_particles, const double *x, const double *absvort)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_load_particles: no panels loaded");
+    if (n_particles < 0 || (n_particles > 0 && !x))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_load_particles: bad arguments (n=%d)", n_particles);
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = ensure_particles(c, n_particles);
+    if (rc) return rc;
+    const int off = c->n_active + c->n_passive;
+    rc = upload_aos(c, x, n_particles, c->x0, c->t_stride, off);
+    if (rc) return rc;
+    c->particles_have_absvort = absvort != nullptr;
+    if (absvort && n_particles > 0)
+        HIP_TRY(c, hipMemcpyAsync(c->absvort, absvort, sizeof(double) * n_particles,
+                                  hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BVE_OK;
+}
+
+int bve_advance(bve_ctx *c, double dt, int n_steps)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_advance: no panels loaded");
+    if (n_steps < 0) return c->fail(BVE_ERR_ARGUMENT, "bve_advance: n_steps < 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    return run_steps(c, dt, n_steps);
+}
+
+int bve_store_particles(bve_ctx *c, double *x, double *relvort)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_store_particles: no panels loaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int off = c->n_active + c->n_passive;
+    int rc = download_aos(c, c->x0, c->t_stride, off, c->n_particles, x);
+    if (rc) return rc;
+    if (relvort && c->n_particles > 0) {
+        if (!c->particles_have_absvort)
+            return c->fail(BVE_ERR_STATE, "bve_store_particles: particles were loaded without absvort");
+        HIP_TRY(c, hipMemcpyAsync(relvort, c->relvort, sizeof(double) * c->n_particles,
+                                  hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    return BVE_OK;
+}
+
+int bve_get_kinetic_energy(bve_ctx *c, double *total_ke, double *ke)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_get_kinetic_energy: no panels loaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (total_ke)
+        HIP_TRY(c, hipMemcpyAsync(total_ke, c->ke_total, sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+    if (ke)
+        HIP_TRY(c, hipMemcpyAsync(ke, c->ke, sizeof(double) * c->n_active, hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    return BVE_OK;
+}
+
+int bve_synchronize(bve_ctx *c)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    return BVE_OK;
+}
+
+int bve_rk4_step(bve_ctx *c, double dt, int n_particles, double *x_particles,
+                 const double *absvort, double *relvort, double *total_ke, double *ke)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_rk4_step: no panels loaded");
+    if (n_particles < 0 || (n_particles > 0 && !x_particles))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_rk4_step: bad particle arguments (n=%d)", n_particles);
+    if (relvort && !absvort)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_rk4_step: relvort requested without absvort");
+    int rc = bve_load_particles(c, n_particles, x_particles, absvort);
+    if (rc) return rc;
+    rc = run_steps(c, dt, 1);
+    if (rc) return rc;
+    rc = bve_store_particles(c, x_particles, relvort);
+    if (rc) return rc;
+    return bve_get_kinetic_energy(c, total_ke, ke);
+}
+
+int bve_store_panels(bve_ctx *c, double *x_active, double *relvort, double *x_passive)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_store_panels: no panels loaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = download_aos(c, c->x0, c->t_stride, 0, c->n_active, x_active);
+    if (rc) return rc;
+    rc = download_aos(c, c->x0, c->t_stride, c->n_active, c->n_passive, x_passive);
+    if (rc) return rc;
+    if (relvort)
+        HIP_TRY(c, hipMemcpyAsync(relvort, c->zeta0, sizeof(double) * c->n_active,
+                                  hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BVE_OK;
+}
+
+int bve_reset(bve_ctx *c)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    free_state(c);
+    return BVE_OK;
+}
+
+int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_end,
+                 int smooth_begin, int smooth_end, int n_sources, const double *x_sources,
+                 const double *vort, const double *area, double *u, double *dvort)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (n_targets < 0 || n_sources < 0 || (n_targets > 0 && (!x_targets || !u)) ||
+        (n_sources > 0 && (!x_sources || !vort || !area)))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_velocity: bad sizes or null arrays");
+    if (self_end < 0 || self_end > n_targets || self_end > n_sources || smooth_begin < 0 ||
+        smooth_end < smooth_begin || smooth_end > n_targets)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_velocity: bad target classes (self_end=%d, "
+                       "smooth=[%d,%d), n_targets=%d, n_sources=%d)", self_end, smooth_begin,
+                       smooth_end, n_targets, n_sources);
+    if (n_targets == 0) return BVE_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int stride = round_up(n_targets, kTargetAlign);
+    const int ns_pad = round_up(std::max(n_sources, 1), kSourceAlign);
+    double *t = nullptr, *xs = nullptr, *vv = nullptr, *aa = nullptr, *uu = nullptr, *dv = nullptr;
+    Source *src = nullptr;
+    int rc = BVE_OK;
+    hipError_t e = hipSuccess;
+    auto cleanup = [&]() {
+        dev_free(t); dev_free(xs); dev_free(vv); dev_free(aa); dev_free(uu); dev_free(dv);
+        dev_free(src);
+    };
+#define VTRY(call)                                       \
+    do {                                                 \
+        e = (call);                                      \
+        if (e != hipSuccess) {                           \
+            rc = c->hip_fail(e, #call);                  \
+            (void)hipStreamSynchronize(c->stream);       \
+            cleanup();                                   \
+            return rc;                                   \
+        }                                                \
+    } while (0)
+    VTRY(dev_alloc(&t, 3 * (size_t)stride));
+    VTRY(dev_alloc(&src, ns_pad));
+    VTRY(dev_alloc(&uu, 3 * (size_t)n_targets));
+    VTRY(hipMemsetAsync(t, 0, 3 * sizeof(double) * stride, c->stream));
+    VTRY(hipMemsetAsync(src, 0, sizeof(Source) * ns_pad, c->stream));
+    if (n_sources > 0) {
+        VTRY(dev_alloc(&xs, 3 * (size_t)n_sources));
+        VTRY(dev_alloc(&vv, n_sources));
+        VTRY(dev_alloc(&aa, n_sources));
+        VTRY(hipMemcpyAsync(xs, x_sources, 3 * sizeof(double) * n_sources, hipMemcpyHostToDevice, c->stream));
+        VTRY(hipMemcpyAsync(vv, vort, sizeof(double) * n_sources, hipMemcpyHostToDevice, c->stream));
+        VTRY(hipMemcpyAsync(aa, area, sizeof(double) * n_sources, hipMemcpyHostToDevice, c->stream));
+        VTRY(launch_make_sources(xs, vv, aa, n_sources, src, c->stream));
+    }
+    rc = ensure_staging(c, 3 * (size_t)n_targets);
+    if (rc) { cleanup(); return rc; }
+    VTRY(hipMemcpyAsync(c->staging, x_targets, 3 * sizeof(double) * n_targets, hipMemcpyHostToDevice, c->stream));
+    VTRY(launch_aos_to_soa(c->staging, n_targets, t, stride, 0, c->stream));
+    const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(n_targets);
+    const int tile = c->plan_tile ? c->plan_tile : 256;
+    const PairPlan plan = make_plan(n_targets, ns_pad, T, tile, c->plan_wg ? c->plan_wg : 2048);
+    rc = ensure_partial(c, (size_t)plan.n_chunks * 3 * stride);
+    if (rc) { cleanup(); return rc; }
+    PairArgs pa{};
+    pa.tx = t;
+    pa.ty = t + stride;
+    pa.tz = t + 2 * (size_t)stride;
+    pa.t_stride = stride;
+    pa.cls.self_end = self_end;
+    pa.cls.smooth_begin = smooth_begin;
+    pa.cls.smooth_end = smooth_end;
+    pa.cls.eps2 = c->smooth * c->smooth;
+    pa.src = src;
+    pa.n_src_pad = ns_pad;
+    pa.plan = plan;
+    pa.partial = c->partial;
+    rc = pair_pass(c, pa);
+    if (rc) { (void)hipStreamSynchronize(c->stream); cleanup(); return rc; }
+    if (dvort && self_end > 0) VTRY(dev_alloc(&dv, self_end));
+    VTRY(launch_finalize_velocity(c->partial, plan.n_chunks, t, stride, n_targets, c->omega, uu,
+                                  dv, self_end, c->stream));
+    VTRY(hipMemcpyAsync(u, uu, 3 * sizeof(double) * n_targets, hipMemcpyDeviceToHost, c->stream));
+    if (dv) VTRY(hipMemcpyAsync(dvort, dv, sizeof(double) * self_end, hipMemcpyDeviceToHost, c->stream));
+    VTRY(hipStreamSynchronize(c->stream));
+#undef VTRY
+    collect_profile(c);
+    cleanup();
+    return BVE_OK;
+}
+
+int bve_voronoi_rk4_step(bve_ctx *c, double dt, int n, double *x, double *y, double *z,
+                         double *relvort, const double *area, double *total_ke, double *ke)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (n < 1 || !x || !y || !z || !relvort || !area)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_voronoi_rk4_step: bad sizes or null arrays (n=%d)", n);
+    std::vector<double> aos(3 * (size_t)n);
+    for (int j = 0; j < n; ++j) {
+        aos[3 * (size_t)j] = x[j];
+        aos[3 * (size_t)j + 1] = y[j];
+        aos[3 * (size_t)j + 2] = z[j];
+    }
+    const int saved_renorm = c->renormalize;
+    int rc = bve_load_panels(c, n, aos.data(), relvort, area, 0, nullptr);
+    if (rc) return rc;
+    rc = ensure_particles(c, 0);
+    if (rc) return rc;
+    c->renormalize = 1;   // NormalizeGrid after every Voronoi step (VoronoiVelocityParallel.f90:480)
+    rc = run_steps(c, dt, 1);
+    c->renormalize = saved_renorm;
+    if (rc) return rc;
+    rc = bve_store_panels(c, aos.data(), relvort, nullptr);
+    if (rc) return rc;
+    rc = bve_get_kinetic_energy(c, total_ke, ke);
+    if (rc) return rc;
+    for (int j = 0; j < n; ++j) {
+        x[j] = aos[3 * (size_t)j];
+        y[j] = aos[3 * (size_t)j + 1];
+        z[j] = aos[3 * (size_t)j + 2];
+    }
+    return bve_reset(c);
+}
+
+int bve_set_profiling(bve_ctx *c, int on)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    c->profiling = on != 0;
+    c->prof_launches = 0;
+    c->prof_ms = 0.0;
+    return BVE_OK;
+}
+
+int bve_get_profile(bve_ctx *c, long long *launches, double *total_ms)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    if (launches) *launches = c->prof_launches;
+    if (total_ms) *total_ms = c->prof_ms;
+    return BVE_OK;
+}
+
+int bve_pairs_per_step(const bve_ctx *c, double *pairs)
+{
+    if (!c || !pairs) return BVE_ERR_ARGUMENT;
+    const double na = c->n_active, other = (double)c->n_passive + (double)std::max(c->n_particles, 0);
+    *pairs = 4.0 * (na * (na - 1.0) + other * na);
+    return BVE_OK;
+}
+
+}  // extern "C"

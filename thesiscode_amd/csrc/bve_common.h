@@ -1,0 +1,47 @@
+// Shared definitions of the Biot-Savart / RK4 HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace bve {
+
+// PI of the reference's NumberKinds.f90:12, rounded to float64
+constexpr double kPi = 3.141592653589793238462643383279502797479068098137295573004504331874296718662975536062731407582759857177734375;
+
+constexpr int kBlock = 256;        // threads per workgroup of the pair kernels (4 wave64)
+constexpr int kTargetAlign = 1024; // target arrays are padded to this many entries
+constexpr int kSourceAlign = 512;  // source arrays are padded to this many entries
+
+// One source panel as the pair kernels read it: position and weight zeta*A.
+struct __align__(32) Source {
+    double x, y, z, w;
+};
+
+// Which targets a pair launch treats specially.  Targets are one array of
+// n_targets points; targets [0, self_end) exclude the source of the same
+// index (active panels, BVEActiveRHS), targets [smooth_begin, smooth_end)
+// use the denominator (1 - x.y) + eps2 (divided parent panels,
+// BVESmoothVelocity); all others use 1 - x.y (BVEPassiveRHS).
+struct TargetClasses {
+    int self_end;
+    int smooth_begin;
+    int smooth_end;
+    double eps2;
+};
+
+// Launch geometry of one pair-sum pass (fixed per context and size).
+struct PairPlan {
+    int targets_per_lane;   // 1, 2 or 4 targets held in registers per lane
+    int tile;               // sources per LDS tile
+    int n_tiles;            // target tiles of kBlock * targets_per_lane
+    int chunk;              // sources per chunk (multiple of tile)
+    int n_chunks;           // source chunks; partial sums per chunk
+    int n_work;             // n_tiles * n_chunks
+    int grid;               // launched workgroups (n_work rounded up to 8 XCDs)
+};
+
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace bve

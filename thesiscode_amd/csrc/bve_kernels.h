@@ -1,0 +1,66 @@
+// Host-side launchers of the Biot-Savart / RK4 kernels.
+#pragma once
+#include "bve_common.h"
+
+namespace bve {
+
+// One pair-sum pass: targets SoA (rows tx, ty, tz of length t_stride),
+// sources AoS records (n_src_pad, zero-weight padding), partial sums
+// partial[(chunk * 3 + c) * t_stride + i].
+struct PairArgs {
+    const double *tx, *ty, *tz;
+    int t_stride;
+    TargetClasses cls;
+    const Source *src;
+    int n_src_pad;
+    PairPlan plan;
+    double *partial;
+};
+
+// Device state of a panel RK4 step.  Targets are [active | passive | particles]
+// in SoA rows of length t_stride.
+struct StepState {
+    int n_targets, t_stride, n_active;
+    int particle_begin, particle_end;
+    double *x0;      // step-start positions (3 rows)
+    double *xs;      // current stage positions (3 rows)
+    double *acc;     // RK4 accumulator x + sum b_s k_s (3 rows)
+    double *zeta0;   // active relative vorticity at step start
+    double *zacc;    // its RK4 accumulator
+    const double *area;
+    Source *src;     // active source records of the current stage
+    const double *partial;
+    double *ke;      // stage-1 |u|^2 of active panels
+    double *ke_part; // per-block partial sums of ke * area
+    double *ke_total;
+    const double *particle_absvort;   // may be null
+    double *particle_relvort;
+};
+
+struct StageParams {
+    int stage;        // 0..3
+    int n_chunks;
+    double dt, omega;
+    double a_next;    // 0.5, 0.5, 1.0 (next stage input x + a k); unused at stage 3
+    double divisor;   // 6, 3, 3, 6
+    int renormalize;
+};
+
+bool plan_supported(int targets_per_lane, int tile);
+hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s);
+hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s);
+hipError_t launch_prepare_step(const StepState &st, hipStream_t s);
+int ke_partial_count(int n_targets);
+
+hipError_t launch_aos_to_soa(const double *aos, int n, double *dst, int stride, int off,
+                             hipStream_t s);
+hipError_t launch_soa_to_aos(const double *src, int stride, int off, int n, double *aos,
+                             hipStream_t s);
+hipError_t launch_make_sources(const double *xyz, const double *vort, const double *area, int n,
+                               Source *src, hipStream_t s);
+hipError_t launch_finalize_velocity(const double *partial, int n_chunks, const double *t,
+                                    int stride, int n, double omega, double *u, double *dvort,
+                                    int n_dvort, hipStream_t s);
+hipError_t launch_normalize_soa(double *t, int stride, int n, hipStream_t s);
+
+}  // namespace bve

@@ -8,6 +8,11 @@ the state resident in HBM (bve_advance); pairs are counted as
 4 (N_a (N_a - 1) + N_p N_a) per step -- active and vertex targets -- and the
 parent-panel targets are computed but not counted.
 
+Each arithmetic mode is timed separately; its normwise velocity error is
+measured against the oracle (float64 in the reference's rounding order) on
+64 active targets.  ``value`` is the pair rate of the fastest mode whose
+error is <= 1e-10.
+
 Prints one JSON line.  The roofline block uses HIP events recorded by the
 library around every pair-kernel launch (one launch per RK4 stage) at 20
 flops per pair against the MI355X float64 vector peak.  The CPU baseline
@@ -27,7 +32,11 @@ sys.path.insert(0, ROOT)
 
 FLOPS_PER_PAIR = 20.0
 PEAK_FP64_TFLOPS = 78.6      # MI355X float64 vector
+ERR_BAR = 1e-10
 METRIC = "Biot-Savart pair-interactions/s per RK4 step at <=1e-10 rel err; adjoint/fwd ratio"
+MODES = {"fp64": 0, "fp64_fast": 2}
+KERNEL_NAMES = {"fp64": "pair_sum_f64_kernel<T,TILE,R=1>",
+                "fp64_fast": "pair_sum_f64_kernel<T,TILE,R=4>"}
 
 
 def parse():
@@ -37,32 +46,65 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--level", type=int, default=7)
     p.add_argument("--dt", type=float, default=0.005)
+    p.add_argument("--modes", type=str, default="fp64,fp64_fast")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target duration of the CPU-baseline sample (0 = skip)")
     p.add_argument("--plan", type=str, default="",
-                   help="targets_per_lane,tile,workgroups for the pair kernel")
+                   help="targets_per_lane,tile,workgroups,recip for the pair kernel")
     return p.parse_args()
 
 
 def cpu_baseline(grid, zeta, seconds):
-    """Oracle BVEActiveRHS over the first M active targets (all sources)."""
+    """Oracle pair sums of whole RK4 stages (all three target classes) until
+    about ``seconds`` have passed; at least one stage."""
     import oracle
     x, a = grid.active_x, grid.active_area
     n = grid.n_active
     threads = oracle.num_threads()
-    m = max(64, threads * 8)
+    pairs_stage = n * (n - 1.0) + (grid.n_particles + grid.n_passive) * n
+    stages, el = 0, 0.0
     t0 = time.perf_counter()
-    oracle.active_rhs(x, zeta, a, 2 * np.pi, 0, m)
-    dt0 = time.perf_counter() - t0
-    m = int(min(n, max(m, m * seconds / max(dt0, 1e-3))))
+    while stages == 0 or el + el / stages <= 1.5 * seconds:
+        oracle.active_rhs(x, zeta, a, 2 * np.pi)
+        oracle.passive_rhs(grid.particles_x, x, zeta, a)
+        oracle.smooth_velocity(grid.passive_x, x, zeta, a, 0.05)
+        stages += 1
+        el = time.perf_counter() - t0
+    return dict(value=stages * pairs_stage / el, unit="pair-interactions/s", cores=threads,
+                kind="port",
+                sample=f"{stages} RK4 stage(s) of the oracle's BVEActiveRHS + BVEPassiveRHS + "
+                       f"BVESmoothVelocity (reference rounding order, IEEE divisions) at level "
+                       f"{grid.level}: {stages * pairs_stage:.4g} pairs in {el:.1f} s, "
+                       f"{threads} OpenMP threads")
+
+
+def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all):
+    import oracle
+    from thesiscode_amd import _lib
+    ctx.set_precision(MODES[mode])
+    ctx.advance(args.dt, args.warmup)
+    ctx.synchronize()
+    ctx.set_profiling(True)
     t0 = time.perf_counter()
-    oracle.active_rhs(x, zeta, a, 2 * np.pi, 0, m)
+    ctx.advance(args.dt, args.steps)
+    ctx.synchronize()
     el = time.perf_counter() - t0
-    pairs = m * (n - 1)
-    return dict(value=pairs / el, unit="pair-interactions/s", cores=threads, kind="port",
-                sample=f"oracle BVEActiveRHS (reference rounding order, IEEE divisions), "
-                       f"{m} of {n} active targets x {n - 1} sources at level {grid.level}, "
-                       f"{el:.1f} s, {threads} OpenMP threads")
+    launches, kernel_ms = ctx.profile()
+    ctx.set_profiling(False)
+    # accuracy of this mode: first 64 active targets vs the oracle
+    u_gpu = ctx.velocity(grid.active_x[:64], grid.active_x, za, grid.active_area, self_end=64)
+    u_ref, _ = oracle.active_rhs(grid.active_x, za, grid.active_area, 2 * np.pi, 0, 64)
+    u_ref = u_ref[:64]
+    err = float(np.max(np.linalg.norm(u_gpu - u_ref, axis=1)) /
+                np.max(np.linalg.norm(u_ref, axis=1)))
+    ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    avg_ms = kernel_ms / max(launches, 1)
+    achieved = pairs_all / 4.0 * FLOPS_PER_PAIR / (avg_ms * 1e-3) / 1e12
+    return dict(pair_rate=pairs_primary * args.steps / el,
+                pair_rate_all_classes=pairs_all * args.steps / el,
+                ms_per_step=el * 1e3 / max(args.steps, 1), rel_err_vs_fp64_oracle=err,
+                launches=launches, avg_launch_ms=avg_ms, achieved_tflops=achieved,
+                roofline_frac=achieved / PEAK_FP64_TFLOPS)
 
 
 def main():
@@ -93,38 +135,23 @@ def main():
     print(f"# level {args.level}: {na} active, {npart} particles, {npp} passive", file=sys.stderr,
           flush=True)
 
-    ctx.advance(args.dt, args.warmup)
-    ctx.synchronize()
-    ctx.set_profiling(True)
-    t0 = time.perf_counter()
-    ctx.advance(args.dt, args.steps)
-    ctx.synchronize()
-    el = time.perf_counter() - t0
-    launches, kernel_ms = ctx.profile()
-    ctx.set_profiling(False)
-    ms_per_step = el * 1e3 / max(args.steps, 1)
-    value = pairs_primary * args.steps / el
-
-    # accuracy of the measured path: first 64 active targets vs the oracle
-    import oracle
-    u_gpu = ctx.velocity(g.active_x[:64], g.active_x, za, g.active_area, self_end=64)
-    u_ref, _ = oracle.active_rhs(g.active_x, za, g.active_area, omega, 0, 64)
-    u_ref = u_ref[:64]
-    err = float(np.max(np.linalg.norm(u_gpu - u_ref, axis=1)) / np.max(np.linalg.norm(u_ref, axis=1)))
+    results = {}
+    for mode in [m for m in args.modes.split(",") if m]:
+        results[mode] = measure(ctx, g, za, mode, args, pairs_primary, pairs_all)
+        print(f"# {mode}: {results[mode]}", file=sys.stderr, flush=True)
     ctx.close()
 
-    avg_launch_ms = kernel_ms / max(launches, 1)
-    flops_per_launch = pairs_all / 4.0 * FLOPS_PER_PAIR
-    achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
-
+    ok = [m for m in results if results[m]["rel_err_vs_fp64_oracle"] <= ERR_BAR]
+    best = max(ok, key=lambda m: results[m]["pair_rate"]) if ok else None
+    r = results[best] if best else {}
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r.get("pair_rate"),
         "unit": "pair-interactions/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r.get("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -133,22 +160,21 @@ def main():
         "config": {
             "workload": f"gauss_vortex_cubed_sphere_L{args.level}_rk4_step",
             "level": args.level, "n_active": na, "n_particles": npart, "n_passive": npp,
-            "dt": args.dt, "precision": "fp64",
+            "dt": args.dt, "mode": best,
             "pairs_per_step": pairs_primary, "pairs_per_step_all_classes": pairs_all,
-            "pair_rate_all_classes": pairs_all * args.steps / el,
-            "rel_err_vs_fp64_oracle": err,
+            "modes": results,
         },
         "roofline": {
             "bound": "valu",
-            "achieved": achieved,
+            "achieved": r.get("achieved_tflops"),
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS,
+            "frac": r.get("roofline_frac"),
             "traffic": None,
-            "kernel": "pair_sum_f64_kernel",
-            "launches": launches,
-            "avg_launch_ms": avg_launch_ms,
-            "flops_per_launch": flops_per_launch,
+            "kernel": KERNEL_NAMES.get(best),
+            "launches": r.get("launches"),
+            "avg_launch_ms": r.get("avg_launch_ms"),
+            "flops_per_launch": pairs_all / 4.0 * FLOPS_PER_PAIR,
         },
     }
     if args.cpu_seconds > 0:

@@ -34,7 +34,11 @@ enum {
     BVE_ERR_MEMORY = 4    /* device allocation failed */
 };
 
-enum { BVE_PRECISION_FP64 = 0, BVE_PRECISION_MIXED = 1 };
+enum {
+    BVE_PRECISION_FP64 = 0,      /* float64 pairs, every 1/d refined to full precision */
+    BVE_PRECISION_MIXED = 1,     /* float32 far field, float64 near field and reductions */
+    BVE_PRECISION_FP64_FAST = 2  /* float64 pairs, one reciprocal shared by 2 or 4 pairs */
+};
 
 /* ---- lifetime -------------------------------------------------------- */
 
@@ -54,7 +58,8 @@ int bve_set_omega(bve_ctx *ctx, double omega);
  * Replaces SetSmooth / GetSmooth, PanelVelocityParallel.f90:96-105 (default 0.05, :27). */
 int bve_set_smooth(bve_ctx *ctx, double smooth);
 int bve_get_smooth(const bve_ctx *ctx, double *smooth);
-/* Arithmetic of the pair sum: BVE_PRECISION_FP64 (default) or
+/* Arithmetic of the pair sum: BVE_PRECISION_FP64 (default),
+ * BVE_PRECISION_FP64_FAST (float64, relative error per pair ~1e-15) or
  * BVE_PRECISION_MIXED (float32 far field, float64 for pairs closer than
  * nearfield_radius, float64 target reduction).  No reference counterpart. */
 int bve_set_precision(bve_ctx *ctx, int precision, double nearfield_radius);
@@ -63,9 +68,11 @@ int bve_set_precision(bve_ctx *ctx, int precision, double nearfield_radius);
  * device); 0 (default) = the reference's BVERK4, which never does. */
 int bve_set_renormalize(bve_ctx *ctx, int on);
 /* Launch geometry of the pair kernel: targets held per lane (1, 2, 4),
- * sources per LDS tile (256, 512) and the workgroup count to aim for; 0
+ * sources per LDS tile (256, 512), the workgroup count to aim for and, for
+ * BVE_PRECISION_FP64_FAST, the pairs sharing one reciprocal (2, 4); 0
  * keeps the built-in choice.  Used by the autotuner. */
-int bve_set_plan(bve_ctx *ctx, int targets_per_lane, int tile, int target_workgroups);
+int bve_set_plan(bve_ctx *ctx, int targets_per_lane, int tile, int target_workgroups,
+                 int recip);
 
 /* ---- panel RK4 path (drop-in for PanelVelocityParallelModule) --------- */
 

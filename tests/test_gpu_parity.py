@@ -133,18 +133,56 @@ def test_reruns_are_bitwise_reproducible(ctx):
     assert outs[0][2] == outs[1][2] and np.array_equal(outs[0][3], outs[1][3])
 
 
-@pytest.mark.parametrize("plan", [(1, 256, 0), (2, 512, 64), (4, 256, 4096), (4, 512, 0)])
-def test_launch_plans_agree(ctx, plan):
-    """Every supported launch geometry gives the oracle's velocities."""
+@pytest.mark.parametrize("precision,plan", [
+    (_lib.BVE_PRECISION_FP64, (1, 256, 0, 0)), (_lib.BVE_PRECISION_FP64, (2, 512, 64, 0)),
+    (_lib.BVE_PRECISION_FP64, (4, 256, 4096, 0)), (_lib.BVE_PRECISION_FP64, (4, 512, 0, 0)),
+    (_lib.BVE_PRECISION_FP64_FAST, (1, 256, 0, 2)), (_lib.BVE_PRECISION_FP64_FAST, (2, 512, 64, 4)),
+    (_lib.BVE_PRECISION_FP64_FAST, (4, 256, 4096, 4)), (_lib.BVE_PRECISION_FP64_FAST, (4, 512, 0, 2))])
+def test_launch_plans_agree(ctx, precision, plan):
+    """Every supported launch geometry and float64 reciprocal scheme gives
+    the oracle's velocities for all three target classes."""
     g = gridmod.cubed_sphere(4)
     z = ic.rh4_wave(g.active_x)
-    ref, _ = oracle.active_rhs(g.active_x, z, g.active_area, 0.0)
+    na, npp = g.n_active, g.n_passive
+    ref_a, _ = oracle.active_rhs(g.active_x, z, g.active_area, 0.0)
+    ref_s = oracle.smooth_velocity(g.passive_x, g.active_x, z, g.active_area, 0.05)
+    ref_p = oracle.passive_rhs(g.particles_x, g.active_x, z, g.active_area)
+    t = np.concatenate([g.active_x, g.passive_x, g.particles_x])
+    ctx.set_smooth(0.05)
+    ctx.set_precision(precision)
     ctx.set_plan(*plan)
     try:
-        u = ctx.velocity(g.active_x, g.active_x, z, g.active_area, self_end=g.n_active)
+        u = ctx.velocity(t, g.active_x, z, g.active_area, self_end=na, smooth_range=(na, na + npp))
     finally:
-        ctx.set_plan(0, 0, 0)
-    assert normwise_rel(u, ref) < TOL
+        ctx.set_plan(0, 0, 0, 0)
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    assert normwise_rel(u[:na], ref_a) < TOL
+    assert normwise_rel(u[na:na + npp], ref_s) < TOL
+    assert normwise_rel(u[na + npp:], ref_p) < TOL
+
+
+def test_fast_fp64_step_matches_reference(ctx):
+    """BVE_PRECISION_FP64_FAST through 12 RK4 steps against the compiled
+    reference (the golden 12-step case)."""
+    name = "rh4_L4_12steps"
+    c = golden_manifest()[name]
+    g, s = golden_state(name)
+    ctx.reset()
+    ctx.set_precision(_lib.BVE_PRECISION_FP64_FAST)
+    try:
+        ctx.set_omega(c["omega"])
+        ctx.set_smooth(c["smooth"])
+        ctx.load_panels(s["ax"], s["av"], s["ar"], s["px"])
+        qx = s["qx"]
+        for _ in range(c["nsteps"]):
+            qx, _, _, _ = ctx.rk4_step(c["dt"], qx, s["qa"])
+        ax, av, px = ctx.store_panels()
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    disp = np.linalg.norm(g["sn_active_x"] - s["ax"], axis=1).max()
+    assert np.abs(ax - g["sn_active_x"]).max() < 10 * TOL * disp
+    assert np.abs(qx - g["sn_particles_x"]).max() < 10 * TOL * disp
+    assert np.abs(px - g["sn_passive_x"]).max() < 10 * TOL * disp
 
 
 def test_level7_invariants(ctx):

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libbve.so")
 BVE_OK = 0
 BVE_PRECISION_FP64 = 0
 BVE_PRECISION_MIXED = 1
+BVE_PRECISION_FP64_FAST = 2
 
 # every symbol include/bve.h declares, with its ctypes signature
 _d, _i, _p, _ll = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong
@@ -27,7 +28,7 @@ SIGNATURES = {
     "bve_get_smooth": (_i, [_p, _dp]),
     "bve_set_precision": (_i, [_p, _i, _d]),
     "bve_set_renormalize": (_i, [_p, _i]),
-    "bve_set_plan": (_i, [_p, _i, _i, _i]),
+    "bve_set_plan": (_i, [_p, _i, _i, _i, _i]),
     "bve_load_panels": (_i, [_p, _i, _p, _p, _p, _i, _p]),
     "bve_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
     "bve_store_panels": (_i, [_p, _p, _p, _p]),
@@ -129,8 +130,8 @@ class Context:
     def set_renormalize(self, on):
         self._check(self._L.bve_set_renormalize(self._h, int(bool(on))), "bve_set_renormalize")
 
-    def set_plan(self, targets_per_lane=0, tile=0, target_workgroups=0):
-        self._check(self._L.bve_set_plan(self._h, targets_per_lane, tile, target_workgroups),
+    def set_plan(self, targets_per_lane=0, tile=0, target_workgroups=0, recip=0):
+        self._check(self._L.bve_set_plan(self._h, targets_per_lane, tile, target_workgroups, recip),
                     "bve_set_plan")
 
     # panel path

@@ -45,7 +45,7 @@ struct bve_ctx {
     int precision = BVE_PRECISION_FP64;
     double nearfield = 0.02;
     int renormalize = 0;
-    int plan_t = 0, plan_tile = 0, plan_wg = 0;
+    int plan_t = 0, plan_tile = 0, plan_wg = 0, plan_recip = 0;
 
     // problem sizes
     bool panels_loaded = false;
@@ -104,11 +104,12 @@ int choose_targets_per_lane(int n_targets)
     return 1;
 }
 
-PairPlan make_plan(int n_targets, int n_src_pad, int T, int tile, int target_wg)
+PairPlan make_plan(int n_targets, int n_src_pad, int T, int tile, int target_wg, int recip)
 {
     PairPlan p{};
     p.targets_per_lane = T;
     p.tile = tile;
+    p.recip = recip;
     p.n_tiles = std::max(1, ceil_div(n_targets, kBlock * T));
     const int want_chunks = std::max(1, ceil_div(target_wg, p.n_tiles));
     p.chunk = round_up(std::max(tile, ceil_div(n_src_pad, want_chunks)), tile);
@@ -158,12 +159,24 @@ void free_state(bve_ctx *c)
     c->particles_have_absvort = false;
 }
 
+int recip_for(const bve_ctx *c)
+{
+    if (c->precision != BVE_PRECISION_FP64_FAST) return 1;
+    return c->plan_recip ? c->plan_recip : 4;
+}
+
+PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
+{
+    const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(n_targets);
+    const int tile = c->plan_tile ? c->plan_tile : 256;
+    const int wg = c->plan_wg ? c->plan_wg : 4096;
+    return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c));
+}
+
 int update_plan(bve_ctx *c)
 {
-    const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_targets);
-    const int tile = c->plan_tile ? c->plan_tile : 256;
-    const int wg = c->plan_wg ? c->plan_wg : 2048;
-    c->plan = make_plan(c->n_targets, c->n_src_pad, T, tile, wg);
+    c->plan = plan_for(c, c->n_targets, c->n_src_pad);
+    const int T = c->plan.targets_per_lane;
     if (c->plan.n_tiles * kBlock * T > c->t_stride)
         return c->fail(BVE_ERR_STATE, "internal: target tiles exceed the padded target arrays");
     return ensure_partial(c, (size_t)c->plan.n_chunks * 3 * c->t_stride);
@@ -429,7 +442,8 @@ int bve_get_smooth(const bve_ctx *c, double *smooth)
 int bve_set_precision(bve_ctx *c, int precision, double nearfield_radius)
 {
     if (!c) return BVE_ERR_ARGUMENT;
-    if (precision != BVE_PRECISION_FP64 && precision != BVE_PRECISION_MIXED)
+    if (precision != BVE_PRECISION_FP64 && precision != BVE_PRECISION_MIXED &&
+        precision != BVE_PRECISION_FP64_FAST)
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: unknown precision %d", precision);
     if (precision == BVE_PRECISION_MIXED)
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: mixed precision is not built yet");
@@ -437,6 +451,10 @@ int bve_set_precision(bve_ctx *c, int precision, double nearfield_radius)
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: nearfield_radius must be >= 0");
     c->precision = precision;
     c->nearfield = nearfield_radius;
+    if (c->panels_loaded && c->x0) {
+        HIP_TRY(c, hipSetDevice(c->device));
+        return update_plan(c);
+    }
     return BVE_OK;
 }
 
@@ -447,17 +465,19 @@ int bve_set_renormalize(bve_ctx *c, int on)
     return BVE_OK;
 }
 
-int bve_set_plan(bve_ctx *c, int targets_per_lane, int tile, int target_workgroups)
+int bve_set_plan(bve_ctx *c, int targets_per_lane, int tile, int target_workgroups, int recip)
 {
     if (!c) return BVE_ERR_ARGUMENT;
     const int T = targets_per_lane ? targets_per_lane : 1;
     const int TL = tile ? tile : 256;
-    if (!plan_supported(T, TL) || target_workgroups < 0)
-        return c->fail(BVE_ERR_ARGUMENT, "bve_set_plan: unsupported plan (T=%d, tile=%d, wg=%d)",
-                       targets_per_lane, tile, target_workgroups);
+    if (!plan_supported(T, TL) || target_workgroups < 0 ||
+        !(recip == 0 || recip == 2 || recip == 4))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_set_plan: unsupported plan (T=%d, tile=%d, wg=%d, "
+                       "recip=%d)", targets_per_lane, tile, target_workgroups, recip);
     c->plan_t = targets_per_lane;
     c->plan_tile = tile;
     c->plan_wg = target_workgroups;
+    c->plan_recip = recip;
     if (c->panels_loaded && c->x0) {
         HIP_TRY(c, hipSetDevice(c->device));
         return update_plan(c);
@@ -672,9 +692,7 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
     if (rc) { cleanup(); return rc; }
     VTRY(hipMemcpyAsync(c->staging, x_targets, 3 * sizeof(double) * n_targets, hipMemcpyHostToDevice, c->stream));
     VTRY(launch_aos_to_soa(c->staging, n_targets, t, stride, 0, c->stream));
-    const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(n_targets);
-    const int tile = c->plan_tile ? c->plan_tile : 256;
-    const PairPlan plan = make_plan(n_targets, ns_pad, T, tile, c->plan_wg ? c->plan_wg : 2048);
+    const PairPlan plan = plan_for(c, n_targets, ns_pad);
     rc = ensure_partial(c, (size_t)plan.n_chunks * 3 * stride);
     if (rc) { cleanup(); return rc; }
     PairArgs pa{};

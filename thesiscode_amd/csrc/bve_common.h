@@ -34,6 +34,7 @@ struct TargetClasses {
 struct PairPlan {
     int targets_per_lane;   // 1, 2 or 4 targets held in registers per lane
     int tile;               // sources per LDS tile
+    int recip;              // sources sharing one float64 reciprocal: 1, 2 or 4
     int n_tiles;            // target tiles of kBlock * targets_per_lane
     int chunk;              // sources per chunk (multiple of tile)
     int n_chunks;           // source chunks; partial sums per chunk

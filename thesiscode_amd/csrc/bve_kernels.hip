@@ -26,7 +26,8 @@ namespace bve {
 // Pair kernel, float64
 // ---------------------------------------------------------------------------
 
-// 1/d: v_rcp_f64 seed refined by two Newton steps r <- r + r (1 - d r).
+// 1/d: v_rcp_f64 seed (accurate to about 2^-24) refined by two Newton
+// steps r <- r + r (1 - d r): correctly rounded to within an ulp or two.
 __device__ __forceinline__ double recip_f64(double d)
 {
     double r = __builtin_amdgcn_rcp(d);
@@ -36,26 +37,69 @@ __device__ __forceinline__ double recip_f64(double d)
     return __builtin_fma(r, e, r);
 }
 
-template <int T, int TILE, bool MASK>
+// 1/p with one Newton step: relative error about 2^-48.
+__device__ __forceinline__ double recip1_f64(double p)
+{
+    const double r = __builtin_amdgcn_rcp(p);
+    const double e = __builtin_fma(-p, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// One LDS tile of sources against the T targets of this lane.
+//
+// R = 1: every pair gets its own reciprocal (recip_f64).
+// R = 2, 4: R consecutive sources share one reciprocal of the product of
+// their denominators (1/d_a = d_b / (d_a d_b)), refined by one Newton step,
+// which trades R-1 v_rcp_f64 and Newton steps for 2(R-1) multiplies.
+// Denominators lie in [~1e-6, 2 + eps2], so products of four stay far from
+// float64 underflow and overflow.
+//
+// MASK: the tile holds the own-index source of some target of this wave;
+// that pair's denominator (~0) is replaced by 1 and its weight by 0.
+template <int T, int TILE, int R, bool MASK>
 __device__ __forceinline__ void tile_f64(const Source *__restrict__ lds, int jt,
                                          const double (&nx)[T], const double (&ny)[T],
                                          const double (&nz)[T], const double (&c0)[T],
                                          const int (&self)[T], double (&sx)[T],
                                          double (&sy)[T], double (&sz)[T])
 {
-#pragma unroll 2
-    for (int k = 0; k < TILE; ++k) {
-        const Source s = lds[k];
+#pragma unroll(R == 1 ? 2 : 1)
+    for (int k = 0; k < TILE; k += R) {
+        Source s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = lds[k + r];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-            double d = __builtin_fma(nx[t], s.x, c0[t]);
-            d = __builtin_fma(ny[t], s.y, d);
-            d = __builtin_fma(nz[t], s.z, d);
-            double q = s.w * recip_f64(d);
-            if (MASK) q = (jt + k == self[t]) ? 0.0 : q;
-            sx[t] = __builtin_fma(q, s.x, sx[t]);
-            sy[t] = __builtin_fma(q, s.y, sy[t]);
-            sz[t] = __builtin_fma(q, s.z, sz[t]);
+            double d[R], q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                d[r] = __builtin_fma(nx[t], s[r].x, c0[t]);
+                d[r] = __builtin_fma(ny[t], s[r].y, d[r]);
+                d[r] = __builtin_fma(nz[t], s[r].z, d[r]);
+                if (MASK) d[r] = (jt + k + r == self[t]) ? 1.0 : d[r];
+            }
+            if constexpr (R == 1) {
+                q[0] = s[0].w * recip_f64(d[0]);
+            } else if constexpr (R == 2) {
+                const double rr = recip1_f64(d[0] * d[1]);
+                q[0] = s[0].w * d[1] * rr;
+                q[1] = s[1].w * d[0] * rr;
+            } else {
+                const double p01 = d[0] * d[1], p23 = d[2] * d[3];
+                const double rr = recip1_f64(p01 * p23);
+                const double r01 = rr * p23, r23 = rr * p01;
+                q[0] = s[0].w * d[1] * r01;
+                q[1] = s[1].w * d[0] * r01;
+                q[2] = s[2].w * d[3] * r23;
+                q[3] = s[3].w * d[2] * r23;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (MASK) q[r] = (jt + k + r == self[t]) ? 0.0 : q[r];
+                sx[t] = __builtin_fma(q[r], s[r].x, sx[t]);
+                sy[t] = __builtin_fma(q[r], s[r].y, sy[t]);
+                sz[t] = __builtin_fma(q[r], s[r].z, sz[t]);
+            }
         }
     }
 }
@@ -67,7 +111,7 @@ __device__ __forceinline__ int xcd_work_item(int g, int grid)
     return (g % 8) * (grid / 8) + g / 8;
 }
 
-template <int T, int TILE>
+template <int T, int TILE, int R>
 __global__ __launch_bounds__(kBlock) void pair_sum_f64_kernel(
     const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
     int t_stride, TargetClasses cls, const Source *__restrict__ src, int n_src_pad,
@@ -110,9 +154,9 @@ __global__ __launch_bounds__(kBlock) void pair_sum_f64_kernel(
             diag |= (lo < hi) && (lo < jt + TILE) && (jt < hi);
         }
         if (diag)
-            tile_f64<T, TILE, true>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
+            tile_f64<T, TILE, R, true>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
         else
-            tile_f64<T, TILE, false>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
+            tile_f64<T, TILE, R, false>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
     }
 
     double *px = partial + (size_t)(chunk * 3) * t_stride;
@@ -125,23 +169,34 @@ __global__ __launch_bounds__(kBlock) void pair_sum_f64_kernel(
     }
 }
 
-template <int T, int TILE>
+template <int T, int TILE, int R>
 static hipError_t launch_f64(const PairArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((pair_sum_f64_kernel<T, TILE>), dim3(a.plan.grid), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL((pair_sum_f64_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0, s,
                        a.tx, a.ty, a.tz, a.t_stride, a.cls, a.src, a.n_src_pad, a.plan, a.partial);
     return hipGetLastError();
+}
+
+template <int T, int TILE>
+static hipError_t launch_f64_r(const PairArgs &a, hipStream_t s)
+{
+    switch (a.plan.recip) {
+    case 1: return launch_f64<T, TILE, 1>(a, s);
+    case 2: return launch_f64<T, TILE, 2>(a, s);
+    case 4: return launch_f64<T, TILE, 4>(a, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s)
 {
     const int T = a.plan.targets_per_lane, TL = a.plan.tile;
-    if (T == 1 && TL == 256) return launch_f64<1, 256>(a, s);
-    if (T == 2 && TL == 256) return launch_f64<2, 256>(a, s);
-    if (T == 4 && TL == 256) return launch_f64<4, 256>(a, s);
-    if (T == 1 && TL == 512) return launch_f64<1, 512>(a, s);
-    if (T == 2 && TL == 512) return launch_f64<2, 512>(a, s);
-    if (T == 4 && TL == 512) return launch_f64<4, 512>(a, s);
+    if (T == 1 && TL == 256) return launch_f64_r<1, 256>(a, s);
+    if (T == 2 && TL == 256) return launch_f64_r<2, 256>(a, s);
+    if (T == 4 && TL == 256) return launch_f64_r<4, 256>(a, s);
+    if (T == 1 && TL == 512) return launch_f64_r<1, 512>(a, s);
+    if (T == 2 && TL == 512) return launch_f64_r<2, 512>(a, s);
+    if (T == 4 && TL == 512) return launch_f64_r<4, 512>(a, s);
     return hipErrorInvalidValue;
 }
 

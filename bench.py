@@ -51,6 +51,8 @@ def parse():
                    help="target duration of the CPU-baseline sample (0 = skip)")
     p.add_argument("--plan", type=str, default="",
                    help="targets_per_lane,tile,workgroups,recip for the pair kernel")
+    p.add_argument("--adjoint-level", type=int, default=6,
+                   help="cubed-sphere level of the adjoint workload (0 = skip)")
     return p.parse_args()
 
 
@@ -76,6 +78,71 @@ def cpu_baseline(grid, zeta, seconds):
                        f"BVESmoothVelocity (reference rounding order, IEEE divisions) at level "
                        f"{grid.level}: {stages * pairs_stage:.4g} pairs in {el:.1f} s, "
                        f"{threads} OpenMP threads")
+
+
+def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64"):
+    """Gradient workload: RH4 on the level-6 cubed sphere, 20 RK4 steps at
+    dt = 0.01, checkpoints every 5, J = sum_i A_i zeta_i(T) g(x_i(T)) with a
+    width-0.2 Gaussian weight at 30N 90E.  Times the gradient (forward sweep
+    with checkpoints + reverse sweep) against the forward solve of the same
+    steps with all three target classes, and checks dJ/dzeta_j(0) against
+    central differences of the GPU forward map on n_fd seeded panels."""
+    import math
+    from thesiscode_amd import _lib
+    from thesiscode_amd import grid as gridmod
+    from thesiscode_amd import initial_conditions as ic
+    from thesiscode_amd.adjoint import GaussianWeightFunctional, gradient
+
+    g = gridmod.cubed_sphere(level)
+    z = ic.rh4_wave(g.active_x)
+    omega = ic.OMEGA_EARTH
+    absv = ic.rh4_wave(g.particles_x) + 2 * omega * g.particles_x[:, 2]
+    F = GaussianWeightFunctional()
+    ctx = _lib.Context(0)
+    ctx.set_omega(omega)
+    ctx.set_precision(MODES[mode])
+
+    def load(zz, particles=True):
+        ctx.reset()
+        ctx.load_panels(g.active_x, zz, g.active_area, g.passive_x)
+        if particles:
+            ctx.load_particles(g.particles_x, absv)
+
+    load(z)
+    ctx.advance(dt, 1)
+    ctx.synchronize()
+    load(z)
+    t0 = time.perf_counter()
+    ctx.advance(dt, steps)
+    ctx.synchronize()
+    t_fwd = time.perf_counter() - t0
+    load(z, particles=False)
+    gradient(ctx, F, g.active_area, dt, 2, 1)
+    t0 = time.perf_counter()
+    J, gz, gx = gradient(ctx, F, g.active_area, dt, steps, every)
+    t_adj = time.perf_counter() - t0
+
+    idx = np.random.default_rng(0).choice(g.n_active, n_fd, replace=False)
+
+    def Jof(zz):
+        load(zz, particles=False)
+        xT, zT = ctx.adjoint_forward(dt, steps, steps)
+        return math.fsum((g.active_area * zT * F.weight(xT)).tolist())
+
+    fd_report = {}
+    for eps in (1e-6, 1e-3):
+        fd = np.array([(Jof(z + eps * (np.arange(g.n_active) == j)) -
+                        Jof(z - eps * (np.arange(g.n_active) == j))) / (2 * eps) for j in idx])
+        rel = np.abs(gz[idx] - fd) / np.abs(fd)
+        fd_report[f"eps_{eps:g}"] = dict(max_rel_err=float(rel.max()),
+                                         median_rel_err=float(np.median(rel)),
+                                         normwise_rel_err=float(np.abs(gz[idx] - fd).max() /
+                                                                np.abs(fd).max()))
+    ctx.close()
+    return dict(workload=f"rh4_cubed_sphere_L{level}_{steps}steps_dt{dt:g}_checkpoint{every}",
+                mode=mode, n_active=g.n_active, J=J, forward_ms=t_fwd * 1e3,
+                gradient_ms=t_adj * 1e3, adjoint_fwd_ratio=t_adj / t_fwd,
+                fd_panels=int(n_fd), fd=fd_report)
 
 
 def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all):
@@ -177,6 +244,11 @@ def main():
             "flops_per_launch": pairs_all / 4.0 * FLOPS_PER_PAIR,
         },
     }
+    if args.adjoint_level > 0:
+        adj = adjoint_measurement(args.adjoint_level)
+        print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
+        out["adjoint_fwd_ratio"] = adj["adjoint_fwd_ratio"]
+        out["config"]["adjoint"] = adj
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(g, za, args.cpu_seconds)
     print(json.dumps(out), flush=True)

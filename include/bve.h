@@ -35,9 +35,11 @@ enum {
 };
 
 enum {
-    BVE_PRECISION_FP64 = 0,      /* float64 pairs, every 1/d refined to full precision */
-    BVE_PRECISION_MIXED = 1,     /* float32 far field, float64 near field and reductions */
-    BVE_PRECISION_FP64_FAST = 2  /* float64 pairs, one reciprocal shared by 2 or 4 pairs */
+    BVE_PRECISION_FP64 = 0,       /* float64 pairs, every 1/d refined to full precision */
+    BVE_PRECISION_MIXED = 1,      /* float32 far field, float64 near field and reductions;
+                                     float32 sums of each 256-source tile carried in float64 */
+    BVE_PRECISION_FP64_FAST = 2,  /* float64 pairs, one reciprocal shared by 2 or 4 pairs */
+    BVE_PRECISION_MIXED_KAHAN = 3 /* as MIXED with Kahan-compensated float32 sums */
 };
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -136,6 +138,24 @@ int bve_velocity(bve_ctx *ctx, int n_targets, const double *x_targets, int self_
  * as for the panel step (may be NULL). */
 int bve_voronoi_rk4_step(bve_ctx *ctx, double dt, int n, double *x, double *y, double *z,
                          double *relvort, const double *area, double *total_ke, double *ke);
+
+/* ---- adjoint (gradient of a final-time functional of the active panels) */
+
+/* No reference counterpart.  Runs n_steps RK4 steps of the active panels
+ * only (passive panels and particles do not act on them) from the resident
+ * state, keeping float64 checkpoints every checkpoint_every steps, and
+ * returns the final state x_T (3*n_active) and zeta_T (n_active, may be
+ * NULL).  The resident state is left at its starting value.  Float64
+ * precision modes only, renormalisation off. */
+int bve_adjoint_forward(bve_ctx *ctx, double dt, int n_steps, int checkpoint_every,
+                        double *x_T, double *zeta_T);
+/* Reverse sweep for the cotangent (dJ/dx_T, dJ/dzeta_T) of a functional J of
+ * the final active state: recomputes each checkpoint segment's stage inputs
+ * and applies the transposed RK4 stages with the hand-written vector-
+ * Jacobian product of the pair sum.  Returns dJ/dx_0 (3*n_active, may be
+ * NULL) and dJ/dzeta_0 (n_active). */
+int bve_adjoint_backward(bve_ctx *ctx, const double *lambda_x, const double *lambda_zeta,
+                         double *grad_x0, double *grad_zeta0);
 
 /* ---- measurement ------------------------------------------------------ */
 

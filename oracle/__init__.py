@@ -40,6 +40,8 @@ def lib():
         L.bvo_rk4_step.argtypes = [_i, _p, _p, _p, _i, _p, _i, _p, _p, _p, _d, _d, _d, _p, _p]
         L.bvo_renormalize.argtypes = [_i, _p]
         L.bvo_velocity_q.argtypes = [_i, _p, _i, _p, _p, _i, _d, _p]
+        L.bvo_jvp_active.argtypes = [_i, _p, _p, _p, _p, _p, _d, _p, _p]
+        L.bvo_tlm_rk4.argtypes = [_i, _p, _p, _p, _p, _p, _d, _d, _i]
         L.bvo_num_threads.restype = _i
         L.bvo_set_num_threads.argtypes = [_i]
         _lib = L
@@ -129,6 +131,27 @@ def velocity_q(X, x, w, self_off=-1, eps2=0.0):
     u = np.zeros((len(X), 3))
     lib().bvo_velocity_q(len(X), _ptr(X), len(w), _ptr(x), _ptr(w), self_off, eps2, _ptr(u))
     return u
+
+
+def jvp_active(x, zeta, area, dx, dzeta, omega):
+    """Jacobian-vector product of the active-panel right-hand side."""
+    x, zeta, area, dx, dzeta = (_c(a) for a in (x, zeta, area, dx, dzeta))
+    n = len(zeta)
+    du = np.zeros((n, 3))
+    dz = np.zeros(n)
+    lib().bvo_jvp_active(n, _ptr(x), _ptr(zeta), _ptr(area), _ptr(dx), _ptr(dzeta), omega,
+                         _ptr(du), _ptr(dz))
+    return du, dz
+
+
+def tlm_rk4(x, zeta, area, dx, dzeta, dt, omega, n_steps):
+    """Tangent-linear model of n_steps RK4 steps of the active panels.
+    Returns (x_T, zeta_T, dx_T, dzeta_T)."""
+    x, zeta, dx, dzeta = (_c(a).copy() for a in (x, zeta, dx, dzeta))
+    area = _c(area)
+    lib().bvo_tlm_rk4(len(zeta), _ptr(x), _ptr(zeta), _ptr(area), _ptr(dx), _ptr(dzeta), dt, omega,
+                      n_steps)
+    return x, zeta, dx, dzeta
 
 
 def num_threads():

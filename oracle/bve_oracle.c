@@ -256,6 +256,109 @@ void bvo_velocity_q(int m, const double *X, int n, const double *x, const double
     }
 }
 
+/* ---- tangent-linear model of the active-panel dynamics ----------------
+ *
+ * State y = (x_i, zeta_i), i < n, areas A fixed; f(y) = (U, -2 Omega U_z)
+ * with U the BVEActiveRHS sum (PanelVelocityParallel.f90:1287-1334) and
+ * w_j = zeta_j A_j.  Its Jacobian-vector product, differentiated by hand:
+ *
+ *   dU_i = -(1/4 PI) sum_{j != i} [ dw_j (x_i x x_j)/D
+ *                                 + w_j (dx_i x x_j + x_i x dx_j)/D
+ *                                 + w_j (x_i x x_j)(dx_i.x_j + x_i.dx_j)/D^2 ],
+ *   D = 1 - x_i.x_j,  dw_j = dzeta_j A_j,  dzeta_dot_i = -2 Omega dU_z,i.
+ *
+ * Pinned by central differences of bvo_rk4_step (which is bitwise the
+ * reference) in tests/test_oracle.py. */
+void bvo_jvp_active(int n, const double *x, const double *zeta, const double *area,
+                    const double *dx, const double *dzeta, double omega, double *du,
+                    double *dzdot)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+        const double *xi = x + 3 * i, *dxi = dx + 3 * i;
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double *xj = x + 3 * j, *dxj = dx + 3 * j;
+            const double D = 1.0 - (xi[0] * xj[0] + xi[1] * xj[1] + xi[2] * xj[2]);
+            const double w = zeta[j] * area[j], dw = dzeta[j] * area[j];
+            const double c0 = xi[1] * xj[2] - xi[2] * xj[1];
+            const double c1 = xi[2] * xj[0] - xi[0] * xj[2];
+            const double c2 = xi[0] * xj[1] - xi[1] * xj[0];
+            /* dx_i x x_j + x_i x dx_j */
+            const double e0 = (dxi[1] * xj[2] - dxi[2] * xj[1]) + (xi[1] * dxj[2] - xi[2] * dxj[1]);
+            const double e1 = (dxi[2] * xj[0] - dxi[0] * xj[2]) + (xi[2] * dxj[0] - xi[0] * dxj[2]);
+            const double e2 = (dxi[0] * xj[1] - dxi[1] * xj[0]) + (xi[0] * dxj[1] - xi[1] * dxj[0]);
+            const double dD = dxi[0] * xj[0] + dxi[1] * xj[1] + dxi[2] * xj[2] +
+                              xi[0] * dxj[0] + xi[1] * dxj[1] + xi[2] * dxj[2];
+            const double r = 1.0 / D;
+            a0 += dw * c0 * r + w * e0 * r + w * c0 * dD * r * r;
+            a1 += dw * c1 * r + w * e1 * r + w * c1 * dD * r * r;
+            a2 += dw * c2 * r + w * e2 * r + w * c2 * dD * r * r;
+        }
+        const double c = -4.0 * BVO_PI;
+        du[3 * i] = a0 / c;
+        du[3 * i + 1] = a1 / c;
+        du[3 * i + 2] = a2 / c;
+        dzdot[i] = -(2.0 * omega * du[3 * i + 2]);
+    }
+}
+
+/* Tangent of n_steps RK4 steps (BVERK4's scheme, active panels only):
+ * (x, zeta, dx, dzeta) advanced in place; the base state is bvo_rk4_step's
+ * up to rounding (the update is accumulated as x + sum b_s k_s). */
+void bvo_tlm_rk4(int n, double *x, double *zeta, const double *area, double *dx, double *dzeta,
+                 double dt, double omega, int n_steps)
+{
+    const size_t n3 = 3 * (size_t)n;
+    double *X = malloc(sizeof(double) * n3), *Z = malloc(sizeof(double) * n);
+    double *dX = malloc(sizeof(double) * n3), *dZ = malloc(sizeof(double) * n);
+    double *u = malloc(sizeof(double) * n3), *zd = malloc(sizeof(double) * n);
+    double *du = malloc(sizeof(double) * n3), *dzd = malloc(sizeof(double) * n);
+    double *kx = malloc(sizeof(double) * n3), *kz = malloc(sizeof(double) * n);
+    double *dkx = malloc(sizeof(double) * n3), *dkz = malloc(sizeof(double) * n);
+    double *ax = malloc(sizeof(double) * n3), *az = malloc(sizeof(double) * n);
+    double *dax = malloc(sizeof(double) * n3), *daz = malloc(sizeof(double) * n);
+    static const double a_in[4] = {0.0, 0.5, 0.5, 1.0};
+    static const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    for (int step = 0; step < n_steps; ++step) {
+        memcpy(ax, x, sizeof(double) * n3);
+        memcpy(az, zeta, sizeof(double) * n);
+        memcpy(dax, dx, sizeof(double) * n3);
+        memcpy(daz, dzeta, sizeof(double) * n);
+        for (int s = 0; s < 4; ++s) {
+            for (size_t k = 0; k < n3; ++k) {
+                X[k] = x[k] + a_in[s] * (s ? kx[k] : 0.0);
+                dX[k] = dx[k] + a_in[s] * (s ? dkx[k] : 0.0);
+            }
+            for (int k = 0; k < n; ++k) {
+                Z[k] = zeta[k] + a_in[s] * (s ? kz[k] : 0.0);
+                dZ[k] = dzeta[k] + a_in[s] * (s ? dkz[k] : 0.0);
+            }
+            bvo_active_rhs(n, X, Z, area, 0, n, omega, u, zd);
+            bvo_jvp_active(n, X, Z, area, dX, dZ, omega, du, dzd);
+            for (size_t k = 0; k < n3; ++k) {
+                kx[k] = dt * u[k];
+                dkx[k] = dt * du[k];
+                ax[k] += b[s] * kx[k];
+                dax[k] += b[s] * dkx[k];
+            }
+            for (int k = 0; k < n; ++k) {
+                kz[k] = dt * zd[k];
+                dkz[k] = dt * dzd[k];
+                az[k] += b[s] * kz[k];
+                daz[k] += b[s] * dkz[k];
+            }
+        }
+        memcpy(x, ax, sizeof(double) * n3);
+        memcpy(zeta, az, sizeof(double) * n);
+        memcpy(dx, dax, sizeof(double) * n3);
+        memcpy(dzeta, daz, sizeof(double) * n);
+    }
+    free(X); free(Z); free(dX); free(dZ); free(u); free(zd); free(du); free(dzd);
+    free(kx); free(kz); free(dkx); free(dkz); free(ax); free(az); free(dax); free(daz);
+}
+
 /* Threads the OpenMP loops above use (1 when built without OpenMP). */
 int bvo_num_threads(void)
 {

@@ -185,6 +185,53 @@ def test_fast_fp64_step_matches_reference(ctx):
     assert np.abs(px - g["sn_passive_x"]).max() < 10 * TOL * disp
 
 
+@pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_MIXED, _lib.BVE_PRECISION_MIXED_KAHAN])
+def test_mixed_with_everything_near_is_float64(ctx, precision):
+    """With a near-field radius above the sphere's diameter every pair takes
+    the mixed kernel's float64 path: the result must meet the float64 bar
+    (this pins the near-field branch and the pair bookkeeping: no pair lost
+    or counted twice, self pairs excluded, smoothed class honoured)."""
+    g = gridmod.cubed_sphere(4)
+    z = ic.rh4_wave(g.active_x)
+    na, npp = g.n_active, g.n_passive
+    ref_a, _ = oracle.active_rhs(g.active_x, z, g.active_area, 0.0)
+    ref_s = oracle.smooth_velocity(g.passive_x, g.active_x, z, g.active_area, 0.05)
+    ref_p = oracle.passive_rhs(g.particles_x, g.active_x, z, g.active_area)
+    t = np.concatenate([g.active_x, g.passive_x, g.particles_x])
+    ctx.set_smooth(0.05)
+    ctx.set_precision(precision, 2.5)
+    try:
+        u = ctx.velocity(t, g.active_x, z, g.active_area, self_end=na, smooth_range=(na, na + npp))
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    assert normwise_rel(u[:na], ref_a) < TOL
+    assert normwise_rel(u[na:na + npp], ref_s) < TOL
+    assert normwise_rel(u[na + npp:], ref_p) < TOL
+
+
+@pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_MIXED, _lib.BVE_PRECISION_MIXED_KAHAN])
+@pytest.mark.parametrize("rnf", [0.0, 0.02, 0.1])
+def test_mixed_error(ctx, precision, rnf):
+    """float32 far field: the normwise error against the float64 oracle is
+    bounded by the float32 term rounding (~1e-7 per term); the bound here
+    is loose, the measured values are reported by the benchmark."""
+    g = gridmod.cubed_sphere(5)
+    za, zp, zq = ic.gauss_vortex(g, width=0.05)
+    na, npp = g.n_active, g.n_passive
+    ref_a, _ = oracle.active_rhs(g.active_x, za, g.active_area, 0.0)
+    ref_p = oracle.passive_rhs(g.particles_x, g.active_x, za, g.active_area)
+    t = np.concatenate([g.active_x, g.passive_x, g.particles_x])
+    ctx.set_precision(precision, rnf)
+    try:
+        u = ctx.velocity(t, g.active_x, za, g.active_area, self_end=na, smooth_range=(na, na + npp))
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    ea = normwise_rel(u[:na], ref_a)
+    ep = normwise_rel(u[na + npp:], ref_p)
+    print(f"mixed precision={precision} rnf={rnf}: active {ea:.3e} particles {ep:.3e}")
+    assert ea < 1e-5 and ep < 1e-5
+
+
 def test_level7_invariants(ctx):
     """Full benchmark size (98304 active panels): tangency u.x = 0 and the
     active-block momentum sum_i w_i u_i = 0, which no subset of a wrong sum

@@ -163,3 +163,47 @@ def test_renormalize():
     x = rng.normal(size=(1000, 3))
     y = oracle.renormalize(x)
     assert np.abs(np.linalg.norm(y, axis=1) - 1).max() < 3e-16
+
+
+def test_jvp_matches_central_differences(l4_state):
+    """The hand-derived Jacobian-vector product of the active sum against
+    central differences of the (bitwise-reference) oracle sum."""
+    g, z = l4_state
+    x, a = g.active_x, g.active_area
+    rng = np.random.default_rng(11)
+    dx = rng.normal(size=x.shape) * 1e-2
+    dz = rng.normal(size=z.shape)
+    du, dzd = oracle.jvp_active(x, z, a, dx, dz, 2 * np.pi)
+    errs = []
+    for eps in (1e-4, 1e-5, 1e-6):
+        up, zp = oracle.active_rhs(x + eps * dx, z + eps * dz, a, 2 * np.pi)
+        um, zm = oracle.active_rhs(x - eps * dx, z - eps * dz, a, 2 * np.pi)
+        errs.append(normwise_rel(du, (up - um) / (2 * eps)))
+    # central differences converge as eps^2 to the exact derivative
+    assert errs[0] / errs[1] > 50 and errs[2] < 2e-8, errs
+    assert np.abs(dzd - (zp - zm) / (2 * eps)).max() < 2e-8 * np.abs(dzd).max()
+
+
+def test_tlm_matches_central_differences_of_rk4(l4_state):
+    """Tangent-linear of 3 RK4 steps against central differences of the
+    oracle's RK4 (bitwise the reference's BVERK4)."""
+    g, z = l4_state
+    x, a = g.active_x, g.active_area
+    rng = np.random.default_rng(5)
+    dx = np.zeros_like(x)
+    dz = rng.normal(size=z.shape)
+    eps, dt, omega, nsteps = 1e-5, 0.01, 2 * np.pi, 3
+
+    def run(zeta0):
+        st = (x, zeta0)
+        for _ in range(nsteps):
+            r = oracle.rk4_step(st[0], st[1], a, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0),
+                                dt, omega, 0.05)
+            st = (r["active_x"], r["active_relvort"])
+        return st
+    xp, zp = run(z + eps * dz)
+    xm, zm = run(z - eps * dz)
+    xT, zT, dxT, dzT = oracle.tlm_rk4(x, z, a, dx, dz, dt, omega, nsteps)
+    assert normwise_rel(dxT, (xp - xm) / (2 * eps)) < 1e-7
+    assert np.abs(dzT - (zp - zm) / (2 * eps)).max() < 1e-7 * np.abs(dzT).max()
+    assert np.abs(xT - run(z)[0]).max() < 1e-14

@@ -15,6 +15,7 @@ BVE_OK = 0
 BVE_PRECISION_FP64 = 0
 BVE_PRECISION_MIXED = 1
 BVE_PRECISION_FP64_FAST = 2
+BVE_PRECISION_MIXED_KAHAN = 3
 
 # every symbol include/bve.h declares, with its ctypes signature
 _d, _i, _p, _ll = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong
@@ -40,6 +41,8 @@ SIGNATURES = {
     "bve_synchronize": (_i, [_p]),
     "bve_velocity": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p]),
     "bve_voronoi_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p, _p, _p]),
+    "bve_adjoint_forward": (_i, [_p, _d, _i, _i, _p, _p]),
+    "bve_adjoint_backward": (_i, [_p, _p, _p, _p, _p]),
     "bve_set_profiling": (_i, [_p, _i]),
     "bve_get_profile": (_i, [_p, ctypes.POINTER(_ll), _dp]),
     "bve_pairs_per_step": (_i, [_p, _dp]),
@@ -221,6 +224,26 @@ class Context:
         self._check(self._L.bve_voronoi_rk4_step(self._h, dt, n, *(_ptr(a) for a in xs), _ptr(ar),
                                                  _ptr(tot), _ptr(ke)), "bve_voronoi_rk4_step")
         return xs[0], xs[1], xs[2], xs[3], float(tot[0]), ke
+
+    # adjoint
+    def adjoint_forward(self, dt, n_steps, checkpoint_every):
+        """Active-panel forward sweep with checkpoints; returns (x_T, zeta_T)."""
+        xT = np.zeros((self.n_active, 3))
+        zT = np.zeros(self.n_active)
+        self._check(self._L.bve_adjoint_forward(self._h, dt, n_steps, checkpoint_every, _ptr(xT),
+                                                _ptr(zT)), "bve_adjoint_forward")
+        return xT, zT
+
+    def adjoint_backward(self, lambda_x, lambda_zeta):
+        """Reverse sweep; returns (dJ/dx_0, dJ/dzeta_0)."""
+        n = self.n_active
+        lx = _f64(lambda_x, 3 * n)
+        lz = _f64(lambda_zeta, n)
+        gx = np.zeros((n, 3))
+        gz = np.zeros(n)
+        self._check(self._L.bve_adjoint_backward(self._h, _ptr(lx), _ptr(lz), _ptr(gx), _ptr(gz)),
+                    "bve_adjoint_backward")
+        return gx, gz
 
     # measurement
     def set_profiling(self, on):

@@ -62,9 +62,24 @@ struct bve_ctx {
     double *ke = nullptr, *ke_part = nullptr, *ke_total = nullptr;
     int ke_part_cap = 0;
     double *staging = nullptr;   // AoS staging for uploads/downloads
+    float4 *src32 = nullptr;     // mixed precision: float32 source records
+    float *srcc = nullptr;
+    double4 *tile_box = nullptr, *group_box = nullptr;
+    int mixed_src_cap = 0, mixed_tgt_cap = 0;
     size_t staging_cap = 0;      // doubles
     bool particles_have_absvort = false;
     PairPlan plan{};
+
+    // adjoint (active panels only)
+    int adj_n = 0, adj_pad = 0, adj_steps = 0, adj_every = 0, adj_ckpts = 0;
+    double adj_dt = 0.0;
+    bool adj_ready = false;
+    double *adj_ckpt = nullptr;     // adj_ckpts x 4 rows (x, y, z, zeta) x adj_pad
+    Source *adj_stages = nullptr;   // adj_every steps x 4 stages x adj_pad records
+    double *adj_lam = nullptr, *adj_lam_acc = nullptr, *adj_ybar = nullptr, *adj_m = nullptr;
+    AdjRec *adj_rec = nullptr;
+    double *adj_pt = nullptr, *adj_ps = nullptr;
+    size_t adj_pt_cap = 0, adj_ps_cap = 0, adj_ckpt_cap = 0, adj_stage_cap = 0;
 
     // profiling
     bool profiling = false;
@@ -167,8 +182,11 @@ int recip_for(const bve_ctx *c)
 
 PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
 {
+    const bool mixed = c->precision == BVE_PRECISION_MIXED ||
+                       c->precision == BVE_PRECISION_MIXED_KAHAN;
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(n_targets);
-    const int tile = c->plan_tile ? c->plan_tile : 256;
+    // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
+    const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
     const int wg = c->plan_wg ? c->plan_wg : 4096;
     return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c));
 }
@@ -278,14 +296,59 @@ void collect_profile(bve_ctx *c)
     c->pending.clear();
 }
 
-int pair_pass(bve_ctx *c, const PairArgs &a)
+bool is_mixed(const bve_ctx *c)
 {
+    return c->precision == BVE_PRECISION_MIXED || c->precision == BVE_PRECISION_MIXED_KAHAN;
+}
+
+// Buffers of the mixed-precision pass for n_src_pad sources and t_stride targets.
+int ensure_mixed(bve_ctx *c, int n_src_pad, int t_stride)
+{
+    if (n_src_pad > c->mixed_src_cap) {
+        dev_free(c->src32);
+        dev_free(c->srcc);
+        dev_free(c->tile_box);
+        HIP_TRY(c, dev_alloc(&c->src32, n_src_pad));
+        HIP_TRY(c, dev_alloc(&c->srcc, n_src_pad));
+        HIP_TRY(c, dev_alloc(&c->tile_box, n_src_pad / 256));
+        c->mixed_src_cap = n_src_pad;
+    }
+    if (t_stride > c->mixed_tgt_cap) {
+        dev_free(c->group_box);
+        HIP_TRY(c, dev_alloc(&c->group_box, t_stride / 64));
+        c->mixed_tgt_cap = t_stride;
+    }
+    return BVE_OK;
+}
+
+// One pair-sum pass over n_targets targets and n_sources sources in the
+// context's arithmetic (the mixed pass first builds its per-stage records).
+int pair_pass(bve_ctx *c, const PairArgs &a, int n_targets, int n_sources)
+{
+    MixedArgs ma{};
+    if (is_mixed(c)) {
+        int rc = ensure_mixed(c, a.n_src_pad, a.t_stride);
+        if (rc) return rc;
+        ma.base = a;
+        ma.n_targets = n_targets;
+        ma.n_sources = n_sources;
+        ma.src32 = c->src32;
+        ma.srcc = c->srcc;
+        ma.tile_box = c->tile_box;
+        ma.group_box = c->group_box;
+        ma.rnf = c->nearfield;
+        ma.kahan = c->precision == BVE_PRECISION_MIXED_KAHAN;
+        HIP_TRY(c, launch_mixed_prep(ma, c->stream));
+    }
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (c->profiling) {
         ev = take_events(c);
         HIP_TRY(c, hipEventRecord(ev.first, c->stream));
     }
-    HIP_TRY(c, launch_pair_sum_f64(a, c->stream));
+    if (is_mixed(c))
+        HIP_TRY(c, launch_pair_sum_mixed(ma, c->stream));
+    else
+        HIP_TRY(c, launch_pair_sum_f64(a, c->stream));
     if (c->profiling) {
         HIP_TRY(c, hipEventRecord(ev.second, c->stream));
         c->pending.push_back(ev);
@@ -293,9 +356,15 @@ int pair_pass(bve_ctx *c, const PairArgs &a)
     return BVE_OK;
 }
 
-int run_steps(bve_ctx *c, double dt, int n_steps)
+// n_steps RK4 steps of the first n_targets targets ([active | passive |
+// particles] prefix) with launch plan `plan`.  If stage_store is given, the
+// source records of every stage input are copied there, stage_stride records
+// apart, in step-major order (the adjoint's recomputation).
+int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPlan &plan,
+                 Source *stage_store, size_t stage_stride)
 {
     StepState st = step_state(c);
+    st.n_targets = n_targets;
     PairArgs pa{};
     pa.tx = c->xs;
     pa.ty = c->xs + c->t_stride;
@@ -307,18 +376,22 @@ int run_steps(bve_ctx *c, double dt, int n_steps)
     pa.cls.eps2 = c->smooth * c->smooth;
     pa.src = c->src;
     pa.n_src_pad = c->n_src_pad;
-    pa.plan = c->plan;
+    pa.plan = plan;
     pa.partial = c->partial;
     static const double a_next[4] = {0.5, 0.5, 1.0, 0.0};
     static const double divisor[4] = {6.0, 3.0, 3.0, 6.0};
     HIP_TRY(c, launch_prepare_step(st, c->stream));
     for (int n = 0; n < n_steps; ++n) {
         for (int s = 0; s < 4; ++s) {
-            int rc = pair_pass(c, pa);
+            if (stage_store)
+                HIP_TRY(c, hipMemcpyAsync(stage_store + (size_t)(4 * n + s) * stage_stride, c->src,
+                                          sizeof(Source) * c->n_src_pad, hipMemcpyDeviceToDevice,
+                                          c->stream));
+            int rc = pair_pass(c, pa, n_targets, c->n_active);
             if (rc) return rc;
             StageParams sp{};
             sp.stage = s;
-            sp.n_chunks = c->plan.n_chunks;
+            sp.n_chunks = plan.n_chunks;
             sp.dt = dt;
             sp.omega = c->omega;
             sp.a_next = a_next[s];
@@ -327,6 +400,109 @@ int run_steps(bve_ctx *c, double dt, int n_steps)
             HIP_TRY(c, launch_stage_update(st, sp, c->stream));
         }
     }
+    return BVE_OK;
+}
+
+int run_steps(bve_ctx *c, double dt, int n_steps)
+{
+    return run_steps_on(c, dt, n_steps, c->n_targets, c->plan, nullptr, 0);
+}
+
+// ---- adjoint helpers -------------------------------------------------------
+
+void free_adjoint(bve_ctx *c)
+{
+    dev_free(c->adj_ckpt);
+    dev_free(c->adj_stages);
+    dev_free(c->adj_lam);
+    dev_free(c->adj_lam_acc);
+    dev_free(c->adj_ybar);
+    dev_free(c->adj_m);
+    dev_free(c->adj_rec);
+    dev_free(c->adj_pt);
+    dev_free(c->adj_ps);
+    c->adj_pt_cap = c->adj_ps_cap = c->adj_ckpt_cap = c->adj_stage_cap = 0;
+    c->adj_ready = false;
+    c->adj_n = c->adj_pad = 0;
+}
+
+template <class P>
+int grow(bve_ctx *c, P *&p, size_t &cap, size_t count, bool zero)
+{
+    if (count <= cap && p) return BVE_OK;
+    dev_free(p);
+    HIP_TRY(c, dev_alloc(&p, count));
+    if (zero) HIP_TRY(c, hipMemsetAsync(p, 0, sizeof(P) * count, c->stream));
+    cap = count;
+    return BVE_OK;
+}
+
+// Copies the active panels' positions and vorticity (x0 rows, zeta0) to or
+// from 4 rows of adj_pad entries.
+int copy_active_state(bve_ctx *c, double *rows, bool save)
+{
+    const size_t n = (size_t)c->n_active, P = (size_t)c->adj_pad;
+    for (int r = 0; r < 3; ++r) {
+        double *dev = c->x0 + (size_t)r * c->t_stride;
+        if (save)
+            HIP_TRY(c, hipMemcpyAsync(rows + r * P, dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        else
+            HIP_TRY(c, hipMemcpyAsync(dev, rows + r * P, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (save)
+        HIP_TRY(c, hipMemcpyAsync(rows + 3 * P, c->zeta0, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    else
+        HIP_TRY(c, hipMemcpyAsync(c->zeta0, rows + 3 * P, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    return BVE_OK;
+}
+
+// Plans of the adjoint: the active-only forward pass and the VJP passes.
+int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
+{
+    fwd = plan_for(c, c->n_active, c->n_src_pad);
+    int rc = ensure_partial(c, (size_t)fwd.n_chunks * 3 * c->t_stride);
+    if (rc) return rc;
+    const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_active);
+    const int wg = c->plan_wg ? c->plan_wg : 4096;
+    vjp = make_plan(c->n_active, c->adj_pad, T, 256, wg,
+                    c->precision == BVE_PRECISION_FP64_FAST ? 4 : 1);
+    rc = grow(c, c->adj_pt, c->adj_pt_cap, (size_t)vjp.n_chunks * 6 * c->adj_pad, false);
+    if (rc) return rc;
+    return grow(c, c->adj_ps, c->adj_ps_cap, (size_t)vjp.n_chunks * 4 * c->adj_pad, false);
+}
+
+// One reverse RK4 step: lam <- lam + sum_s Ybar_s with the stage inputs in
+// `stages` (4 x adj_pad source records).
+int adjoint_reverse_step(bve_ctx *c, const Source *stages, const PairPlan &vjp)
+{
+    static const double B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    static const double C[4] = {0.5, 0.5, 1.0, 0.0};   // a of the next stage's input
+    const size_t P = (size_t)c->adj_pad;
+    HIP_TRY(c, hipMemcpyAsync(c->adj_lam_acc, c->adj_lam, sizeof(double) * 4 * P,
+                              hipMemcpyDeviceToDevice, c->stream));
+    for (int s = 3; s >= 0; --s) {
+        AdjStage a{};
+        a.n = c->n_active;
+        a.n_pad = c->adj_pad;
+        a.stride = c->adj_pad;
+        a.h = c->adj_dt;
+        a.omega = c->omega;
+        a.b = B[s];
+        a.c = C[s];
+        a.lam = c->adj_lam;
+        a.ybar = c->adj_ybar;
+        a.lam_acc = c->adj_lam_acc;
+        a.m = c->adj_m;
+        a.rec = c->adj_rec;
+        a.ys = stages + (size_t)s * P;
+        a.area = c->area;
+        a.plan = vjp;
+        a.partial_t = c->adj_pt;
+        a.partial_s = c->adj_ps;
+        a.n_chunks_t = a.n_chunks_s = vjp.n_chunks;
+        HIP_TRY(c, launch_adjoint_stage(a, c->stream));
+    }
+    std::swap(c->adj_lam, c->adj_lam_acc);
     return BVE_OK;
 }
 
@@ -401,8 +577,13 @@ int bve_destroy(bve_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_state(c);
+    free_adjoint(c);
     dev_free(c->partial);
     dev_free(c->staging);
+    dev_free(c->src32);
+    dev_free(c->srcc);
+    dev_free(c->tile_box);
+    dev_free(c->group_box);
     for (auto &ev : c->pending) c->event_pool.push_back(ev);
     for (auto &ev : c->event_pool) {
         (void)hipEventDestroy(ev.first);
@@ -443,10 +624,8 @@ int bve_set_precision(bve_ctx *c, int precision, double nearfield_radius)
 {
     if (!c) return BVE_ERR_ARGUMENT;
     if (precision != BVE_PRECISION_FP64 && precision != BVE_PRECISION_MIXED &&
-        precision != BVE_PRECISION_FP64_FAST)
+        precision != BVE_PRECISION_FP64_FAST && precision != BVE_PRECISION_MIXED_KAHAN)
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: unknown precision %d", precision);
-    if (precision == BVE_PRECISION_MIXED)
-        return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: mixed precision is not built yet");
     if (!(nearfield_radius >= 0.0))
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: nearfield_radius must be >= 0");
     c->precision = precision;
@@ -496,6 +675,7 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     free_state(c);
+    free_adjoint(c);
     c->n_active = n_active;
     c->n_passive = n_passive;
     c->n_src_pad = round_up(n_active, kSourceAlign);
@@ -636,6 +816,7 @@ int bve_reset(bve_ctx *c)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     collect_profile(c);
     free_state(c);
+    free_adjoint(c);
     return BVE_OK;
 }
 
@@ -708,7 +889,7 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
     pa.n_src_pad = ns_pad;
     pa.plan = plan;
     pa.partial = c->partial;
-    rc = pair_pass(c, pa);
+    rc = pair_pass(c, pa, n_targets, n_sources);
     if (rc) { (void)hipStreamSynchronize(c->stream); cleanup(); return rc; }
     if (dvort && self_end > 0) VTRY(dev_alloc(&dv, self_end));
     VTRY(launch_finalize_velocity(c->partial, plan.n_chunks, t, stride, n_targets, c->omega, uu,
@@ -753,6 +934,110 @@ int bve_voronoi_rk4_step(bve_ctx *c, double dt, int n, double *x, double *y, dou
         z[j] = aos[3 * (size_t)j + 2];
     }
     return bve_reset(c);
+}
+
+int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every, double *x_T,
+                        double *zeta_T)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_adjoint_forward: no panels loaded");
+    if (n_steps < 1 || checkpoint_every < 1)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_adjoint_forward: n_steps and checkpoint_every must be >= 1");
+    if (c->precision != BVE_PRECISION_FP64 && c->precision != BVE_PRECISION_FP64_FAST)
+        return c->fail(BVE_ERR_STATE, "bve_adjoint_forward: the adjoint runs in the float64 modes");
+    if (c->renormalize)
+        return c->fail(BVE_ERR_STATE, "bve_adjoint_forward: renormalisation is not differentiated; switch it off");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int every = std::min(checkpoint_every, n_steps);
+    const int ckpts = ceil_div(n_steps, every);
+    if (c->adj_n != c->n_active) free_adjoint(c);
+    c->adj_n = c->n_active;
+    c->adj_pad = round_up(c->n_active, 1024);
+    const size_t P = (size_t)c->adj_pad;
+    int rc = grow(c, c->adj_ckpt, c->adj_ckpt_cap, (size_t)ckpts * 4 * P, true);
+    if (!rc) rc = grow(c, c->adj_stages, c->adj_stage_cap, (size_t)every * 4 * P, true);
+    size_t cap4 = 0;
+    if (!rc && !c->adj_lam) {
+        rc = grow(c, c->adj_lam, cap4, 4 * P, true);
+        cap4 = 0;
+        if (!rc) rc = grow(c, c->adj_lam_acc, cap4, 4 * P, true);
+        cap4 = 0;
+        if (!rc) rc = grow(c, c->adj_ybar, cap4, 4 * P, true);
+        cap4 = 0;
+        if (!rc) rc = grow(c, c->adj_m, cap4, 3 * P, true);
+        cap4 = 0;
+        if (!rc) rc = grow(c, c->adj_rec, cap4, P, true);
+    }
+    if (rc) return rc;
+    PairPlan fwd, vjp;
+    rc = adjoint_plans(c, fwd, vjp);
+    if (rc) return rc;
+    for (int k = 0; k < ckpts; ++k) {
+        rc = copy_active_state(c, c->adj_ckpt + (size_t)k * 4 * P, true);
+        if (!rc) rc = run_steps_on(c, dt, std::min(every, n_steps - k * every), c->n_active, fwd,
+                                   nullptr, 0);
+        if (rc) return rc;
+    }
+    rc = download_aos(c, c->x0, c->t_stride, 0, c->n_active, x_T);
+    if (rc) return rc;
+    if (zeta_T)
+        HIP_TRY(c, hipMemcpyAsync(zeta_T, c->zeta0, sizeof(double) * c->n_active,
+                                  hipMemcpyDeviceToHost, c->stream));
+    rc = copy_active_state(c, c->adj_ckpt, false);   // the resident state is left at t = 0
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    c->adj_dt = dt;
+    c->adj_steps = n_steps;
+    c->adj_every = every;
+    c->adj_ckpts = ckpts;
+    c->adj_ready = true;
+    return BVE_OK;
+}
+
+int bve_adjoint_backward(bve_ctx *c, const double *lambda_x, const double *lambda_zeta,
+                         double *grad_x0, double *grad_zeta0)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->adj_ready || c->adj_n != c->n_active || !c->panels_loaded)
+        return c->fail(BVE_ERR_STATE, "bve_adjoint_backward: call bve_adjoint_forward first");
+    if (!lambda_x || !lambda_zeta)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_adjoint_backward: null cotangent");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t P = (size_t)c->adj_pad;
+    const int n = c->n_active;
+    int rc = ensure_staging(c, 3 * (size_t)n);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->adj_lam, 0, sizeof(double) * 4 * P, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->staging, lambda_x, 3 * sizeof(double) * n, hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, launch_aos_to_soa(c->staging, n, c->adj_lam, c->adj_pad, 0, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->adj_lam + 3 * P, lambda_zeta, sizeof(double) * n,
+                              hipMemcpyHostToDevice, c->stream));
+    PairPlan fwd, vjp;
+    rc = adjoint_plans(c, fwd, vjp);
+    if (rc) return rc;
+    for (int seg = c->adj_ckpts - 1; seg >= 0; --seg) {
+        const int n0 = seg * c->adj_every;
+        const int len = std::min(c->adj_every, c->adj_steps - n0);
+        rc = copy_active_state(c, c->adj_ckpt + (size_t)seg * 4 * P, false);
+        if (!rc) rc = run_steps_on(c, c->adj_dt, len, c->n_active, fwd, c->adj_stages, P);
+        if (rc) return rc;
+        for (int l = len - 1; l >= 0; --l) {
+            rc = adjoint_reverse_step(c, c->adj_stages + (size_t)l * 4 * P, vjp);
+            if (rc) return rc;
+        }
+    }
+    rc = download_aos(c, c->adj_lam, c->adj_pad, 0, n, grad_x0);
+    if (rc) return rc;
+    if (grad_zeta0)
+        HIP_TRY(c, hipMemcpyAsync(grad_zeta0, c->adj_lam + 3 * P, sizeof(double) * n,
+                                  hipMemcpyDeviceToHost, c->stream));
+    rc = copy_active_state(c, c->adj_ckpt, false);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    collect_profile(c);
+    return BVE_OK;
 }
 
 int bve_set_profiling(bve_ctx *c, int on)

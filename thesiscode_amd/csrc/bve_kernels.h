@@ -17,6 +17,48 @@ struct PairArgs {
     double *partial;
 };
 
+// One mixed-precision pair-sum pass: the float64 pass's arguments plus the
+// per-stage float32 source records, bounding spheres and near-field radius.
+struct MixedArgs {
+    PairArgs base;
+    int n_targets;        // real targets (padding is left out of the spheres)
+    int n_sources;        // real sources
+    float4 *src32;        // per source: x - tile centre (float32), weight
+    float *srcc;          // per source: (1 - |x|^2) / 2
+    double4 *tile_box;    // per 256-source tile: centre, radius
+    double4 *group_box;   // per 64 targets: centre, radius
+    double rnf;           // near-field radius (float64 below it)
+    int kahan;            // 1: Kahan-compensated float32 sums
+};
+
+// Cotangent record of one target in the adjoint: position and n = m x x.
+struct __align__(16) AdjRec {
+    double x, y, z, nx, ny, nz;
+};
+
+// One reverse RK4 stage of the adjoint on n active panels.  Row arrays have
+// `stride` entries per row (rows x, y, z, zeta); source and record arrays
+// have n_pad (a multiple of 1024) entries, zero beyond n.
+struct AdjStage {
+    int n, n_pad, stride;
+    double h, omega;
+    double b;              // RK4 weight of this stage's k in the update
+    double c;              // coefficient of this stage's k in the next stage's input
+    const double *lam;     // lambda' (cotangent of the step's output), 4 rows
+    double *ybar;          // in: Ybar of the next stage; out: Ybar of this stage, 4 rows
+    double *lam_acc;       // lambda' + sum of the Ybar so far, 4 rows
+    double *m;             // 3 rows
+    AdjRec *rec;
+    const Source *ys;      // this stage's input as source records {x, zeta A}
+    const double *area;
+    PairPlan plan;
+    double *partial_t;     // target-major pass: 6 rows per chunk
+    double *partial_s;     // source-major pass: 4 rows per chunk
+    int n_chunks_t, n_chunks_s;
+};
+
+hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s);
+
 // Device state of a panel RK4 step.  Targets are [active | passive | particles]
 // in SoA rows of length t_stride.
 struct StepState {
@@ -48,6 +90,8 @@ struct StageParams {
 
 bool plan_supported(int targets_per_lane, int tile);
 hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s);
+hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s);
+hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s);
 hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s);
 hipError_t launch_prepare_step(const StepState &st, hipStream_t s);
 int ke_partial_count(int n_targets);

@@ -157,6 +157,45 @@ int bve_adjoint_forward(bve_ctx *ctx, double dt, int n_steps, int checkpoint_eve
 int bve_adjoint_backward(bve_ctx *ctx, const double *lambda_x, const double *lambda_zeta,
                          double *grad_x0, double *grad_zeta0);
 
+/* ---- ensembles (many independent problems, shared launches) ----------- */
+
+/* No reference counterpart (the reference runs one problem per MPI job).
+ * An ensemble holds n_members panel problems of any sizes and advances all
+ * of them with one persistent pair-sum launch per RK4 stage (work items of
+ * every member, largest first) and one stage-update launch; the steps can
+ * be captured once into a HIP graph and replayed. */
+typedef struct bve_ensemble bve_ensemble;
+int bve_ensemble_create(bve_ensemble **ens, int device, int n_members);
+int bve_ensemble_destroy(bve_ensemble *ens);
+const char *bve_ensemble_last_error(const bve_ensemble *ens);
+/* Omega, smooth and arithmetic of every member (see bve_set_*). */
+int bve_ensemble_set_params(bve_ensemble *ens, double omega, double smooth, int precision,
+                            double nearfield_radius);
+/* The member's panels (as bve_load_panels) and particles (as bve_load_particles). */
+int bve_ensemble_load_member(bve_ensemble *ens, int member, int n_active, const double *x_active,
+                             const double *relvort_active, const double *area_active,
+                             int n_passive, const double *x_passive, int n_particles,
+                             const double *x_particles, const double *absvort);
+/* Launch geometry of one member: targets per lane (1, 2, 4) and sources per
+ * work item (a multiple of 256); 0 keeps the built-in choice.  Set by the
+ * autotuner per problem size. */
+int bve_ensemble_set_member_plan(bve_ensemble *ens, int member, int targets_per_lane, int chunk);
+/* Workgroups of the persistent launch (0 = 4 per CU). */
+int bve_ensemble_set_grid(bve_ensemble *ens, int workgroups);
+/* n_steps RK4 steps of every member; use_graph = 1 captures the steps into
+ * a HIP graph (re-captured when dt or n_steps change) and launches it.
+ * Asynchronous until bve_ensemble_synchronize or a store. */
+int bve_ensemble_advance(bve_ensemble *ens, double dt, int n_steps, int use_graph);
+int bve_ensemble_synchronize(bve_ensemble *ens);
+/* The member's state (any pointer may be NULL); total_ke of the last step. */
+int bve_ensemble_store_member(bve_ensemble *ens, int member, double *x_active,
+                              double *relvort_active, double *x_passive, double *x_particles,
+                              double *relvort_particles, double *total_ke);
+/* HIP events around every pair launch (or every graph launch). */
+int bve_ensemble_set_profiling(bve_ensemble *ens, int on);
+int bve_ensemble_get_profile(bve_ensemble *ens, long long *launches, double *total_ms);
+int bve_ensemble_pairs_per_step(const bve_ensemble *ens, double *pairs);
+
 /* ---- measurement ------------------------------------------------------ */
 
 /* 1 = bracket every pair-kernel launch with HIP events on the context's

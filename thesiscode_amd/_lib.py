@@ -43,6 +43,19 @@ SIGNATURES = {
     "bve_voronoi_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p, _p, _p]),
     "bve_adjoint_forward": (_i, [_p, _d, _i, _i, _p, _p]),
     "bve_adjoint_backward": (_i, [_p, _p, _p, _p, _p]),
+    "bve_ensemble_create": (_i, [ctypes.POINTER(_p), _i, _i]),
+    "bve_ensemble_destroy": (_i, [_p]),
+    "bve_ensemble_last_error": (ctypes.c_char_p, [_p]),
+    "bve_ensemble_set_params": (_i, [_p, _d, _d, _i, _d]),
+    "bve_ensemble_load_member": (_i, [_p, _i, _i, _p, _p, _p, _i, _p, _i, _p, _p]),
+    "bve_ensemble_set_member_plan": (_i, [_p, _i, _i, _i]),
+    "bve_ensemble_set_grid": (_i, [_p, _i]),
+    "bve_ensemble_advance": (_i, [_p, _d, _i, _i]),
+    "bve_ensemble_synchronize": (_i, [_p]),
+    "bve_ensemble_store_member": (_i, [_p, _i, _p, _p, _p, _p, _p, _p]),
+    "bve_ensemble_set_profiling": (_i, [_p, _i]),
+    "bve_ensemble_get_profile": (_i, [_p, ctypes.POINTER(_ll), _dp]),
+    "bve_ensemble_pairs_per_step": (_i, [_p, _dp]),
     "bve_set_profiling": (_i, [_p, _i]),
     "bve_get_profile": (_i, [_p, ctypes.POINTER(_ll), _dp]),
     "bve_pairs_per_step": (_i, [_p, _dp]),
@@ -259,4 +272,97 @@ class Context:
     def pairs_per_step(self):
         v = ctypes.c_double()
         self._check(self._L.bve_pairs_per_step(self._h, ctypes.byref(v)), "bve_pairs_per_step")
+        return v.value
+
+
+class Ensemble:
+    """Many independent panel problems advanced by shared launches."""
+
+    def __init__(self, n_members, device=0):
+        L = load()
+        h = _p()
+        rc = L.bve_ensemble_create(ctypes.byref(h), device, n_members)
+        if rc != BVE_OK:
+            raise BVEError(f"bve_ensemble_create failed ({rc}): "
+                           f"{L.bve_ensemble_last_error(None).decode()}")
+        self._h, self._L = h, L
+        self.n_members = n_members
+        self._sizes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bve_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != BVE_OK:
+            raise BVEError(f"{what} failed ({rc}): {self._L.bve_ensemble_last_error(self._h).decode()}")
+
+    def set_params(self, omega, smooth=0.05, precision=BVE_PRECISION_FP64, nearfield_radius=0.02):
+        self._check(self._L.bve_ensemble_set_params(self._h, omega, smooth, precision,
+                                                    nearfield_radius), "bve_ensemble_set_params")
+
+    def load_member(self, m, active_x, active_relvort, active_area, passive_x, particles_x,
+                    particles_absvort=None):
+        na, npp, npt = len(active_relvort), len(passive_x), len(particles_x)
+        ax, av, ar = _f64(active_x, 3 * na), _f64(active_relvort, na), _f64(active_area, na)
+        px = _f64(passive_x, 3 * npp) if npp else None
+        qx = _f64(particles_x, 3 * npt) if npt else None
+        qa = _f64(particles_absvort, npt) if particles_absvort is not None else None
+        self._check(self._L.bve_ensemble_load_member(self._h, m, na, _ptr(ax), _ptr(av), _ptr(ar),
+                                                     npp, _ptr(px), npt, _ptr(qx), _ptr(qa)),
+                    "bve_ensemble_load_member")
+        self._sizes[m] = (na, npp, npt)
+
+    def set_member_plan(self, m, targets_per_lane=0, chunk=0):
+        self._check(self._L.bve_ensemble_set_member_plan(self._h, m, targets_per_lane, chunk),
+                    "bve_ensemble_set_member_plan")
+
+    def set_grid(self, workgroups):
+        self._check(self._L.bve_ensemble_set_grid(self._h, workgroups), "bve_ensemble_set_grid")
+
+    def advance(self, dt, n_steps, use_graph=True):
+        self._check(self._L.bve_ensemble_advance(self._h, dt, n_steps, int(bool(use_graph))),
+                    "bve_ensemble_advance")
+
+    def synchronize(self):
+        self._check(self._L.bve_ensemble_synchronize(self._h), "bve_ensemble_synchronize")
+
+    def store_member(self, m):
+        na, npp, npt = self._sizes[m]
+        ax, av, px = np.zeros((na, 3)), np.zeros(na), np.zeros((npp, 3))
+        qx, qr, tot = np.zeros((npt, 3)), np.zeros(npt), np.zeros(1)
+        self._check(self._L.bve_ensemble_store_member(self._h, m, _ptr(ax), _ptr(av), _ptr(px),
+                                                      _ptr(qx), None, _ptr(tot)),
+                    "bve_ensemble_store_member")
+        return dict(active_x=ax, active_relvort=av, passive_x=px, particles_x=qx,
+                    total_ke=float(tot[0]))
+
+    def set_profiling(self, on):
+        self._check(self._L.bve_ensemble_set_profiling(self._h, int(bool(on))),
+                    "bve_ensemble_set_profiling")
+
+    def profile(self):
+        n = ctypes.c_longlong()
+        ms = ctypes.c_double()
+        self._check(self._L.bve_ensemble_get_profile(self._h, ctypes.byref(n), ctypes.byref(ms)),
+                    "bve_ensemble_get_profile")
+        return n.value, ms.value
+
+    def pairs_per_step(self):
+        v = ctypes.c_double()
+        self._check(self._L.bve_ensemble_pairs_per_step(self._h, ctypes.byref(v)),
+                    "bve_ensemble_pairs_per_step")
         return v.value

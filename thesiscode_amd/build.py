@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbve.so")
-SOURCES = ["bve_kernels.hip", "bve_mixed.hip", "bve_adjoint.hip", "bve_api.hip"]
+SOURCES = ["bve_kernels.hip", "bve_mixed.hip", "bve_adjoint.hip", "bve_ensemble.hip", "bve_api.hip"]
 HEADERS = ["bve_common.h", "bve_kernels.h"]
 ARCH = os.environ.get("BVE_OFFLOAD_ARCH", "gfx950")
 

@@ -46,6 +46,7 @@ struct bve_ctx {
     double nearfield = 0.02;
     int renormalize = 0;
     int plan_t = 0, plan_tile = 0, plan_wg = 0, plan_recip = 0;
+    int plan_chunk = 0;   // explicit source chunk length (ensemble members), 0 = derived
 
     // problem sizes
     bool panels_loaded = false;
@@ -119,7 +120,10 @@ int choose_targets_per_lane(int n_targets)
     return 1;
 }
 
-PairPlan make_plan(int n_targets, int n_src_pad, int T, int tile, int target_wg, int recip)
+// Launch geometry: target tiles of kBlock*T, and a source chunk length that
+// gives about target_wg work items (or exactly `chunk` sources when > 0).
+PairPlan make_plan(int n_targets, int n_src_pad, int T, int tile, int target_wg, int recip,
+                   int chunk = 0)
 {
     PairPlan p{};
     p.targets_per_lane = T;
@@ -127,7 +131,8 @@ PairPlan make_plan(int n_targets, int n_src_pad, int T, int tile, int target_wg,
     p.recip = recip;
     p.n_tiles = std::max(1, ceil_div(n_targets, kBlock * T));
     const int want_chunks = std::max(1, ceil_div(target_wg, p.n_tiles));
-    p.chunk = round_up(std::max(tile, ceil_div(n_src_pad, want_chunks)), tile);
+    p.chunk = chunk > 0 ? round_up(chunk, tile)
+                        : round_up(std::max(tile, ceil_div(n_src_pad, want_chunks)), tile);
     p.n_chunks = std::max(1, ceil_div(n_src_pad, p.chunk));
     p.n_work = p.n_tiles * p.n_chunks;
     p.grid = round_up(p.n_work, 8);
@@ -188,7 +193,7 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
     // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
     const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
     const int wg = c->plan_wg ? c->plan_wg : 4096;
-    return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c));
+    return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
 }
 
 int update_plan(bve_ctx *c)
@@ -321,6 +326,41 @@ int ensure_mixed(bve_ctx *c, int n_src_pad, int t_stride)
     return BVE_OK;
 }
 
+// Mixed-pass arguments for the float64 pass `a` over n_targets / n_sources.
+MixedArgs mixed_args_of(const bve_ctx *c, const PairArgs &a, int n_targets, int n_sources)
+{
+    MixedArgs ma{};
+    ma.base = a;
+    ma.n_targets = n_targets;
+    ma.n_sources = n_sources;
+    ma.src32 = c->src32;
+    ma.srcc = c->srcc;
+    ma.tile_box = c->tile_box;
+    ma.group_box = c->group_box;
+    ma.rnf = c->nearfield;
+    ma.kahan = c->precision == BVE_PRECISION_MIXED_KAHAN;
+    return ma;
+}
+
+// Pair-pass arguments of the context's resident state with launch plan `plan`.
+PairArgs pair_args_of(const bve_ctx *c, const PairPlan &plan)
+{
+    PairArgs pa{};
+    pa.tx = c->xs;
+    pa.ty = c->xs + c->t_stride;
+    pa.tz = c->xs + 2 * (size_t)c->t_stride;
+    pa.t_stride = c->t_stride;
+    pa.cls.self_end = c->n_active;
+    pa.cls.smooth_begin = c->n_active;
+    pa.cls.smooth_end = c->n_active + c->n_passive;
+    pa.cls.eps2 = c->smooth * c->smooth;
+    pa.src = c->src;
+    pa.n_src_pad = c->n_src_pad;
+    pa.plan = plan;
+    pa.partial = c->partial;
+    return pa;
+}
+
 // One pair-sum pass over n_targets targets and n_sources sources in the
 // context's arithmetic (the mixed pass first builds its per-stage records).
 int pair_pass(bve_ctx *c, const PairArgs &a, int n_targets, int n_sources)
@@ -329,15 +369,7 @@ int pair_pass(bve_ctx *c, const PairArgs &a, int n_targets, int n_sources)
     if (is_mixed(c)) {
         int rc = ensure_mixed(c, a.n_src_pad, a.t_stride);
         if (rc) return rc;
-        ma.base = a;
-        ma.n_targets = n_targets;
-        ma.n_sources = n_sources;
-        ma.src32 = c->src32;
-        ma.srcc = c->srcc;
-        ma.tile_box = c->tile_box;
-        ma.group_box = c->group_box;
-        ma.rnf = c->nearfield;
-        ma.kahan = c->precision == BVE_PRECISION_MIXED_KAHAN;
+        ma = mixed_args_of(c, a, n_targets, n_sources);
         HIP_TRY(c, launch_mixed_prep(ma, c->stream));
     }
     std::pair<hipEvent_t, hipEvent_t> ev{};
@@ -365,19 +397,7 @@ int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPl
 {
     StepState st = step_state(c);
     st.n_targets = n_targets;
-    PairArgs pa{};
-    pa.tx = c->xs;
-    pa.ty = c->xs + c->t_stride;
-    pa.tz = c->xs + 2 * (size_t)c->t_stride;
-    pa.t_stride = c->t_stride;
-    pa.cls.self_end = c->n_active;
-    pa.cls.smooth_begin = c->n_active;
-    pa.cls.smooth_end = c->n_active + c->n_passive;
-    pa.cls.eps2 = c->smooth * c->smooth;
-    pa.src = c->src;
-    pa.n_src_pad = c->n_src_pad;
-    pa.plan = plan;
-    pa.partial = c->partial;
+    const PairArgs pa = pair_args_of(c, plan);
     static const double a_next[4] = {0.5, 0.5, 1.0, 0.0};
     static const double divisor[4] = {6.0, 3.0, 3.0, 6.0};
     HIP_TRY(c, launch_prepare_step(st, c->stream));
@@ -1068,6 +1088,450 @@ int bve_pairs_per_step(const bve_ctx *c, double *pairs)
     if (!c || !pairs) return BVE_ERR_ARGUMENT;
     const double na = c->n_active, other = (double)c->n_passive + (double)std::max(c->n_particles, 0);
     *pairs = 4.0 * (na * (na - 1.0) + other * na);
+    return BVE_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Ensembles: many independent problems advanced by shared launches
+// ===========================================================================
+
+struct bve_ensemble {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    std::vector<bve_ctx *> members;
+    int grid_wg = 0;   // persistent workgroups, 0 = 4 per CU
+    int n_cu = 256;
+    int precision = BVE_PRECISION_FP64;
+
+    // device tables (rebuilt when a member or a parameter changes)
+    bool dirty = true;
+    EnsItem *d_items = nullptr;
+    PairArgs *d_pair = nullptr;
+    MixedArgs *d_mixed = nullptr;
+    StepState *d_states = nullptr;
+    int *d_chunks = nullptr;
+    int2 *d_stage_map = nullptr, *d_src_map = nullptr, *d_tgt_map = nullptr;
+    EnsLaunch launch{};
+
+    // captured steps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double g_dt = 0.0;
+    int g_steps = 0;
+
+    // profiling: HIP events around pair launches (or graph launches)
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending, pool;
+    long long prof_launches = 0;
+    double prof_ms = 0.0;
+
+    int fail(int code, const char *fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        error = buf;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char *what)
+    {
+        return fail(e == hipErrorOutOfMemory ? BVE_ERR_MEMORY : BVE_ERR_DEVICE, "%s: %s", what,
+                    hipGetErrorString(e));
+    }
+};
+
+namespace {
+
+void ens_free_tables(bve_ensemble *e)
+{
+    dev_free(e->d_items);
+    dev_free(e->d_pair);
+    dev_free(e->d_mixed);
+    dev_free(e->d_states);
+    dev_free(e->d_chunks);
+    dev_free(e->d_stage_map);
+    dev_free(e->d_src_map);
+    dev_free(e->d_tgt_map);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    e->dirty = true;
+}
+
+template <class P>
+int ens_upload(bve_ensemble *e, P *&dst, const std::vector<P> &v)
+{
+    dev_free(dst);
+    if (v.empty()) return BVE_OK;
+    HIP_TRY(e, dev_alloc(&dst, v.size()));
+    HIP_TRY(e, hipMemcpyAsync(dst, v.data(), sizeof(P) * v.size(), hipMemcpyHostToDevice, e->stream));
+    return BVE_OK;
+}
+
+int ens_build(bve_ensemble *e)
+{
+    if (!e->dirty) return BVE_OK;
+    ens_free_tables(e);
+    const bool mixed = e->precision == BVE_PRECISION_MIXED ||
+                       e->precision == BVE_PRECISION_MIXED_KAHAN;
+    std::vector<PairArgs> pairs;
+    std::vector<MixedArgs> mixeds;
+    std::vector<StepState> states;
+    std::vector<int> chunks;
+    std::vector<std::pair<long long, EnsItem>> costed;
+    std::vector<int2> stage_map, src_map, tgt_map;
+    for (int m = 0; m < (int)e->members.size(); ++m) {
+        bve_ctx *c = e->members[m];
+        if (!c->panels_loaded || !c->x0)
+            return e->fail(BVE_ERR_STATE, "ensemble member %d has no panels loaded", m);
+        // the persistent kernel runs 256-source tiles
+        if (c->plan_tile != 256 || c->plan.tile != 256) {
+            c->plan_tile = 256;
+            int rc = update_plan(c);
+            if (rc) return e->fail(rc, "member %d: %s", m, c->error.c_str());
+        }
+        if (mixed) {
+            int rc = ensure_mixed(c, c->n_src_pad, c->t_stride);
+            if (rc) return e->fail(rc, "member %d: %s", m, c->error.c_str());
+        }
+        const PairArgs pa = pair_args_of(c, c->plan);
+        pairs.push_back(pa);
+        mixeds.push_back(mixed_args_of(c, pa, c->n_targets, c->n_active));
+        states.push_back(step_state(c));
+        chunks.push_back(c->plan.n_chunks);
+        const long long cost = (long long)kBlock * c->plan.targets_per_lane * c->plan.chunk;
+        for (int ch = 0; ch < c->plan.n_chunks; ++ch)
+            for (int t = 0; t < c->plan.n_tiles; ++t) costed.push_back({cost, EnsItem{m, ch, t}});
+        for (int b = 0; b < ceil_div(c->n_targets, 256); ++b) stage_map.push_back(make_int2(m, b));
+        for (int b = 0; b < c->n_src_pad / 256; ++b) src_map.push_back(make_int2(m, b));
+        for (int b = 0; b < c->t_stride / 256; ++b) tgt_map.push_back(make_int2(m, b));
+    }
+    // largest items first; the persistent grid deals them out round-robin
+    std::stable_sort(costed.begin(), costed.end(),
+                     [](const auto &a, const auto &b) { return a.first > b.first; });
+    std::vector<EnsItem> items;
+    items.reserve(costed.size());
+    for (auto &ci : costed) items.push_back(ci.second);
+    int rc = ens_upload(e, e->d_items, items);
+    if (!rc) rc = ens_upload(e, e->d_pair, pairs);
+    if (!rc) rc = ens_upload(e, e->d_mixed, mixeds);
+    if (!rc) rc = ens_upload(e, e->d_states, states);
+    if (!rc) rc = ens_upload(e, e->d_chunks, chunks);
+    if (!rc) rc = ens_upload(e, e->d_stage_map, stage_map);
+    if (!rc) rc = ens_upload(e, e->d_src_map, src_map);
+    if (!rc) rc = ens_upload(e, e->d_tgt_map, tgt_map);
+    if (rc) return rc;
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    EnsLaunch &L = e->launch;
+    L = EnsLaunch{};
+    L.n_members = (int)e->members.size();
+    L.grid = std::max(1, std::min((int)items.size(), e->grid_wg ? e->grid_wg : 4 * e->n_cu));
+    L.items = e->d_items;
+    L.n_items = (int)items.size();
+    L.pair_members = e->d_pair;
+    L.mixed_members = e->d_mixed;
+    L.mixed = mixed;
+    L.kahan = e->precision == BVE_PRECISION_MIXED_KAHAN;
+    L.recip = e->precision == BVE_PRECISION_FP64_FAST ? 4 : 1;
+    L.states = e->d_states;
+    L.n_chunks = e->d_chunks;
+    L.stage_map = e->d_stage_map;
+    L.n_stage_blocks = (int)stage_map.size();
+    L.src_map = e->d_src_map;
+    L.n_src_blocks = (int)src_map.size();
+    L.tgt_map = e->d_tgt_map;
+    L.n_tgt_blocks = (int)tgt_map.size();
+    e->dirty = false;
+    return BVE_OK;
+}
+
+std::pair<hipEvent_t, hipEvent_t> ens_events(bve_ensemble *e)
+{
+    if (!e->pool.empty()) {
+        auto ev = e->pool.back();
+        e->pool.pop_back();
+        return ev;
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    return {a, b};
+}
+
+void ens_collect(bve_ensemble *e)
+{
+    for (auto &ev : e->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+            e->prof_ms += ms;
+            e->prof_launches += 1;
+        }
+        e->pool.push_back(ev);
+    }
+    e->pending.clear();
+}
+
+// Enqueues n_steps RK4 steps of every member (timed pair launches unless
+// the steps are being captured into a graph).
+int ens_enqueue(bve_ensemble *e, double dt, int n_steps, bool timed)
+{
+    static const double a_next[4] = {0.5, 0.5, 1.0, 0.0};
+    static const double divisor[4] = {6.0, 3.0, 3.0, 6.0};
+    const EnsLaunch &L = e->launch;
+    const double omega = e->members.empty() ? 0.0 : e->members[0]->omega;
+    HIP_TRY(e, launch_ensemble_prepare(L, e->stream));
+    for (int n = 0; n < n_steps; ++n) {
+        for (int s = 0; s < 4; ++s) {
+            if (L.mixed) HIP_TRY(e, launch_ensemble_mixed_prep(L, e->stream));
+            std::pair<hipEvent_t, hipEvent_t> ev{};
+            if (timed) {
+                ev = ens_events(e);
+                HIP_TRY(e, hipEventRecord(ev.first, e->stream));
+            }
+            HIP_TRY(e, launch_ensemble_pairs(L, e->stream));
+            if (timed) {
+                HIP_TRY(e, hipEventRecord(ev.second, e->stream));
+                e->pending.push_back(ev);
+            }
+            StageParams sp{};
+            sp.stage = s;
+            sp.dt = dt;
+            sp.omega = omega;
+            sp.a_next = a_next[s];
+            sp.divisor = divisor[s];
+            sp.renormalize = e->members.empty() ? 0 : e->members[0]->renormalize;
+            HIP_TRY(e, launch_ensemble_stage(L, sp, e->stream));
+        }
+    }
+    return BVE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bve_ensemble_create(bve_ensemble **out, int device, int n_members)
+{
+    if (!out || n_members < 1) {
+        g_create_error = "bve_ensemble_create: bad arguments";
+        return BVE_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    bve_ensemble *e = new bve_ensemble;
+    e->device = device;
+    for (int m = 0; m < n_members; ++m) {
+        bve_ctx *c = nullptr;
+        const int rc = bve_create(&c, device);
+        if (rc) {
+            for (bve_ctx *d : e->members) bve_destroy(d);
+            delete e;
+            return rc;
+        }
+        e->members.push_back(c);
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        e->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        for (bve_ctx *d : e->members) bve_destroy(d);
+        delete e;
+        g_create_error = "bve_ensemble_create: hipStreamCreate failed";
+        return BVE_ERR_DEVICE;
+    }
+    *out = e;
+    return BVE_OK;
+}
+
+int bve_ensemble_destroy(bve_ensemble *e)
+{
+    if (!e) return BVE_OK;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    ens_free_tables(e);
+    for (auto &ev : e->pending) e->pool.push_back(ev);
+    for (auto &ev : e->pool) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    for (bve_ctx *c : e->members) bve_destroy(c);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return BVE_OK;
+}
+
+const char *bve_ensemble_last_error(const bve_ensemble *e)
+{
+    return e ? e->error.c_str() : g_create_error.c_str();
+}
+
+int bve_ensemble_set_params(bve_ensemble *e, double omega, double smooth, int precision,
+                            double nearfield_radius)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    for (bve_ctx *c : e->members) {
+        int rc = bve_set_omega(c, omega);
+        if (!rc) rc = bve_set_smooth(c, smooth);
+        if (!rc) rc = bve_set_precision(c, precision, nearfield_radius);
+        if (rc) return e->fail(rc, "%s", c->error.c_str());
+    }
+    e->precision = precision;
+    e->dirty = true;
+    ens_free_tables(e);
+    return BVE_OK;
+}
+
+int bve_ensemble_load_member(bve_ensemble *e, int member, int n_active, const double *x_active,
+                             const double *relvort, const double *area, int n_passive,
+                             const double *x_passive, int n_particles, const double *x_particles,
+                             const double *absvort)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    if (member < 0 || member >= (int)e->members.size())
+        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_load_member: member %d out of range", member);
+    bve_ctx *c = e->members[member];
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    ens_free_tables(e);
+    int rc = bve_load_panels(c, n_active, x_active, relvort, area, n_passive, x_passive);
+    if (!rc) rc = bve_load_particles(c, n_particles, x_particles, absvort);
+    if (rc) return e->fail(rc, "member %d: %s", member, c->error.c_str());
+    return BVE_OK;
+}
+
+int bve_ensemble_set_member_plan(bve_ensemble *e, int member, int targets_per_lane, int chunk)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    if (member < 0 || member >= (int)e->members.size())
+        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_set_member_plan: member %d out of range", member);
+    if (!(targets_per_lane == 0 || targets_per_lane == 1 || targets_per_lane == 2 ||
+          targets_per_lane == 4) || chunk < 0)
+        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_set_member_plan: unsupported plan (T=%d, chunk=%d)",
+                       targets_per_lane, chunk);
+    bve_ctx *c = e->members[member];
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    ens_free_tables(e);
+    c->plan_t = targets_per_lane;
+    c->plan_tile = 256;
+    c->plan_chunk = chunk;
+    if (c->panels_loaded && c->x0) {
+        const int rc = update_plan(c);
+        if (rc) return e->fail(rc, "member %d: %s", member, c->error.c_str());
+    }
+    return BVE_OK;
+}
+
+int bve_ensemble_set_grid(bve_ensemble *e, int workgroups)
+{
+    if (!e || workgroups < 0) return BVE_ERR_ARGUMENT;
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    ens_free_tables(e);
+    e->grid_wg = workgroups;
+    return BVE_OK;
+}
+
+int bve_ensemble_advance(bve_ensemble *e, double dt, int n_steps, int use_graph)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    if (n_steps < 0) return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_advance: n_steps < 0");
+    HIP_TRY(e, hipSetDevice(e->device));
+    int rc = ens_build(e);
+    if (rc) return rc;
+    if (!use_graph) return ens_enqueue(e, dt, n_steps, e->profiling);
+    if (!e->gexec || e->g_dt != dt || e->g_steps != n_steps) {
+        if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+        if (e->graph) (void)hipGraphDestroy(e->graph);
+        e->gexec = nullptr;
+        e->graph = nullptr;
+        HIP_TRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        rc = ens_enqueue(e, dt, n_steps, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return e->hip_fail(ec, "hipStreamEndCapture");
+        e->graph = g;
+        HIP_TRY(e, hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+        e->g_dt = dt;
+        e->g_steps = n_steps;
+    }
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (e->profiling) {
+        ev = ens_events(e);
+        HIP_TRY(e, hipEventRecord(ev.first, e->stream));
+    }
+    HIP_TRY(e, hipGraphLaunch(e->gexec, e->stream));
+    if (e->profiling) {
+        HIP_TRY(e, hipEventRecord(ev.second, e->stream));
+        e->pending.push_back(ev);
+    }
+    return BVE_OK;
+}
+
+int bve_ensemble_synchronize(bve_ensemble *e)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    HIP_TRY(e, hipSetDevice(e->device));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    ens_collect(e);
+    return BVE_OK;
+}
+
+int bve_ensemble_store_member(bve_ensemble *e, int member, double *x_active, double *relvort,
+                              double *x_passive, double *x_particles, double *relvort_particles,
+                              double *total_ke)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    if (member < 0 || member >= (int)e->members.size())
+        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_store_member: member %d out of range", member);
+    int rc = bve_ensemble_synchronize(e);
+    if (rc) return rc;
+    bve_ctx *c = e->members[member];
+    rc = bve_store_panels(c, x_active, relvort, x_passive);
+    if (!rc && (x_particles || relvort_particles))
+        rc = bve_store_particles(c, x_particles, relvort_particles);
+    if (!rc && total_ke) rc = bve_get_kinetic_energy(c, total_ke, nullptr);
+    if (rc) return e->fail(rc, "member %d: %s", member, c->error.c_str());
+    return BVE_OK;
+}
+
+int bve_ensemble_set_profiling(bve_ensemble *e, int on)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    int rc = bve_ensemble_synchronize(e);
+    if (rc) return rc;
+    e->profiling = on != 0;
+    e->prof_launches = 0;
+    e->prof_ms = 0.0;
+    return BVE_OK;
+}
+
+int bve_ensemble_get_profile(bve_ensemble *e, long long *launches, double *total_ms)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    int rc = bve_ensemble_synchronize(e);
+    if (rc) return rc;
+    if (launches) *launches = e->prof_launches;
+    if (total_ms) *total_ms = e->prof_ms;
+    return BVE_OK;
+}
+
+int bve_ensemble_pairs_per_step(const bve_ensemble *e, double *pairs)
+{
+    if (!e || !pairs) return BVE_ERR_ARGUMENT;
+    double total = 0.0;
+    for (const bve_ctx *c : e->members) {
+        double p = 0.0;
+        bve_pairs_per_step(c, &p);
+        total += p;
+    }
+    *pairs = total;
     return BVE_OK;
 }
 

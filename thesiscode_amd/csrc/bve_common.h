@@ -42,7 +42,7 @@ struct PairPlan {
     int grid;               // launched workgroups (n_work rounded up to 8 XCDs)
 };
 
-inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
-inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace bve

@@ -1,0 +1,562 @@
+// Device-side bodies of the Biot-Savart pair sums and the RK4 stage update,
+// shared by the per-context launches and the persistent ensemble launch.
+//
+// The velocity at a target x_i is (PanelVelocityParallel.f90:1287-1412)
+//
+//     u_i = -1/(4 PI) sum_j w_j (x_i x x_j) / d_ij,   d_ij = (1 - x_i.x_j) + eps2,
+//
+// with w_j = zeta_j A_j.  The float64 body evaluates the factored form
+// S_i = sum_j (w_j / d_ij) x_j and the stage body forms u_i = (x_i x S_i) / (-4 PI),
+// which saves the per-pair cross product.  Per pair that is 3 FMA for the
+// denominator, the reciprocal (below), one multiply by w and 3 FMA into S.
+//
+// Work decomposition: a work item is (target tile, source chunk).  Targets
+// are held in registers (T per lane), sources stream through LDS in tiles
+// of AoS {x, y, z, w} records, and each item writes its own partial sums,
+// which the stage body adds in a fixed order: results do not depend on
+// scheduling and reruns are bitwise reproducible.
+#pragma once
+#include "bve_common.h"
+#include "bve_kernels.h"
+
+namespace bve {
+
+// ---------------------------------------------------------------------------
+// Reciprocals
+// ---------------------------------------------------------------------------
+
+// 1/d: v_rcp_f64 seed (accurate to about 2^-24) refined by two Newton
+// steps r <- r + r (1 - d r): correctly rounded to within an ulp or two.
+__device__ __forceinline__ double recip_f64(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// 1/p with one Newton step: relative error about 2^-48.
+__device__ __forceinline__ double recip1_f64(double p)
+{
+    const double r = __builtin_amdgcn_rcp(p);
+    const double e = __builtin_fma(-p, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// Maps workgroup g of a grid of `grid` workgroups (a multiple of 8) to a
+// work item so that XCD (g % 8) runs a contiguous run of the chunk-major work
+// list: the source chunks an XCD reads stay in its L2.
+__device__ __forceinline__ int xcd_work_item(int g, int grid)
+{
+    return (g % 8) * (grid / 8) + g / 8;
+}
+
+// Wave-uniform: does source tile [jt, jt + tile) hold the own index of one
+// of the 64 lanes whose targets start at lo (targets [0, self_end) are
+// self-excluded)?
+__device__ __forceinline__ bool diag_tile(int lo, int self_end, int jt, int tile)
+{
+    const int hi = min(lo + 64, self_end);
+    return (lo < hi) && (lo < jt + tile) && (jt < hi);
+}
+
+// ---------------------------------------------------------------------------
+// float64 pair body
+// ---------------------------------------------------------------------------
+
+// One LDS tile of sources against the T targets of this lane.
+//
+// R = 1: every pair gets its own reciprocal (recip_f64).
+// R = 2, 4: R consecutive sources share one reciprocal of the product of
+// their denominators (1/d_a = d_b / (d_a d_b)), refined by one Newton step,
+// which trades R-1 v_rcp_f64 and Newton steps for 2(R-1) multiplies.
+// Denominators lie in [~1e-6, 2 + eps2], so products of four stay far from
+// float64 underflow and overflow.
+//
+// MASK: the tile holds the own-index source of some target of this wave;
+// that pair's denominator (~0) is replaced by 1 and its weight by 0.
+template <int T, int TILE, int R, bool MASK>
+__device__ __forceinline__ void tile_f64(const Source *__restrict__ lds, int jt,
+                                         const double (&nx)[T], const double (&ny)[T],
+                                         const double (&nz)[T], const double (&c0)[T],
+                                         const int (&self)[T], double (&sx)[T],
+                                         double (&sy)[T], double (&sz)[T])
+{
+#pragma unroll(R == 1 ? 2 : 1)
+    for (int k = 0; k < TILE; k += R) {
+        Source s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = lds[k + r];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            double d[R], q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                d[r] = __builtin_fma(nx[t], s[r].x, c0[t]);
+                d[r] = __builtin_fma(ny[t], s[r].y, d[r]);
+                d[r] = __builtin_fma(nz[t], s[r].z, d[r]);
+                if (MASK) d[r] = (jt + k + r == self[t]) ? 1.0 : d[r];
+            }
+            if constexpr (R == 1) {
+                q[0] = s[0].w * recip_f64(d[0]);
+            } else if constexpr (R == 2) {
+                const double rr = recip1_f64(d[0] * d[1]);
+                q[0] = s[0].w * d[1] * rr;
+                q[1] = s[1].w * d[0] * rr;
+            } else {
+                const double p01 = d[0] * d[1], p23 = d[2] * d[3];
+                const double rr = recip1_f64(p01 * p23);
+                const double r01 = rr * p23, r23 = rr * p01;
+                q[0] = s[0].w * d[1] * r01;
+                q[1] = s[1].w * d[0] * r01;
+                q[2] = s[2].w * d[3] * r23;
+                q[3] = s[3].w * d[2] * r23;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (MASK) q[r] = (jt + k + r == self[t]) ? 0.0 : q[r];
+                sx[t] = __builtin_fma(q[r], s[r].x, sx[t]);
+                sy[t] = __builtin_fma(q[r], s[r].y, sy[t]);
+                sz[t] = __builtin_fma(q[r], s[r].z, sz[t]);
+            }
+        }
+    }
+}
+
+// Work item (target tile ttile, source chunk `chunk`) of a float64 pass;
+// lds holds TILE records.
+template <int T, int TILE, int R>
+__device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int ttile,
+                                              Source *lds)
+{
+    const int tile_base = ttile * (kBlock * T);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int stride = a.t_stride;
+    const TargetClasses cls = a.cls;
+
+    double nx[T], ny[T], nz[T], c0[T], sx[T], sy[T], sz[T];
+    int self[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        nx[t] = -a.tx[i];
+        ny[t] = -a.ty[i];
+        nz[t] = -a.tz[i];
+        c0[t] = (i >= cls.smooth_begin && i < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
+        self[t] = (i < cls.self_end) ? i : -1;
+        sx[t] = sy[t] = sz[t] = 0.0;
+    }
+
+    const int j_begin = chunk * a.plan.chunk;
+    const int j_end = min(j_begin + a.plan.chunk, a.n_src_pad);
+    for (int jt = j_begin; jt < j_end; jt += TILE) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < TILE; k += kBlock) lds[k] = a.src[jt + k];
+        __syncthreads();
+        bool diag = false;
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+            diag |= diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, TILE);
+        if (diag)
+            tile_f64<T, TILE, R, true>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
+        else
+            tile_f64<T, TILE, R, false>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
+    }
+
+    double *px = a.partial + (size_t)(chunk * 3) * stride;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        px[i] = sx[t];
+        px[stride + i] = sy[t];
+        px[2 * (size_t)stride + i] = sz[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision pair body (float32 far field, float64 near field)
+// ---------------------------------------------------------------------------
+//
+// The sum is written in difference form,
+//
+//     S_i = sum_j q_ij (x_j - x_i),  q_ij = w_j / D_ij,
+//     D_ij = |x_i - x_j|^2 / 2 + c_i + c_j (+ eps2),  c = (1 - |x|^2) / 2,
+//
+// which is 1 - x_i.x_j (+ eps2) exactly, without its cancellation for close
+// pairs, and x_i x S_i is the same cross product as in the float64 body
+// (x_i x x_i = 0).  Float32 differences are formed from per-tile float64
+// origins: every source tile (256 sources) and every group of 64 targets
+// has a float64 centre c; records hold o = fl32(x - c), and a pair's
+// difference is (fl32(c_g - c_s) + o_i) - o_j, so its rounding error scales
+// with |x_i - x_j| plus the tile sizes, not with |x| = 1.
+//
+// Pairs closer than the near-field radius are evaluated in float64 from
+// the float64 source records.  A tile pair is sent down the careful path
+// (per-pair distance test) only when the bounding spheres of the target
+// group and the source tile come within the radius, or when the tile holds
+// a target's own index; all other tiles run the float32 loop.  Float32
+// partial sums are carried into float64 accumulators after every tile, or
+// after every pair with Kahan compensation when requested.
+
+constexpr int kMixedTile = 256;   // sources per LDS tile and per source bounding sphere
+constexpr int kGroup = 64;        // targets per bounding sphere (one wave's lanes)
+
+template <bool KAHAN>
+__device__ __forceinline__ void add_f32(float &acc, float &comp, float term)
+{
+    if (KAHAN) {
+        const float y = term - comp;
+        const float t = acc + y;
+        comp = (t - acc) - y;
+        acc = t;
+    } else {
+        acc += term;
+    }
+}
+
+template <int T, bool KAHAN>
+__device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, int ttile,
+                                                float4 *lds, float *ldsc)
+{
+    constexpr int TILE = kMixedTile;
+    const PairPlan &plan = a.base.plan;
+    const int tile_base = ttile * (kBlock * T);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int stride = a.base.t_stride;
+    const TargetClasses cls = a.base.cls;
+
+    double xi[T], yi[T], zi[T], ci[T], gx[T], gy[T], gz[T], gr[T];
+    double S64x[T], S64y[T], S64z[T];
+    float oix[T], oiy[T], oiz[T], cif[T];
+    int self[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        xi[t] = a.base.tx[i];
+        yi[t] = a.base.ty[i];
+        zi[t] = a.base.tz[i];
+        const bool smooth = i >= cls.smooth_begin && i < cls.smooth_end;
+        ci[t] = 0.5 * (1.0 - (xi[t] * xi[t] + yi[t] * yi[t] + zi[t] * zi[t])) +
+                (smooth ? cls.eps2 : 0.0);
+        cif[t] = (float)ci[t];
+        self[t] = (i < cls.self_end) ? i : -1;
+        const double4 gb = a.group_box[(tile_base + t * kBlock) / kGroup + wave];
+        gx[t] = gb.x;
+        gy[t] = gb.y;
+        gz[t] = gb.z;
+        gr[t] = gb.w;
+        oix[t] = (float)(xi[t] - gx[t]);
+        oiy[t] = (float)(yi[t] - gy[t]);
+        oiz[t] = (float)(zi[t] - gz[t]);
+        S64x[t] = S64y[t] = S64z[t] = 0.0;
+    }
+    const float rnf2 = (float)(a.rnf * a.rnf);
+
+    const int j_begin = chunk * plan.chunk;
+    const int j_end = min(j_begin + plan.chunk, a.base.n_src_pad);
+    for (int jt = j_begin; jt < j_end; jt += TILE) {
+        __syncthreads();
+        lds[threadIdx.x] = a.src32[jt + threadIdx.x];
+        ldsc[threadIdx.x] = a.srcc[jt + threadIdx.x];
+        __syncthreads();
+        const double4 sb = a.tile_box[jt / TILE];
+        float ax[T], ay[T], az[T];
+        bool careful = false;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const double dx = gx[t] - sb.x, dy = gy[t] - sb.y, dz = gz[t] - sb.z;
+            ax[t] = (float)dx + oix[t];
+            ay[t] = (float)dy + oiy[t];
+            az[t] = (float)dz + oiz[t];
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            // 1e-6: slack for the float32 rounding of the per-pair test below
+            careful |= sb.w >= 0.0 && dist < gr[t] + sb.w + a.rnf + 1e-6;
+            careful |= diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, TILE);
+        }
+        float Sx[T], Sy[T], Sz[T], Cx[T], Cy[T], Cz[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) Sx[t] = Sy[t] = Sz[t] = Cx[t] = Cy[t] = Cz[t] = 0.f;
+        if (!careful) {
+#pragma unroll 2
+            for (int k = 0; k < TILE; ++k) {
+                const float4 s = lds[k];
+                const float cj = ldsc[k];
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const float dx = ax[t] - s.x, dy = ay[t] - s.y, dz = az[t] - s.z;
+                    const float h = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    const float D = __builtin_fmaf(0.5f, h, cif[t] + cj);
+                    const float q = s.w * __builtin_amdgcn_rcpf(D);
+                    add_f32<KAHAN>(Sx[t], Cx[t], -q * dx);
+                    add_f32<KAHAN>(Sy[t], Cy[t], -q * dy);
+                    add_f32<KAHAN>(Sz[t], Cz[t], -q * dz);
+                }
+            }
+        } else {
+            for (int k = 0; k < TILE; ++k) {
+                const float4 s = lds[k];
+                const float cj = ldsc[k];
+                const int j = jt + k;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const float dx = ax[t] - s.x, dy = ay[t] - s.y, dz = az[t] - s.z;
+                    const float h = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    if (j == self[t]) continue;
+                    if (h < rnf2) {
+                        // near field: float64 from the float64 record
+                        const Source sj = a.base.src[j];
+                        const double ex = sj.x - xi[t], ey = sj.y - yi[t], ez = sj.z - zi[t];
+                        const double cj64 = 0.5 * (1.0 - (sj.x * sj.x + sj.y * sj.y + sj.z * sj.z));
+                        const double D = 0.5 * (ex * ex + ey * ey + ez * ez) + (ci[t] + cj64);
+                        const double q = sj.w / D;
+                        S64x[t] = __builtin_fma(q, ex, S64x[t]);
+                        S64y[t] = __builtin_fma(q, ey, S64y[t]);
+                        S64z[t] = __builtin_fma(q, ez, S64z[t]);
+                    } else {
+                        const float D = __builtin_fmaf(0.5f, h, cif[t] + cj);
+                        const float q = s.w * __builtin_amdgcn_rcpf(D);
+                        add_f32<KAHAN>(Sx[t], Cx[t], -q * dx);
+                        add_f32<KAHAN>(Sy[t], Cy[t], -q * dy);
+                        add_f32<KAHAN>(Sz[t], Cz[t], -q * dz);
+                    }
+                }
+            }
+        }
+        // float64 carry of the tile's float32 sums
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            S64x[t] += (double)Sx[t] - (double)Cx[t];
+            S64y[t] += (double)Sy[t] - (double)Cy[t];
+            S64z[t] += (double)Sz[t] - (double)Cz[t];
+        }
+    }
+
+    double *px = a.base.partial + (size_t)(chunk * 3) * stride;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        px[i] = S64x[t];
+        px[stride + i] = S64y[t];
+        px[2 * (size_t)stride + i] = S64z[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (fixed order: deterministic)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision per-stage preparation bodies
+// ---------------------------------------------------------------------------
+
+// Source tile `tile` (kMixedTile threads): centre (mean of the valid
+// sources), radius, and the float32 records {x - c, w} and c_j.
+__device__ __forceinline__ void mixed_source_prep_body(const Source *__restrict__ src,
+                                                       int n_sources, float4 *__restrict__ src32,
+                                                       float *__restrict__ srcc,
+                                                       double4 *__restrict__ tile_box, int tile,
+                                                       double *sh)
+{
+    const int j = tile * kMixedTile + threadIdx.x;
+    const Source s = src[j];
+    const bool valid = j < n_sources;
+    const double cnt = block_sum(valid ? 1.0 : 0.0, sh);
+    const double mx = block_sum(valid ? s.x : 0.0, sh);
+    const double my = block_sum(valid ? s.y : 0.0, sh);
+    const double mz = block_sum(valid ? s.z : 0.0, sh);
+    const double inv = cnt > 0 ? 1.0 / cnt : 0.0;
+    const double cx = mx * inv, cy = my * inv, cz = mz * inv;
+    const double dx = s.x - cx, dy = s.y - cy, dz = s.z - cz;
+    const double r = block_max(valid ? sqrt(dx * dx + dy * dy + dz * dz) : -1.0, sh);
+    if (threadIdx.x == 0) tile_box[tile] = make_double4(cx, cy, cz, r);
+    src32[j] = make_float4((float)dx, (float)dy, (float)dz, (float)s.w);
+    srcc[j] = (float)(0.5 * (1.0 - (s.x * s.x + s.y * s.y + s.z * s.z)));
+}
+
+// Targets [blk * 256, blk * 256 + 256), one wave per group of kGroup:
+// centre and radius of each group.
+__device__ __forceinline__ void mixed_target_prep_body(const double *__restrict__ t, int stride,
+                                                       int n_targets,
+                                                       double4 *__restrict__ group_box, int blk,
+                                                       double *sh)
+{
+    const int i = blk * 256 + threadIdx.x;
+    const bool valid = i < n_targets;
+    const double x = t[i], y = t[stride + i], z = t[2 * (size_t)stride + i];
+    const int lane = threadIdx.x & 63, base = threadIdx.x & ~63;
+    double v[4] = {valid ? 1.0 : 0.0, valid ? x : 0.0, valid ? y : 0.0, valid ? z : 0.0};
+    double m[4];
+    for (int q = 0; q < 4; ++q) {
+        sh[threadIdx.x] = v[q];
+        __syncthreads();
+        for (int s = 32; s > 0; s >>= 1) {
+            if (lane < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        m[q] = sh[base];
+        __syncthreads();
+    }
+    const double inv = m[0] > 0 ? 1.0 / m[0] : 0.0;
+    const double cx = m[1] * inv, cy = m[2] * inv, cz = m[3] * inv;
+    const double dx = x - cx, dy = y - cy, dz = z - cz;
+    sh[threadIdx.x] = valid ? sqrt(dx * dx + dy * dy + dz * dz) : -1.0;
+    __syncthreads();
+    for (int s = 32; s > 0; s >>= 1) {
+        if (lane < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (lane == 0) group_box[i / kGroup] = make_double4(cx, cy, cz, sh[base]);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// RK4 stage update body: chunk reduction, velocity, k_s, accumulator,
+// next-stage inputs and source records, stage-1 kinetic energy.
+// ---------------------------------------------------------------------------
+
+// Sum of the chunk partials of target i, chunk 0 first.
+__device__ __forceinline__ void reduce_chunks(const double *__restrict__ partial, int n_chunks,
+                                              int stride, int i, double &Sx, double &Sy,
+                                              double &Sz)
+{
+    Sx = Sy = Sz = 0.0;
+    for (int c = 0; c < n_chunks; ++c) {
+        const double *p = partial + (size_t)(3 * c) * stride;
+        Sx += p[i];
+        Sy += p[stride + i];
+        Sz += p[2 * (size_t)stride + i];
+    }
+}
+
+// Targets [blk * 256, blk * 256 + 256) of one step state; at stage 0 the
+// block's sum of ke * area goes to st.ke_part[blk].
+__device__ __forceinline__ void stage_update_body(const StepState &st, const StageParams &sp,
+                                                  int blk, double *sh)
+{
+    const int i = blk * 256 + threadIdx.x;
+    const int stride = st.t_stride;
+    double ke_local = 0.0;
+    if (i < st.n_targets) {
+        double Sx, Sy, Sz;
+        reduce_chunks(st.partial, sp.n_chunks, stride, i, Sx, Sy, Sz);
+        const double x = st.xs[i], y = st.xs[stride + i], z = st.xs[2 * stride + i];
+        const double c4 = -4.0 * kPi;
+        // u = sum_j w_j (x_i x x_j)/d_ij / (-4 PI)    (PanelVelocityParallel.f90:1332)
+        const double ux = (y * Sz - z * Sy) / c4;
+        const double uy = (z * Sx - x * Sz) / c4;
+        const double uz = (x * Sy - y * Sx) / c4;
+        const double kx = sp.dt * ux, ky = sp.dt * uy, kz = sp.dt * uz;
+        const double *base = (sp.stage == 0) ? st.x0 : st.acc;
+        // x + k1/6 + k2/3 + k3/3 + k4/6, left to right (:637-644)
+        const double ax = base[i] + kx / sp.divisor;
+        const double ay = base[stride + i] + ky / sp.divisor;
+        const double az = base[2 * stride + i] + kz / sp.divisor;
+        const bool active = i < st.n_active;
+        double zeta_next = 0.0, zacc = 0.0;
+        if (active) {
+            const double dvort = -(2.0 * sp.omega * uz);   // :1333
+            const double kv = sp.dt * dvort;
+            zacc = ((sp.stage == 0) ? st.zeta0[i] : st.zacc[i]) + kv / sp.divisor;
+            st.zacc[i] = zacc;
+            zeta_next = st.zeta0[i] + sp.a_next * kv;
+            if (sp.stage == 0) {
+                const double e = (ux * ux + uy * uy) + uz * uz;   // :495-497
+                st.ke[i] = e;
+                ke_local = e * st.area[i];
+            }
+        }
+        if (sp.stage < 3) {
+            st.acc[i] = ax;
+            st.acc[stride + i] = ay;
+            st.acc[2 * stride + i] = az;
+            // next stage input x + a k  (:513-516, :555-558, :597-600)
+            const double nx = st.x0[i] + sp.a_next * kx;
+            const double ny = st.x0[stride + i] + sp.a_next * ky;
+            const double nz = st.x0[2 * stride + i] + sp.a_next * kz;
+            st.xs[i] = nx;
+            st.xs[stride + i] = ny;
+            st.xs[2 * stride + i] = nz;
+            if (active) st.src[i] = Source{nx, ny, nz, zeta_next * st.area[i]};
+        } else {
+            double nx = ax, ny = ay, nz = az;
+            if (sp.renormalize) {   // RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702
+                const double r = sqrt((nx * nx + ny * ny) + nz * nz);
+                nx /= r;
+                ny /= r;
+                nz /= r;
+            }
+            st.x0[i] = nx;
+            st.x0[stride + i] = ny;
+            st.x0[2 * stride + i] = nz;
+            st.xs[i] = nx;
+            st.xs[stride + i] = ny;
+            st.xs[2 * stride + i] = nz;
+            if (active) {
+                st.zeta0[i] = zacc;
+                st.src[i] = Source{nx, ny, nz, zacc * st.area[i]};
+            }
+            // particles carry absolute vorticity: relVort = absVort - 2 Omega z (:651)
+            if (st.particle_absvort && i >= st.particle_begin && i < st.particle_end) {
+                const int p = i - st.particle_begin;
+                st.particle_relvort[p] = st.particle_absvort[p] - 2.0 * sp.omega * nz;
+            }
+        }
+    }
+    if (sp.stage == 0) {
+        const double b = block_sum(ke_local, sh);
+        if (threadIdx.x == 0) st.ke_part[blk] = b;
+    }
+}
+
+// totalKE = 1/2 sum_j ke_j A_j (:498), the n block partials added in order.
+__device__ __forceinline__ void ke_total_body(const double *__restrict__ part, int n,
+                                              double *__restrict__ out, double *sh)
+{
+    double v = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) v += part[k];
+    const double b = block_sum(v, sh);
+    if (threadIdx.x == 0) *out = 0.5 * b;
+}
+
+// Stage-1 inputs of a step from the step-start state: xs = x0 and the
+// source records {x, zeta A} of the active panels.
+__device__ __forceinline__ void prepare_step_body(const StepState &st, int blk)
+{
+    const int i = blk * 256 + threadIdx.x;
+    if (i >= st.n_targets) return;
+    const int stride = st.t_stride;
+    const double x = st.x0[i], y = st.x0[stride + i], z = st.x0[2 * stride + i];
+    st.xs[i] = x;
+    st.xs[stride + i] = y;
+    st.xs[2 * stride + i] = z;
+    if (i < st.n_active) st.src[i] = Source{x, y, z, st.zeta0[i] * st.area[i]};
+}
+
+}  // namespace bve

@@ -105,6 +105,34 @@ hipError_t launch_make_sources(const double *xyz, const double *vort, const doub
 hipError_t launch_finalize_velocity(const double *partial, int n_chunks, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, hipStream_t s);
-hipError_t launch_normalize_soa(double *t, int stride, int n, hipStream_t s);
+
+// Ensemble: one work item of the persistent pair launch.
+struct EnsItem {
+    int member, chunk, ttile;
+};
+
+// Device tables of an ensemble launch (all pointers are device memory).
+struct EnsLaunch {
+    int n_members;
+    int grid;                        // persistent workgroups
+    const EnsItem *items;
+    int n_items;
+    const PairArgs *pair_members;    // float64 modes
+    const MixedArgs *mixed_members;  // mixed modes
+    int mixed, kahan, recip;
+    const StepState *states;
+    const int *n_chunks;             // per member
+    const int2 *stage_map;           // (member, block) of every stage-update block
+    int n_stage_blocks;
+    const int2 *src_map;             // (member, source tile) of the mixed preparation
+    int n_src_blocks;
+    const int2 *tgt_map;             // (member, 256-target block) of the mixed preparation
+    int n_tgt_blocks;
+};
+
+hipError_t launch_ensemble_pairs(const EnsLaunch &e, hipStream_t s);
+hipError_t launch_ensemble_mixed_prep(const EnsLaunch &e, hipStream_t s);
+hipError_t launch_ensemble_stage(const EnsLaunch &e, const StageParams &sp, hipStream_t s);
+hipError_t launch_ensemble_prepare(const EnsLaunch &e, hipStream_t s);
 
 }  // namespace bve

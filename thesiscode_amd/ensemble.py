@@ -1,0 +1,134 @@
+"""Ensembles of uneven panel problems on one GPU, and the launch-plan
+autotuner they use.
+
+The ensemble workload: member m lives on the cubed sphere of level
+levels[m % len(levels)]; its relative vorticity is the RH4 wave plus a seeded
+smooth random perturbation (sum of Gaussian bumps of width `corr` at random
+centres, scaled to RMS sigma = 0.05 max|zeta| over the active panels).
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from . import grid as gridmod
+from . import initial_conditions as ic
+
+
+def perturbation(grid, seed, sigma, corr=0.3, n_bumps=64):
+    """Seeded smooth random field evaluated at the active panels, particles
+    and passive panels of `grid`; one scaling (RMS sigma over the active
+    panels) for all three."""
+    rng = np.random.default_rng(seed)
+    c = rng.normal(size=(n_bumps, 3))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    amps = rng.normal(size=n_bumps)
+
+    def f(x):
+        if len(x) == 0:
+            return np.zeros(0)
+        return np.exp(-2.0 * (1.0 - x @ c.T) / (corr * corr)) @ amps
+    fa = f(grid.active_x)
+    scale = sigma / np.sqrt(np.mean(fa * fa))
+    return fa * scale, f(grid.particles_x) * scale, f(grid.passive_x) * scale
+
+
+def make_members(n_members=64, levels=(4, 5, 6), seed=0, omega=ic.OMEGA_EARTH):
+    """The ensemble's member inputs (dicts of arrays), member m at level
+    levels[m % len(levels)]."""
+    grids = {L: gridmod.cubed_sphere(L) for L in set(levels)}
+    members = []
+    for m in range(n_members):
+        L = levels[m % len(levels)]
+        g = grids[L]
+        za = ic.rh4_wave(g.active_x)
+        zp = ic.rh4_wave(g.particles_x)
+        zq = ic.rh4_wave(g.passive_x) if g.n_passive else np.zeros(0)
+        pa, pp, pq = perturbation(g, seed + m, 0.05 * np.abs(za).max())
+        members.append(dict(level=L, grid=g, zeta_active=za + pa, zeta_particles=zp + pp,
+                            zeta_passive=zq + pq,
+                            absvort=zp + pp + 2 * omega * g.particles_x[:, 2]))
+    return members
+
+
+def load_members(ens, members):
+    for m, d in enumerate(members):
+        g = d["grid"]
+        ens.load_member(m, g.active_x, d["zeta_active"], g.active_area, g.passive_x,
+                        g.particles_x, d["absvort"])
+
+
+def candidate_plans(level):
+    """(targets per lane, chunk) pairs the autotuner tries for a level."""
+    n_src = gridmod.counts(level)[0]
+    chunks = [c for c in (512, 1024, 2048, 4096, 8192, 16384) if c <= max(512, n_src)]
+    return [(T, c) for T in (1, 2, 4) for c in chunks]
+
+
+def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, omega=ic.OMEGA_EARTH,
+             dt=0.01, device=0, log=None):
+    """Time every candidate plan on a homogeneous ensemble of each level (one
+    RK4 step, HIP events around the pair launches) and return
+    {level: {"targets_per_lane", "chunk", "ms_per_stage", "tried"}}."""
+    table = {}
+    for L in levels:
+        mem = make_members(members_per_level, (L,))
+        best = None
+        tried = []
+        with _lib.Ensemble(members_per_level, device) as ens:
+            ens.set_params(omega, 0.05, precision, 0.02)
+            load_members(ens, mem)
+            for T, chunk in candidate_plans(L):
+                for m in range(members_per_level):
+                    ens.set_member_plan(m, T, chunk)
+                ens.advance(dt, 1, use_graph=False)
+                ens.synchronize()
+                ens.set_profiling(True)
+                ens.advance(dt, 1, use_graph=False)
+                n, ms = ens.profile()
+                ens.set_profiling(False)
+                per = ms / max(n, 1)
+                tried.append((T, chunk, per))
+                if best is None or per < best[2]:
+                    best = (T, chunk, per)
+        table[L] = dict(targets_per_lane=best[0], chunk=best[1], ms_per_stage=best[2],
+                        tried=tried)
+        if log:
+            log(f"autotune level {L}: T={best[0]} chunk={best[1]} {best[2]:.3f} ms/stage")
+    return table
+
+
+def save_table(table, path):
+    with open(path, "w") as f:
+        json.dump({str(k): v for k, v in table.items()}, f, indent=1)
+
+
+def load_table(path):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return {int(k): v for k, v in json.load(f).items()}
+
+
+def run(members, dt, n_steps, precision, table=None, use_graph=True, omega=ic.OMEGA_EARTH,
+        device=0, warmup=1):
+    """Advance the ensemble n_steps after one untimed launch of the same
+    n_steps (which also captures the graph when asked), and return
+    (seconds, pairs per step, ensemble)."""
+    ens = _lib.Ensemble(len(members), device)
+    ens.set_params(omega, 0.05, precision, 0.02)
+    load_members(ens, members)
+    for m, d in enumerate(members):
+        if table and d["level"] in table:
+            t = table[d["level"]]
+            ens.set_member_plan(m, t["targets_per_lane"], t["chunk"])
+    if warmup:
+        ens.advance(dt, n_steps, use_graph)
+        ens.synchronize()
+    t0 = time.perf_counter()
+    ens.advance(dt, n_steps, use_graph)
+    ens.synchronize()
+    el = time.perf_counter() - t0
+    return el, ens.pairs_per_step(), ens

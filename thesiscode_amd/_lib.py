@@ -58,6 +58,8 @@ SIGNATURES = {
     "bve_ensemble_pairs_per_step": (_i, [_p, _dp]),
     "bve_set_profiling": (_i, [_p, _i]),
     "bve_get_profile": (_i, [_p, ctypes.POINTER(_ll), _dp]),
+    "bve_copy_probe": (_i, [_p, _ll]),
+    "bve_get_plan": (_i, [_p, _p, _p, _p, _p, _p]),
     "bve_pairs_per_step": (_i, [_p, _dp]),
 }
 
@@ -273,6 +275,15 @@ class Context:
         v = ctypes.c_double()
         self._check(self._L.bve_pairs_per_step(self._h, ctypes.byref(v)), "bve_pairs_per_step")
         return v.value
+
+    def copy_probe(self, nbytes):
+        self._check(self._L.bve_copy_probe(self._h, nbytes), "bve_copy_probe")
+
+    def plan(self):
+        """(targets_per_lane, tile, chunk, n_chunks, recip) of the pair kernel."""
+        v = [ctypes.c_int() for _ in range(5)]
+        self._check(self._L.bve_get_plan(self._h, *(ctypes.byref(x) for x in v)), "bve_get_plan")
+        return tuple(x.value for x in v)
 
 
 class Ensemble:

@@ -192,7 +192,8 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(n_targets);
     // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
     const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
-    const int wg = c->plan_wg ? c->plan_wg : 4096;
+    // ~8 work items per CU: enough waves to hide latency, few partial sums
+    const int wg = c->plan_wg ? c->plan_wg : 2048;
     return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
 }
 
@@ -483,7 +484,7 @@ int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
     int rc = ensure_partial(c, (size_t)fwd.n_chunks * 3 * c->t_stride);
     if (rc) return rc;
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_active);
-    const int wg = c->plan_wg ? c->plan_wg : 4096;
+    const int wg = c->plan_wg ? c->plan_wg : 2048;
     vjp = make_plan(c->n_active, c->adj_pad, T, 256, wg,
                     c->precision == BVE_PRECISION_FP64_FAST ? 4 : 1);
     rc = grow(c, c->adj_pt, c->adj_pt_cap, (size_t)vjp.n_chunks * 6 * c->adj_pad, false);
@@ -1080,6 +1081,38 @@ int bve_get_profile(bve_ctx *c, long long *launches, double *total_ms)
     collect_profile(c);
     if (launches) *launches = c->prof_launches;
     if (total_ms) *total_ms = c->prof_ms;
+    return BVE_OK;
+}
+
+int bve_get_plan(const bve_ctx *c, int *targets_per_lane, int *tile, int *chunk, int *n_chunks,
+                 int *recip)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return BVE_ERR_STATE;
+    if (targets_per_lane) *targets_per_lane = c->plan.targets_per_lane;
+    if (tile) *tile = c->plan.tile;
+    if (chunk) *chunk = c->plan.chunk;
+    if (n_chunks) *n_chunks = c->plan.n_chunks;
+    if (recip) *recip = c->plan.recip;
+    return BVE_OK;
+}
+
+int bve_copy_probe(bve_ctx *c, long long bytes)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (bytes < 16 || bytes % 16 != 0)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_copy_probe: bytes must be a positive multiple of 16");
+    HIP_TRY(c, hipSetDevice(c->device));
+    char *in = nullptr, *out = nullptr;
+    HIP_TRY(c, dev_alloc(&in, (size_t)bytes));
+    hipError_t e = dev_alloc(&out, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(in, 1, (size_t)bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = launch_copy_probe(in, out, (size_t)bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free(in);
+    dev_free(out);
+    if (e != hipSuccess) return c->hip_fail(e, "bve_copy_probe");
     return BVE_OK;
 }
 

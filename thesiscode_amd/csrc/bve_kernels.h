@@ -105,6 +105,7 @@ hipError_t launch_make_sources(const double *xyz, const double *vort, const doub
 hipError_t launch_finalize_velocity(const double *partial, int n_chunks, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, hipStream_t s);
+hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_t s);
 
 // Ensemble: one work item of the persistent pair launch.
 struct EnsItem {

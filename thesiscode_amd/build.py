@@ -52,7 +52,39 @@ def build(verbose=False, extra_flags=()):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran_host(verbose)
     return LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+HOST_EXE = os.path.join(HERE, "bve_host.exe")
+
+
+def flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/llvm/bin/flang", shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran_host(verbose=False):
+    """The standalone Fortran host (fortran/bve_host.f90 over the
+    ISO_C_BINDING interface), linked against libbve.so; skipped when no
+    Fortran compiler is installed."""
+    fc = flang()
+    if fc is None:
+        return None
+    mod_dir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(mod_dir, exist_ok=True)
+    tmp = HOST_EXE + ".tmp"
+    cmd = [fc, "-O2", "-J", mod_dir, os.path.join(FORTRAN, "bve_interface.f90"),
+           os.path.join(FORTRAN, "bve_host.f90"), "-L", HERE, "-lbve",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HOST_EXE)
+    return HOST_EXE
 
 
 if __name__ == "__main__":

@@ -10,8 +10,13 @@ parent-panel targets are computed but not counted.
 
 Each arithmetic mode is timed separately; its normwise velocity error is
 measured against the oracle (float64 in the reference's rounding order) on
-64 active targets.  ``value`` is the pair rate of the fastest mode whose
-error is <= 1e-10.
+~190 active targets (the first 64, the 64 nearest the vortex centre, 64
+random).  ``value`` is the pair rate of the fastest mode whose error is
+<= 1e-10.
+
+Also reported: the adjoint gradient on level 6 (cost against the forward
+solve, finite-difference check), a 64-member ensemble in one HIP graph and
+a level-8 two-vortex single step in every arithmetic mode.
 
 Prints one JSON line.  The roofline block uses HIP events recorded by the
 library around every pair-kernel launch (one launch per RK4 stage) at 20
@@ -56,6 +61,8 @@ def parse():
                    help="cubed-sphere level of the adjoint workload (0 = skip)")
     p.add_argument("--ensemble-members", type=int, default=64,
                    help="members of the ensemble workload (0 = skip)")
+    p.add_argument("--large-level", type=int, default=8,
+                   help="cubed-sphere level of the two-vortex large solve (0 = skip)")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="only run a plain copy of this many MiB (byte-counter calibration)")
     return p.parse_args()
@@ -173,8 +180,35 @@ def ensemble_measurement(n_members=64, levels=(4, 5, 6), steps=10, dt=0.01,
                          f"one HIP graph", modes=out)
 
 
-def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all):
+def error_sample(grid, centres, n=64, seed=7):
+    """Indices of the active targets whose velocity error is measured: the
+    first n, the n nearest to each vortex centre (the steepest part of the
+    field) and n seeded random ones."""
+    idx = [np.arange(n)]
+    for c in centres:
+        idx.append(np.argsort(1.0 - grid.active_x @ c)[:n])
+    idx.append(np.random.default_rng(seed).choice(grid.n_active, n, replace=False))
+    return np.unique(np.concatenate(idx))
+
+
+def velocity_error(ctx, grid, za, sample):
+    """Velocity of the current arithmetic mode at the sampled active targets
+    against the oracle (float64, reference rounding order): the sources are
+    permuted so that the sampled panels come first and self-exclusion by
+    index still applies.  Returns (normwise, max pointwise) relative error:
+    max_i |du_i| / max_i |u_i| and max_i |du_i| / |u_i| over the sample."""
     import oracle
+    perm = np.concatenate([sample, np.setdiff1d(np.arange(grid.n_active), sample)])
+    xs, zs, As = grid.active_x[perm], za[perm], grid.active_area[perm]
+    k = len(sample)
+    u_gpu = ctx.velocity(xs[:k], xs, zs, As, self_end=k)
+    u_ref = oracle.active_rhs(xs, zs, As, 2 * np.pi, 0, k)[0][:k]
+    du = np.linalg.norm(u_gpu - u_ref, axis=1)
+    un = np.linalg.norm(u_ref, axis=1)
+    return float(du.max() / un.max()), float((du / un).max())
+
+
+def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all, sample=None):
     from thesiscode_amd import _lib
     ctx.set_precision(MODES[mode])
     ctx.advance(args.dt, args.warmup)
@@ -192,23 +226,55 @@ def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all):
     # and its targets read (24 B), plus the sources once (32 B); this model
     # matched rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated) within 2%
     traffic = n_chunks * 48.0 * n_t + 32.0 * grid.n_active
-    # accuracy of this mode: first 64 active targets vs the oracle
-    u_gpu = ctx.velocity(grid.active_x[:64], grid.active_x, za, grid.active_area, self_end=64)
-    u_ref, _ = oracle.active_rhs(grid.active_x, za, grid.active_area, 2 * np.pi, 0, 64)
-    u_ref = u_ref[:64]
-    err = float(np.max(np.linalg.norm(u_gpu - u_ref, axis=1)) /
-                np.max(np.linalg.norm(u_ref, axis=1)))
+    # accuracy of this mode at the sampled active targets vs the oracle
+    err, err_point = velocity_error(ctx, grid, za, sample if sample is not None else np.arange(64))
     ctx.set_precision(_lib.BVE_PRECISION_FP64)
     avg_ms = kernel_ms / max(launches, 1)
     achieved = pairs_all / 4.0 * FLOPS_PER_PAIR / (avg_ms * 1e-3) / 1e12
     return dict(pair_rate=pairs_primary * args.steps / el,
                 pair_rate_all_classes=pairs_all * args.steps / el,
                 ms_per_step=el * 1e3 / max(args.steps, 1), rel_err_vs_fp64_oracle=err,
+                max_pointwise_rel_err=err_point,
+                error_sample=int(len(sample)) if sample is not None else 64,
                 launches=launches, avg_launch_ms=avg_ms, achieved_tflops=achieved,
                 roofline_frac=achieved / PEAK_FP64_TFLOPS,
                 plan=dict(targets_per_lane=T, tile=tile, chunk=chunk, n_chunks=n_chunks,
                           recip=recip),
                 traffic_bytes_per_launch=traffic)
+
+
+def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed_kahan"),
+                            dt=0.002):
+    """Large single solve: two co-rotating Gaussian vortices (amplitude 4 PI,
+    width 0.1, at 20N +-10E, minus the mean) on the level-8 cubed sphere
+    (393,216 active panels, 393,218 vertices, 131,070 parents); one RK4 step
+    at dt = 0.002 per mode after one warm-up step, state resident in HBM."""
+    import types
+    from thesiscode_amd import _lib
+    from thesiscode_amd import grid as gridmod
+    from thesiscode_amd import initial_conditions as ic
+    g = gridmod.cubed_sphere(level)
+    za, zp, _ = ic.two_vortices(g)
+    omega = ic.OMEGA_EARTH
+    absv = zp + 2 * omega * g.particles_x[:, 2]
+    na, npart, npp = g.n_active, g.n_particles, g.n_passive
+    pairs_primary = 4.0 * (na * (na - 1.0) + npart * na)
+    pairs_all = 4.0 * (na * (na - 1.0) + (npart + npp) * na)
+    lat, lon = np.deg2rad(20.0), np.deg2rad(10.0)
+    sample = error_sample(g, [ic.latlon_to_xyz(lat, lon), ic.latlon_to_xyz(lat, -lon)])
+    ctx = _lib.Context(0)
+    ctx.set_omega(omega)
+    ctx.load_panels(g.active_x, za, g.active_area, g.passive_x)
+    ctx.load_particles(g.particles_x, absv)
+    args = types.SimpleNamespace(dt=dt, warmup=1, steps=1)
+    out = {}
+    for mode in modes:
+        out[mode] = measure(ctx, g, za, mode, args, pairs_primary, pairs_all, sample)
+        print(f"# large {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    ctx.close()
+    return dict(workload=f"two_vortices_cubed_sphere_L{level}_rk4_step_dt{dt:g}", level=level,
+                n_active=na, n_particles=npart, n_passive=npp, pairs_per_step=pairs_primary,
+                pairs_per_step_all_classes=pairs_all, modes=out)
 
 
 def main():
@@ -245,9 +311,10 @@ def main():
     print(f"# level {args.level}: {na} active, {npart} particles, {npp} passive", file=sys.stderr,
           flush=True)
 
+    sample = error_sample(g, [ic.latlon_to_xyz(np.pi / 4, 0.0)])
     results = {}
     for mode in [m for m in args.modes.split(",") if m]:
-        results[mode] = measure(ctx, g, za, mode, args, pairs_primary, pairs_all)
+        results[mode] = measure(ctx, g, za, mode, args, pairs_primary, pairs_all, sample)
         print(f"# {mode}: {results[mode]}", file=sys.stderr, flush=True)
     ctx.close()
 
@@ -298,6 +365,9 @@ def main():
         ens = ensemble_measurement(args.ensemble_members)
         print(f"# ensemble: {ens}", file=sys.stderr, flush=True)
         out["config"]["ensemble"] = ens
+    if args.large_level > 0:
+        big = large_solve_measurement(args.large_level)
+        out["config"]["large_solve"] = big
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(g, za, args.cpu_seconds)
     print(json.dumps(out), flush=True)

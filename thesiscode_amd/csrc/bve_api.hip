@@ -189,11 +189,17 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
 {
     const bool mixed = c->precision == BVE_PRECISION_MIXED ||
                        c->precision == BVE_PRECISION_MIXED_KAHAN;
-    const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(n_targets);
+    // mixed: at most 2 targets per lane (register pressure of the two paths)
+    const int T = c->plan_t ? c->plan_t
+                            : std::min(mixed ? 2 : 4, choose_targets_per_lane(n_targets));
     // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
     const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
-    // ~8 work items per CU: enough waves to hide latency, few partial sums
-    const int wg = c->plan_wg ? c->plan_wg : 2048;
+    // Many short work items: a launch holds ~20 rounds of the resident
+    // workgroups (256 CUs x 3-4), so the last, partly filled round and the
+    // costlier items (self-exclusion tiles; near-field tiles of the mixed
+    // kernel, hence its finer split) cost little.  Chunks are never shorter
+    // than one tile.
+    const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
     return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
 }
 
@@ -484,7 +490,7 @@ int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
     int rc = ensure_partial(c, (size_t)fwd.n_chunks * 3 * c->t_stride);
     if (rc) return rc;
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_active);
-    const int wg = c->plan_wg ? c->plan_wg : 2048;
+    const int wg = c->plan_wg ? c->plan_wg : 16384;
     vjp = make_plan(c->n_active, c->adj_pad, T, 256, wg,
                     c->precision == BVE_PRECISION_FP64_FAST ? 4 : 1);
     rc = grow(c, c->adj_pt, c->adj_pt_cap, (size_t)vjp.n_chunks * 6 * c->adj_pad, false);

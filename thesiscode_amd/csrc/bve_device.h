@@ -7,12 +7,13 @@
 //
 // with w_j = zeta_j A_j.  The float64 body evaluates the factored form
 // S_i = sum_j (w_j / d_ij) x_j and the stage body forms u_i = (x_i x S_i) / (-4 PI),
-// which saves the per-pair cross product.  Per pair that is 3 FMA for the
-// denominator, the reciprocal (below), one multiply by w and 3 FMA into S.
+// which saves the per-pair cross product.  The weight is folded into the
+// source when a tile is staged in LDS ({x_j, w_j x_j}), so a pair costs 3 FMA
+// for the denominator, the reciprocal (below) and 3 FMA into S.
 //
 // Work decomposition: a work item is (target tile, source chunk).  Targets
 // are held in registers (T per lane), sources stream through LDS in tiles
-// of AoS {x, y, z, w} records, and each item writes its own partial sums,
+// (HBM records {x, y, z, w}), and each item writes its own partial sums,
 // which the stage body adds in a fixed order: results do not depend on
 // scheduling and reruns are bitwise reproducible.
 #pragma once
@@ -65,6 +66,21 @@ __device__ __forceinline__ bool diag_tile(int lo, int self_end, int jt, int tile
 // float64 pair body
 // ---------------------------------------------------------------------------
 
+// A source as staged in LDS: position and weighted position w x.
+struct alignas(16) StagedSource {
+    double x, y, z, wx, wy, wz;
+};
+
+// Stages `n` HBM records starting at src into LDS (all threads of the block).
+__device__ __forceinline__ void stage_sources(StagedSource *lds, const Source *__restrict__ src,
+                                              int n)
+{
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+        const Source s = src[k];
+        lds[k] = StagedSource{s.x, s.y, s.z, s.w * s.x, s.w * s.y, s.w * s.z};
+    }
+}
+
 // One LDS tile of sources against the T targets of this lane.
 //
 // R = 1: every pair gets its own reciprocal (recip_f64).
@@ -75,17 +91,18 @@ __device__ __forceinline__ bool diag_tile(int lo, int self_end, int jt, int tile
 // float64 underflow and overflow.
 //
 // MASK: the tile holds the own-index source of some target of this wave;
-// that pair's denominator (~0) is replaced by 1 and its weight by 0.
+// that pair's denominator (~0) is replaced by 1 and its 1/d by 0.
 template <int T, int TILE, int R, bool MASK>
-__device__ __forceinline__ void tile_f64(const Source *__restrict__ lds, int jt,
+__device__ __forceinline__ void tile_f64(const StagedSource *__restrict__ lds, int jt,
                                          const double (&nx)[T], const double (&ny)[T],
                                          const double (&nz)[T], const double (&c0)[T],
                                          const int (&self)[T], double (&sx)[T],
                                          double (&sy)[T], double (&sz)[T])
 {
-#pragma unroll(R == 1 ? 2 : 1)
+    constexpr int kUnroll = R == 1 ? 2 : 1;
+#pragma unroll kUnroll
     for (int k = 0; k < TILE; k += R) {
-        Source s[R];
+        StagedSource s[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) s[r] = lds[k + r];
 #pragma unroll
@@ -99,26 +116,26 @@ __device__ __forceinline__ void tile_f64(const Source *__restrict__ lds, int jt,
                 if (MASK) d[r] = (jt + k + r == self[t]) ? 1.0 : d[r];
             }
             if constexpr (R == 1) {
-                q[0] = s[0].w * recip_f64(d[0]);
+                q[0] = recip_f64(d[0]);
             } else if constexpr (R == 2) {
                 const double rr = recip1_f64(d[0] * d[1]);
-                q[0] = s[0].w * d[1] * rr;
-                q[1] = s[1].w * d[0] * rr;
+                q[0] = d[1] * rr;
+                q[1] = d[0] * rr;
             } else {
                 const double p01 = d[0] * d[1], p23 = d[2] * d[3];
                 const double rr = recip1_f64(p01 * p23);
                 const double r01 = rr * p23, r23 = rr * p01;
-                q[0] = s[0].w * d[1] * r01;
-                q[1] = s[1].w * d[0] * r01;
-                q[2] = s[2].w * d[3] * r23;
-                q[3] = s[3].w * d[2] * r23;
+                q[0] = d[1] * r01;
+                q[1] = d[0] * r01;
+                q[2] = d[3] * r23;
+                q[3] = d[2] * r23;
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (MASK) q[r] = (jt + k + r == self[t]) ? 0.0 : q[r];
-                sx[t] = __builtin_fma(q[r], s[r].x, sx[t]);
-                sy[t] = __builtin_fma(q[r], s[r].y, sy[t]);
-                sz[t] = __builtin_fma(q[r], s[r].z, sz[t]);
+                sx[t] = __builtin_fma(q[r], s[r].wx, sx[t]);
+                sy[t] = __builtin_fma(q[r], s[r].wy, sy[t]);
+                sz[t] = __builtin_fma(q[r], s[r].wz, sz[t]);
             }
         }
     }
@@ -128,7 +145,7 @@ __device__ __forceinline__ void tile_f64(const Source *__restrict__ lds, int jt,
 // lds holds TILE records.
 template <int T, int TILE, int R>
 __device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int ttile,
-                                              Source *lds)
+                                              StagedSource *lds)
 {
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
@@ -152,7 +169,7 @@ __device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int 
     const int j_end = min(j_begin + a.plan.chunk, a.n_src_pad);
     for (int jt = j_begin; jt < j_end; jt += TILE) {
         __syncthreads();
-        for (int k = threadIdx.x; k < TILE; k += kBlock) lds[k] = a.src[jt + k];
+        stage_sources(lds, a.src + jt, TILE);
         __syncthreads();
         bool diag = false;
 #pragma unroll

@@ -18,7 +18,7 @@ __global__ __launch_bounds__(kBlock) void ens_pair_f64_kernel(const EnsItem *__r
                                                               int n_items,
                                                               const PairArgs *__restrict__ members)
 {
-    __shared__ Source lds[256];
+    __shared__ StagedSource lds[256];
     for (int k = blockIdx.x; k < n_items; k += gridDim.x) {
         const EnsItem it = items[k];
         const PairArgs a = members[it.member];

@@ -16,7 +16,7 @@ __global__ __launch_bounds__(kBlock) void pair_sum_f64_kernel(PairArgs a)
 {
     const int work = xcd_work_item(blockIdx.x, a.plan.grid);
     if (work >= a.plan.n_work) return;
-    __shared__ Source lds[TILE];
+    __shared__ StagedSource lds[TILE];
     const int chunk = work / a.plan.n_tiles;
     pair_f64_item<T, TILE, R>(a, chunk, work - chunk * a.plan.n_tiles, lds);
 }

@@ -63,7 +63,7 @@ def load_members(ens, members):
 def candidate_plans(level):
     """(targets per lane, chunk) pairs the autotuner tries for a level."""
     n_src = gridmod.counts(level)[0]
-    chunks = [c for c in (512, 1024, 2048, 4096, 8192, 16384) if c <= max(512, n_src)]
+    chunks = [c for c in (256, 512, 1024, 2048, 4096, 8192, 16384) if c <= max(512, n_src)]
     return [(T, c) for T in (1, 2, 4) for c in chunks]
 
 

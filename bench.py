@@ -40,9 +40,14 @@ PEAK_FP64_TFLOPS = 78.6      # MI355X float64 vector
 ERR_BAR = 1e-10
 METRIC = "Biot-Savart pair-interactions/s per RK4 step at <=1e-10 rel err; adjoint/fwd ratio"
 MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3}
-KERNEL_NAMES = {"fp64": "pair_sum_f64_kernel<T,TILE,R=1>", "mixed": "pair_sum_mixed_kernel<T,false>",
-                "mixed_kahan": "pair_sum_mixed_kernel<T,true>",
-                "fp64_fast": "pair_sum_f64_kernel<T,TILE,R=4>"}
+
+
+def kernel_name(mode, T, tile, recip, segments):
+    """Name of the pair kernel instantiation a mode's launch runs."""
+    if mode.startswith("mixed"):
+        return f"pair_sum_mixed_kernel<{T},{str(mode == 'mixed_kahan').lower()}>"
+    kind = "pair_sum_f64_seg_kernel" if segments > 0 else "pair_sum_f64_kernel"
+    return f"{kind}<{T},{tile},{recip}>"
 
 
 def parse():
@@ -221,11 +226,13 @@ def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all, sample=None):
     launches, kernel_ms = ctx.profile()
     ctx.set_profiling(False)
     T, tile, chunk, n_chunks, recip = ctx.plan()
-    n_t = grid.n_active + grid.n_particles + grid.n_passive
-    # HBM bytes per launch: each chunk's partial sums written (24 B per target)
-    # and its targets read (24 B), plus the sources once (32 B); this model
-    # matched rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated) within 2%
-    traffic = n_chunks * 48.0 * n_t + 32.0 * grid.n_active
+    launch = ctx.launch()
+    # HBM bytes per launch: the partial-sum rows written (24 B per target per
+    # row), the targets each row's work item reads (the same count) and the
+    # source records (32 B each); for the float64 segmented launch at level 7
+    # this is 206 MB against 199 MB from rocprofv3 FETCH_SIZE + WRITE_SIZE
+    # (copy-calibrated)
+    traffic = 2.0 * launch["partial_bytes"] + 32.0 * grid.n_active
     # accuracy of this mode at the sampled active targets vs the oracle
     err, err_point = velocity_error(ctx, grid, za, sample if sample is not None else np.arange(64))
     ctx.set_precision(_lib.BVE_PRECISION_FP64)
@@ -239,7 +246,8 @@ def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all, sample=None):
                 launches=launches, avg_launch_ms=avg_ms, achieved_tflops=achieved,
                 roofline_frac=achieved / PEAK_FP64_TFLOPS,
                 plan=dict(targets_per_lane=T, tile=tile, chunk=chunk, n_chunks=n_chunks,
-                          recip=recip),
+                          recip=recip, **launch),
+                kernel=kernel_name(mode, T, tile, recip, launch["segments"]),
                 traffic_bytes_per_launch=traffic)
 
 
@@ -348,9 +356,10 @@ def main():
             "unit": "TFLOP/s",
             "frac": r.get("roofline_frac"),
             "traffic": r.get("traffic_bytes_per_launch"),
-            "traffic_source": "launch-plan model (chunk partials + per-chunk target reads + "
-                              "sources), checked against rocprofv3 FETCH_SIZE/WRITE_SIZE",
-            "kernel": KERNEL_NAMES.get(best),
+            "traffic_source": "launch-plan model (partial-sum rows written + the targets their "
+                              "work items read + sources per XCD), checked against rocprofv3 "
+                              "FETCH_SIZE/WRITE_SIZE",
+            "kernel": r.get("kernel"),
             "launches": r.get("launches"),
             "avg_launch_ms": r.get("avg_launch_ms"),
             "flops_per_launch": pairs_all / 4.0 * FLOPS_PER_PAIR,

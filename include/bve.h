@@ -70,18 +70,33 @@ int bve_set_precision(bve_ctx *ctx, int precision, double nearfield_radius);
  * device); 0 (default) = the reference's BVERK4, which never does. */
 int bve_set_renormalize(bve_ctx *ctx, int on);
 /* Launch geometry of the pair kernel: targets held per lane (1, 2, 4),
- * sources per LDS tile (256, 512), the workgroup count to aim for and, for
+ * sources per LDS tile (256, 512), the work split and, for
  * BVE_PRECISION_FP64_FAST, the pairs sharing one reciprocal (2, 4); 0
- * keeps the built-in choice.  Used by the autotuner. */
+ * keeps the built-in choice.  The float64 modes run a persistent launch
+ * over `target_workgroups` segments of the (target tile, source tile)
+ * sequence; the mixed modes run about `target_workgroups` (target tile,
+ * source chunk) work items.  Used by the autotuner. */
 int bve_set_plan(bve_ctx *ctx, int targets_per_lane, int tile, int target_workgroups,
                  int recip);
+/* chunk > 0: split the sources into chunks of `chunk` sources (rounded up to
+ * whole tiles), one work item per (target tile, chunk), in every arithmetic
+ * mode -- the split ensemble members use, so a problem run alone with the
+ * same chunk and targets per lane reproduces a member bit for bit.  0 (the
+ * default) restores the built-in split. */
+int bve_set_chunk(bve_ctx *ctx, int chunk);
 
 /* The pair kernel's current launch geometry for the loaded problem:
  * targets per lane, sources per LDS tile, sources per chunk, chunks (each
  * chunk's partial sums are written once and read once per stage), pairs per
- * shared reciprocal.  Any pointer may be NULL. */
+ * shared reciprocal.  For a segmented (float64) plan, chunk is the sources
+ * per segment and n_chunks the number of segments.  Any pointer may be NULL. */
 int bve_get_plan(const bve_ctx *ctx, int *targets_per_lane, int *tile, int *chunk,
                  int *n_chunks, int *recip);
+/* Launch shape of the pair kernel: segments (0 for a chunked plan),
+ * launched workgroups, target tiles, and the bytes of partial sums one pass
+ * writes (each is read once by the stage update).  Any pointer may be NULL. */
+int bve_get_launch(const bve_ctx *ctx, int *segments, int *grid, int *n_tiles,
+                   long long *partial_bytes);
 
 /* ---- panel RK4 path (drop-in for PanelVelocityParallelModule) --------- */
 

@@ -13,12 +13,15 @@ pytestmark = pytest.mark.gpu
 OMEGA = 2 * np.pi
 
 
-def alone(d, precision, dt, n_steps):
+def alone(d, precision, dt, n_steps, chunk):
+    """The member's problem in its own context, with the member's launch
+    geometry (256-source tiles, default targets per lane, `chunk`)."""
     g = d["grid"]
     with _lib.Context(0) as c:
         c.set_omega(OMEGA)
         c.set_precision(precision, 0.02)
         c.set_plan(0, 256, 0, 0)
+        c.set_chunk(chunk)
         c.load_panels(g.active_x, d["zeta_active"], g.active_area, g.passive_x)
         c.load_particles(g.particles_x, d["absvort"])
         c.advance(dt, n_steps)
@@ -33,13 +36,16 @@ def alone(d, precision, dt, n_steps):
 def test_members_equal_unbatched_runs(precision, use_graph):
     members = make_members(5, levels=(2, 3, 4), seed=3)
     dt, n = 0.01, 2
+    chunks = [512, 1024, 512, 768, 1536]   # several source tiles per chunk
     with _lib.Ensemble(len(members)) as ens:
         ens.set_params(OMEGA, 0.05, precision, 0.02)
         load_members(ens, members)
+        for m, ch in enumerate(chunks):
+            ens.set_member_plan(m, 0, ch)
         ens.advance(dt, n, use_graph)
         got = [ens.store_member(m) for m in range(len(members))]
     for m, d in enumerate(members):
-        ref = alone(d, precision, dt, n)
+        ref = alone(d, precision, dt, n, chunks[m])
         for k in ("active_x", "active_relvort", "passive_x", "particles_x"):
             assert np.array_equal(got[m][k], ref[k]), (m, k)
 

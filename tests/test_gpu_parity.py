@@ -137,10 +137,15 @@ def test_reruns_are_bitwise_reproducible(ctx):
     (_lib.BVE_PRECISION_FP64, (1, 256, 0, 0)), (_lib.BVE_PRECISION_FP64, (2, 512, 64, 0)),
     (_lib.BVE_PRECISION_FP64, (4, 256, 4096, 0)), (_lib.BVE_PRECISION_FP64, (4, 512, 0, 0)),
     (_lib.BVE_PRECISION_FP64_FAST, (1, 256, 0, 2)), (_lib.BVE_PRECISION_FP64_FAST, (2, 512, 64, 4)),
-    (_lib.BVE_PRECISION_FP64_FAST, (4, 256, 4096, 4)), (_lib.BVE_PRECISION_FP64_FAST, (4, 512, 0, 2))])
+    (_lib.BVE_PRECISION_FP64_FAST, (4, 256, 4096, 4)), (_lib.BVE_PRECISION_FP64_FAST, (4, 512, 0, 2)),
+    # segments longer than a target tile's row of source tiles: one segment
+    # for everything, segments cutting tiles unevenly, segment and tile
+    # boundaries coinciding, and a short last segment
+    (_lib.BVE_PRECISION_FP64, (2, 512, 1, 0)), (_lib.BVE_PRECISION_FP64, (4, 256, 5, 0)),
+    (_lib.BVE_PRECISION_FP64_FAST, (2, 256, 7, 4)), (_lib.BVE_PRECISION_FP64_FAST, (1, 512, 11, 2))])
 def test_launch_plans_agree(ctx, precision, plan):
-    """Every supported launch geometry and float64 reciprocal scheme gives
-    the oracle's velocities for all three target classes."""
+    """Every supported launch geometry, work split and float64 reciprocal
+    scheme gives the oracle's velocities for all three target classes."""
     g = gridmod.cubed_sphere(4)
     z = ic.rh4_wave(g.active_x)
     na, npp = g.n_active, g.n_passive
@@ -322,6 +327,36 @@ def test_small_and_empty_classes(ctx):
     assert np.abs(ax - r["active_x"]).max() < 1e-15
     u = ctx.velocity(x[:1], x[:1], w[:1], np.ones(1), self_end=1)
     assert not u.any()
+
+
+def test_work_split_per_mode(ctx):
+    """Float64 passes run the segmented persistent launch, whose partial sums
+    are far fewer than a chunked split's; mixed passes and an explicit chunk
+    run chunked; a chunked and a segmented step agree to the float64 bar."""
+    g = gridmod.cubed_sphere(5)
+    z = ic.rh4_wave(g.active_x)
+    ctx.reset()
+    ctx.set_plan(0, 0, 0, 0)
+    ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+    try:
+        seg = ctx.launch()
+        assert seg["segments"] > 0 and 0 < seg["grid"] <= seg["segments"]
+        x_seg = ctx.rk4_step(0.01, g.particles_x)[0]
+        ctx.set_chunk(512)
+        ch = ctx.launch()
+        assert ch["segments"] == 0 and ch["partial_bytes"] > seg["partial_bytes"]
+        ctx.reset()
+        ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+        x_ch = ctx.rk4_step(0.01, g.particles_x)[0]
+        disp = np.abs(x_seg - g.particles_x).max()
+        assert np.abs(x_seg - x_ch).max() < 1e-12 * disp
+        ctx.set_chunk(0)
+        ctx.set_precision(_lib.BVE_PRECISION_MIXED)
+        assert ctx.launch()["segments"] == 0
+    finally:
+        ctx.set_chunk(0)
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
 
 
 def test_errors_are_reported(ctx):

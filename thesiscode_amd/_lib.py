@@ -30,6 +30,7 @@ SIGNATURES = {
     "bve_set_precision": (_i, [_p, _i, _d]),
     "bve_set_renormalize": (_i, [_p, _i]),
     "bve_set_plan": (_i, [_p, _i, _i, _i, _i]),
+    "bve_set_chunk": (_i, [_p, _i]),
     "bve_load_panels": (_i, [_p, _i, _p, _p, _p, _i, _p]),
     "bve_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
     "bve_store_panels": (_i, [_p, _p, _p, _p]),
@@ -60,6 +61,7 @@ SIGNATURES = {
     "bve_get_profile": (_i, [_p, ctypes.POINTER(_ll), _dp]),
     "bve_copy_probe": (_i, [_p, _ll]),
     "bve_get_plan": (_i, [_p, _p, _p, _p, _p, _p]),
+    "bve_get_launch": (_i, [_p, _p, _p, _p, _p]),
     "bve_pairs_per_step": (_i, [_p, _dp]),
 }
 
@@ -151,6 +153,9 @@ class Context:
     def set_plan(self, targets_per_lane=0, tile=0, target_workgroups=0, recip=0):
         self._check(self._L.bve_set_plan(self._h, targets_per_lane, tile, target_workgroups, recip),
                     "bve_set_plan")
+
+    def set_chunk(self, chunk=0):
+        self._check(self._L.bve_set_chunk(self._h, chunk), "bve_set_chunk")
 
     # panel path
     def load_panels(self, active_x, active_relvort, active_area, passive_x):
@@ -284,6 +289,15 @@ class Context:
         v = [ctypes.c_int() for _ in range(5)]
         self._check(self._L.bve_get_plan(self._h, *(ctypes.byref(x) for x in v)), "bve_get_plan")
         return tuple(x.value for x in v)
+
+    def launch(self):
+        """{segments, grid, n_tiles, partial_bytes} of the pair kernel."""
+        v = [ctypes.c_int() for _ in range(3)]
+        pb = ctypes.c_longlong()
+        self._check(self._L.bve_get_launch(self._h, *(ctypes.byref(x) for x in v), ctypes.byref(pb)),
+                    "bve_get_launch")
+        return dict(segments=v[0].value, grid=v[1].value, n_tiles=v[2].value,
+                    partial_bytes=pb.value)
 
 
 class Ensemble:

@@ -47,6 +47,9 @@ struct bve_ctx {
     int renormalize = 0;
     int plan_t = 0, plan_tile = 0, plan_wg = 0, plan_recip = 0;
     int plan_chunk = 0;   // explicit source chunk length (ensemble members), 0 = derived
+    bool chunked_only = false;   // ensemble members: the batched launch runs chunked plans
+    int n_cu = 256;              // compute units of the device
+    unsigned *queue = nullptr;   // segment queue of the segmented pair pass (2 counters)
 
     // problem sizes
     bool panels_loaded = false;
@@ -139,6 +142,28 @@ PairPlan make_plan(int n_targets, int n_src_pad, int T, int tile, int target_wg,
     return p;
 }
 
+// Segmented geometry: about `segments` equal runs of the tile-major item
+// sequence, run by a persistent grid of one workgroup per resident slot
+// (slots = 0: one workgroup per segment).
+PairPlan make_segmented_plan(int n_targets, int n_src_pad, int T, int tile, int segments,
+                             int recip, int slots)
+{
+    PairPlan p{};
+    p.targets_per_lane = T;
+    p.tile = tile;
+    p.recip = recip;
+    p.n_tiles = std::max(1, ceil_div(n_targets, kBlock * T));
+    p.n_src_tiles = n_src_pad / tile;
+    const int n_items = p.n_tiles * p.n_src_tiles;
+    p.seg_len = std::max(1, ceil_div(n_items, std::max(1, segments)));
+    p.segments = ceil_div(n_items, p.seg_len);
+    p.grid = slots > 0 ? std::min(slots, p.segments) : p.segments;
+    p.chunk = p.seg_len * tile;
+    p.n_chunks = 0;
+    p.n_work = p.segments;
+    return p;
+}
+
 int ensure_staging(bve_ctx *c, size_t doubles)
 {
     if (doubles <= c->staging_cap) return BVE_OK;
@@ -194,13 +219,24 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
                             : std::min(mixed ? 2 : 4, choose_targets_per_lane(n_targets));
     // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
     const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
-    // Many short work items: a launch holds ~20 rounds of the resident
-    // workgroups (256 CUs x 3-4), so the last, partly filled round and the
-    // costlier items (self-exclusion tiles; near-field tiles of the mixed
-    // kernel, hence its finer split) cost little.  Chunks are never shorter
-    // than one tile.
-    const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
-    return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
+    if (mixed || c->chunked_only || c->plan_chunk > 0) {
+        // Many short work items: a launch holds ~20 rounds of the resident
+        // workgroups, so the last, partly filled round and the costlier
+        // items (self-exclusion tiles; near-field tiles of the mixed kernel,
+        // hence its finer split) cost little.  Chunks are never shorter than
+        // one tile.
+        const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
+        return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
+    }
+    // float64: items cost the same, so a persistent grid of one workgroup
+    // per resident slot works through ~8 segments each (the queue evens out
+    // the self-exclusion tiles); partial rows number segments + tiles
+    // instead of tiles x chunks.  512-source tiles halve the LDS barriers.
+    const int recip = recip_for(c);
+    const int tile64 = c->plan_tile ? c->plan_tile : 512;
+    const int slots = pair_f64_blocks_per_cu(T, tile64, recip) * c->n_cu;
+    const int segments = c->plan_wg ? c->plan_wg : 8 * (slots > 0 ? slots : 2 * c->n_cu);
+    return make_segmented_plan(n_targets, n_src_pad, T, tile64, segments, recip, slots);
 }
 
 int update_plan(bve_ctx *c)
@@ -209,7 +245,7 @@ int update_plan(bve_ctx *c)
     const int T = c->plan.targets_per_lane;
     if (c->plan.n_tiles * kBlock * T > c->t_stride)
         return c->fail(BVE_ERR_STATE, "internal: target tiles exceed the padded target arrays");
-    return ensure_partial(c, (size_t)c->plan.n_chunks * 3 * c->t_stride);
+    return ensure_partial(c, partial_doubles(c->plan, c->t_stride));
 }
 
 // Resizes the target arrays for n_particles particles, keeping the panel part.
@@ -365,6 +401,7 @@ PairArgs pair_args_of(const bve_ctx *c, const PairPlan &plan)
     pa.n_src_pad = c->n_src_pad;
     pa.plan = plan;
     pa.partial = c->partial;
+    pa.queue = c->queue;
     return pa;
 }
 
@@ -418,7 +455,7 @@ int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPl
             if (rc) return rc;
             StageParams sp{};
             sp.stage = s;
-            sp.n_chunks = plan.n_chunks;
+            sp.plan = plan;
             sp.dt = dt;
             sp.omega = c->omega;
             sp.a_next = a_next[s];
@@ -487,7 +524,7 @@ int copy_active_state(bve_ctx *c, double *rows, bool save)
 int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
 {
     fwd = plan_for(c, c->n_active, c->n_src_pad);
-    int rc = ensure_partial(c, (size_t)fwd.n_chunks * 3 * c->t_stride);
+    int rc = ensure_partial(c, partial_doubles(fwd, c->t_stride));
     if (rc) return rc;
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_active);
     const int wg = c->plan_wg ? c->plan_wg : 16384;
@@ -594,6 +631,20 @@ int bve_create(bve_ctx **out, int device)
         delete c;
         return BVE_ERR_DEVICE;
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        c->n_cu = cus;
+    // the segment queue starts at zero and every segmented launch leaves it so
+    e = hipMalloc(&c->queue, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->queue, 0, 2 * sizeof(unsigned));
+    if (e != hipSuccess) {
+        g_create_error = std::string("bve_create: queue: ") + hipGetErrorString(e);
+        if (c->queue) (void)hipFree(c->queue);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return BVE_ERR_MEMORY;
+    }
     *out = c;
     return BVE_OK;
 }
@@ -611,6 +662,7 @@ int bve_destroy(bve_ctx *c)
     dev_free(c->srcc);
     dev_free(c->tile_box);
     dev_free(c->group_box);
+    if (c->queue) (void)hipFree(c->queue);
     for (auto &ev : c->pending) c->event_pool.push_back(ev);
     for (auto &ev : c->event_pool) {
         (void)hipEventDestroy(ev.first);
@@ -684,6 +736,18 @@ int bve_set_plan(bve_ctx *c, int targets_per_lane, int tile, int target_workgrou
     c->plan_tile = tile;
     c->plan_wg = target_workgroups;
     c->plan_recip = recip;
+    if (c->panels_loaded && c->x0) {
+        HIP_TRY(c, hipSetDevice(c->device));
+        return update_plan(c);
+    }
+    return BVE_OK;
+}
+
+int bve_set_chunk(bve_ctx *c, int chunk)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (chunk < 0) return c->fail(BVE_ERR_ARGUMENT, "bve_set_chunk: negative chunk %d", chunk);
+    c->plan_chunk = chunk;
     if (c->panels_loaded && c->x0) {
         HIP_TRY(c, hipSetDevice(c->device));
         return update_plan(c);
@@ -901,7 +965,7 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
     VTRY(hipMemcpyAsync(c->staging, x_targets, 3 * sizeof(double) * n_targets, hipMemcpyHostToDevice, c->stream));
     VTRY(launch_aos_to_soa(c->staging, n_targets, t, stride, 0, c->stream));
     const PairPlan plan = plan_for(c, n_targets, ns_pad);
-    rc = ensure_partial(c, (size_t)plan.n_chunks * 3 * stride);
+    rc = ensure_partial(c, partial_doubles(plan, stride));
     if (rc) { cleanup(); return rc; }
     PairArgs pa{};
     pa.tx = t;
@@ -916,10 +980,11 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
     pa.n_src_pad = ns_pad;
     pa.plan = plan;
     pa.partial = c->partial;
+    pa.queue = c->queue;
     rc = pair_pass(c, pa, n_targets, n_sources);
     if (rc) { (void)hipStreamSynchronize(c->stream); cleanup(); return rc; }
     if (dvort && self_end > 0) VTRY(dev_alloc(&dv, self_end));
-    VTRY(launch_finalize_velocity(c->partial, plan.n_chunks, t, stride, n_targets, c->omega, uu,
+    VTRY(launch_finalize_velocity(c->partial, plan, t, stride, n_targets, c->omega, uu,
                                   dv, self_end, c->stream));
     VTRY(hipMemcpyAsync(u, uu, 3 * sizeof(double) * n_targets, hipMemcpyDeviceToHost, c->stream));
     if (dv) VTRY(hipMemcpyAsync(dvort, dv, sizeof(double) * self_end, hipMemcpyDeviceToHost, c->stream));
@@ -1098,8 +1163,29 @@ int bve_get_plan(const bve_ctx *c, int *targets_per_lane, int *tile, int *chunk,
     if (targets_per_lane) *targets_per_lane = c->plan.targets_per_lane;
     if (tile) *tile = c->plan.tile;
     if (chunk) *chunk = c->plan.chunk;
-    if (n_chunks) *n_chunks = c->plan.n_chunks;
+    if (n_chunks) *n_chunks = c->plan.segments > 0 ? c->plan.segments : c->plan.n_chunks;
     if (recip) *recip = c->plan.recip;
+    return BVE_OK;
+}
+
+int bve_get_launch(const bve_ctx *c, int *segments, int *grid, int *n_tiles,
+                   long long *partial_bytes)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return BVE_ERR_STATE;
+    if (segments) *segments = c->plan.segments;
+    if (grid) *grid = c->plan.grid;
+    if (n_tiles) *n_tiles = c->plan.n_tiles;
+    if (partial_bytes) {
+        // rows actually written: chunked, every chunk's rows of all targets
+        // (padding included); segmented, one row per (segment, tile) piece,
+        // the last piece being row (last segment + last tile)
+        const PairPlan &p = c->plan;
+        const long long rows = (long long)(p.n_tiles * p.n_src_tiles - 1) / p.seg_len + p.n_tiles;
+        *partial_bytes = p.segments > 0
+                             ? rows * 3 * kBlock * p.targets_per_lane * (long long)sizeof(double)
+                             : (long long)p.n_chunks * 3 * c->t_stride * (long long)sizeof(double);
+    }
     return BVE_OK;
 }
 
@@ -1372,6 +1458,7 @@ int bve_ensemble_create(bve_ensemble **out, int device, int n_members)
             delete e;
             return rc;
         }
+        c->chunked_only = true;
         e->members.push_back(c);
     }
     hipDeviceProp_t prop;

@@ -40,9 +40,27 @@ struct PairPlan {
     int n_chunks;           // source chunks; partial sums per chunk
     int n_work;             // n_tiles * n_chunks
     int grid;               // launched workgroups (n_work rounded up to 8 XCDs)
+    // Segmented launch (segments > 0; chunk / n_chunks unused): the
+    // tile-major sequence of (target tile, source tile) items is cut into
+    // `segments` runs of seg_len items, which a persistent grid of `grid`
+    // workgroups takes from a queue.  A segment writes one partial row per
+    // target tile it touches, row (segment + target tile).
+    int segments;
+    int n_src_tiles;        // n_src_pad / tile
+    int seg_len;
 };
 
 __host__ __device__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Doubles of partial sums one pass with plan p writes: chunked, 3 rows of
+// t_stride per chunk; segmented, 3 rows of one target tile per (segment,
+// tile) piece, whose index segment + tile is below segments + n_tiles.
+__host__ __device__ inline size_t partial_doubles(const PairPlan &p, int t_stride)
+{
+    if (p.segments > 0)
+        return (size_t)(p.segments + p.n_tiles) * 3 * kBlock * p.targets_per_lane;
+    return (size_t)p.n_chunks * 3 * t_stride;
+}
 
 }  // namespace bve

@@ -141,11 +141,13 @@ __device__ __forceinline__ void tile_f64(const StagedSource *__restrict__ lds, i
     }
 }
 
-// Work item (target tile ttile, source chunk `chunk`) of a float64 pass;
-// lds holds TILE records.
+// Target tile ttile against sources [j_begin, j_end) (multiples of TILE) of
+// a float64 pass; the sums of target i go to out[i - out_shift] and the rows
+// out_stride and 2 out_stride after it.  lds holds TILE records.
 template <int T, int TILE, int R>
-__device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int ttile,
-                                              StagedSource *lds)
+__device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int j_begin,
+                                               int j_end, double *out, int out_stride,
+                                               int out_shift, StagedSource *lds)
 {
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
@@ -165,8 +167,6 @@ __device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int 
         sx[t] = sy[t] = sz[t] = 0.0;
     }
 
-    const int j_begin = chunk * a.plan.chunk;
-    const int j_end = min(j_begin + a.plan.chunk, a.n_src_pad);
     for (int jt = j_begin; jt < j_end; jt += TILE) {
         __syncthreads();
         stage_sources(lds, a.src + jt, TILE);
@@ -181,13 +181,63 @@ __device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int 
             tile_f64<T, TILE, R, false>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
     }
 
-    double *px = a.partial + (size_t)(chunk * 3) * stride;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int i = tile_base + t * kBlock + threadIdx.x;
-        px[i] = sx[t];
-        px[stride + i] = sy[t];
-        px[2 * (size_t)stride + i] = sz[t];
+        const int i = tile_base + t * kBlock + threadIdx.x - out_shift;
+        out[i] = sx[t];
+        out[out_stride + i] = sy[t];
+        out[2 * (size_t)out_stride + i] = sz[t];
+    }
+}
+
+// Work item (target tile ttile, source chunk `chunk`) of a chunked float64 pass.
+template <int T, int TILE, int R>
+__device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int ttile,
+                                              StagedSource *lds)
+{
+    const int j_begin = chunk * a.plan.chunk;
+    const int j_end = min(j_begin + a.plan.chunk, a.n_src_pad);
+    pair_f64_range<T, TILE, R>(a, ttile, j_begin, j_end, a.partial + (size_t)(chunk * 3) * a.t_stride,
+                               a.t_stride, 0, lds);
+}
+
+// Persistent body of a segmented float64 pass: the workgroup takes segments
+// from a.queue[0] until none are left.  Segment s covers items
+// [s seg_len, (s + 1) seg_len) of the tile-major sequence (item = target tile
+// * n_src_tiles + source tile) and writes row s + t for every target tile t
+// it touches.  The last workgroup to finish clears both counters for the
+// next launch.
+template <int T, int TILE, int R>
+__device__ __forceinline__ void pair_f64_segments(const PairArgs &a, StagedSource *lds,
+                                                  int *seg_slot)
+{
+    constexpr int TT = kBlock * T;
+    const PairPlan &p = a.plan;
+    const int nst = p.n_src_tiles;
+    const int n_items = p.n_tiles * nst;
+    for (;;) {
+        if (threadIdx.x == 0) *seg_slot = (int)atomicAdd(&a.queue[0], 1u);
+        __syncthreads();
+        const int seg = *seg_slot;
+        __syncthreads();
+        if (seg >= p.segments) break;
+        int b = seg * p.seg_len;
+        const int e = min(b + p.seg_len, n_items);
+        while (b < e) {
+            const int t = b / nst;
+            const int k0 = b - t * nst;
+            const int k1 = min(nst, k0 + (e - b));
+            pair_f64_range<T, TILE, R>(a, t, k0 * TILE, k1 * TILE,
+                                       a.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds);
+            b += k1 - k0;
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&a.queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&a.queue[0], 0u);
+            atomicExch(&a.queue[1], 0u);
+        }
     }
 }
 
@@ -458,17 +508,32 @@ __device__ __forceinline__ void mixed_target_prep_body(const double *__restrict_
 // next-stage inputs and source records, stage-1 kinetic energy.
 // ---------------------------------------------------------------------------
 
-// Sum of the chunk partials of target i, chunk 0 first.
-__device__ __forceinline__ void reduce_chunks(const double *__restrict__ partial, int n_chunks,
-                                              int stride, int i, double &Sx, double &Sy,
-                                              double &Sz)
+// Sum of the partials of target i of a pass with plan p, in a fixed order
+// (chunk 0 first, or the first segment touching i's tile first).
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ partial,
+                                                const PairPlan &p, int stride, int i,
+                                                double &Sx, double &Sy, double &Sz)
 {
     Sx = Sy = Sz = 0.0;
-    for (int c = 0; c < n_chunks; ++c) {
-        const double *p = partial + (size_t)(3 * c) * stride;
-        Sx += p[i];
-        Sy += p[stride + i];
-        Sz += p[2 * (size_t)stride + i];
+    if (p.segments > 0) {
+        const int tt = kBlock * p.targets_per_lane;
+        const int t = i / tt;
+        const int loc = i - t * tt;
+        const int first = t * p.n_src_tiles;
+        const int s_hi = (first + p.n_src_tiles - 1) / p.seg_len;
+        for (int s = first / p.seg_len; s <= s_hi; ++s) {
+            const double *q = partial + (size_t)(s + t) * 3 * tt;
+            Sx += q[loc];
+            Sy += q[tt + loc];
+            Sz += q[2 * tt + loc];
+        }
+        return;
+    }
+    for (int c = 0; c < p.n_chunks; ++c) {
+        const double *q = partial + (size_t)(3 * c) * stride;
+        Sx += q[i];
+        Sy += q[stride + i];
+        Sz += q[2 * (size_t)stride + i];
     }
 }
 
@@ -482,7 +547,7 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
     double ke_local = 0.0;
     if (i < st.n_targets) {
         double Sx, Sy, Sz;
-        reduce_chunks(st.partial, sp.n_chunks, stride, i, Sx, Sy, Sz);
+        reduce_partials(st.partial, sp.plan, stride, i, Sx, Sy, Sz);
         const double x = st.xs[i], y = st.xs[stride + i], z = st.xs[2 * stride + i];
         const double c4 = -4.0 * kPi;
         // u = sum_j w_j (x_i x x_j)/d_ij / (-4 PI)    (PanelVelocityParallel.f90:1332)

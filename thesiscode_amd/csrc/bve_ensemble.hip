@@ -55,7 +55,9 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(const int2 *__restrict__
 {
     __shared__ double sh[256];
     const int2 mb = map[blockIdx.x];
-    sp.n_chunks = n_chunks[mb.x];
+    // ensemble members run chunked plans
+    sp.plan.segments = 0;
+    sp.plan.n_chunks = n_chunks[mb.x];
     stage_update_body(states[mb.x], sp, mb.y, sh);
 }
 

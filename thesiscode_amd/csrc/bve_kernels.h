@@ -6,7 +6,10 @@ namespace bve {
 
 // One pair-sum pass: targets SoA (rows tx, ty, tz of length t_stride),
 // sources AoS records (n_src_pad, zero-weight padding), partial sums
-// partial[(chunk * 3 + c) * t_stride + i].
+// partial[(chunk * 3 + c) * t_stride + i] (chunked plans) or
+// partial[((segment + tile) * 3 + c) * tile_targets + i - tile * tile_targets]
+// (segmented plans, tile_targets = kBlock * targets_per_lane).  queue: two
+// counters, zero between launches (segmented plans only).
 struct PairArgs {
     const double *tx, *ty, *tz;
     int t_stride;
@@ -15,6 +18,7 @@ struct PairArgs {
     int n_src_pad;
     PairPlan plan;
     double *partial;
+    unsigned *queue;
 };
 
 // One mixed-precision pair-sum pass: the float64 pass's arguments plus the
@@ -81,7 +85,7 @@ struct StepState {
 
 struct StageParams {
     int stage;        // 0..3
-    int n_chunks;
+    PairPlan plan;    // plan of the pass whose partial sums the stage reduces
     double dt, omega;
     double a_next;    // 0.5, 0.5, 1.0 (next stage input x + a k); unused at stage 3
     double divisor;   // 6, 3, 3, 6
@@ -89,6 +93,9 @@ struct StageParams {
 };
 
 bool plan_supported(int targets_per_lane, int tile);
+// Workgroups of the segmented float64 pair kernel one CU holds at once (0 if
+// the occupancy query fails).
+int pair_f64_blocks_per_cu(int targets_per_lane, int tile, int recip);
 hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s);
 hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s);
 hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s);
@@ -102,7 +109,7 @@ hipError_t launch_soa_to_aos(const double *src, int stride, int off, int n, doub
                              hipStream_t s);
 hipError_t launch_make_sources(const double *xyz, const double *vort, const double *area, int n,
                                Source *src, hipStream_t s);
-hipError_t launch_finalize_velocity(const double *partial, int n_chunks, const double *t,
+hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, hipStream_t s);
 hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_t s);

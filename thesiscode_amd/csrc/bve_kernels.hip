@@ -21,11 +21,60 @@ __global__ __launch_bounds__(kBlock) void pair_sum_f64_kernel(PairArgs a)
     pair_f64_item<T, TILE, R>(a, chunk, work - chunk * a.plan.n_tiles, lds);
 }
 
+// Segmented pass: a persistent grid (about one workgroup per resident slot)
+// takes segments from the queue.
+template <int T, int TILE, int R>
+__global__ __launch_bounds__(kBlock) void pair_sum_f64_seg_kernel(PairArgs a)
+{
+    __shared__ StagedSource lds[TILE];
+    __shared__ int seg;
+    pair_f64_segments<T, TILE, R>(a, lds, &seg);
+}
+
 template <int T, int TILE, int R>
 static hipError_t launch_f64(const PairArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((pair_sum_f64_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0, s, a);
+    if (a.plan.segments > 0) {
+        if (!a.queue) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((pair_sum_f64_seg_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock),
+                           0, s, a);
+    } else {
+        hipLaunchKernelGGL((pair_sum_f64_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0,
+                           s, a);
+    }
     return hipGetLastError();
+}
+
+template <int T, int TILE, int R>
+static int seg_blocks_per_cu()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_sum_f64_seg_kernel<T, TILE, R>,
+                                                     kBlock, 0) != hipSuccess)
+        return 0;
+    return n;
+}
+
+template <int T, int TILE>
+static int seg_blocks_per_cu_r(int recip)
+{
+    switch (recip) {
+    case 1: return seg_blocks_per_cu<T, TILE, 1>();
+    case 2: return seg_blocks_per_cu<T, TILE, 2>();
+    case 4: return seg_blocks_per_cu<T, TILE, 4>();
+    default: return 0;
+    }
+}
+
+int pair_f64_blocks_per_cu(int T, int tile, int recip)
+{
+    if (T == 1 && tile == 256) return seg_blocks_per_cu_r<1, 256>(recip);
+    if (T == 2 && tile == 256) return seg_blocks_per_cu_r<2, 256>(recip);
+    if (T == 4 && tile == 256) return seg_blocks_per_cu_r<4, 256>(recip);
+    if (T == 1 && tile == 512) return seg_blocks_per_cu_r<1, 512>(recip);
+    if (T == 2 && tile == 512) return seg_blocks_per_cu_r<2, 512>(recip);
+    if (T == 4 && tile == 512) return seg_blocks_per_cu_r<4, 512>(recip);
+    return 0;
 }
 
 template <int T, int TILE>
@@ -159,9 +208,9 @@ hipError_t launch_make_sources(const double *xyz, const double *vort, const doub
     return hipGetLastError();
 }
 
-// Single evaluation: u (AoS) from the chunk partials and the SoA targets,
+// Single evaluation: u (AoS) from the pass's partials and the SoA targets,
 // and dVort = -2 Omega u_z for the first n_dvort targets.
-__global__ void finalize_velocity_kernel(const double *__restrict__ partial, int n_chunks,
+__global__ void finalize_velocity_kernel(const double *__restrict__ partial, PairPlan plan,
                                          const double *__restrict__ t, int stride, int n,
                                          double omega, double *__restrict__ u,
                                          double *__restrict__ dvort, int n_dvort)
@@ -169,7 +218,7 @@ __global__ void finalize_velocity_kernel(const double *__restrict__ partial, int
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double Sx, Sy, Sz;
-    reduce_chunks(partial, n_chunks, stride, i, Sx, Sy, Sz);
+    reduce_partials(partial, plan, stride, i, Sx, Sy, Sz);
     const double x = t[i], y = t[stride + i], z = t[2 * stride + i];
     const double c4 = -4.0 * kPi;
     const double uz = (x * Sy - y * Sx) / c4;
@@ -196,13 +245,13 @@ hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_finalize_velocity(const double *partial, int n_chunks, const double *t,
+hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(finalize_velocity_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, partial,
-                       n_chunks, t, stride, n, omega, u, dvort, n_dvort);
+                       plan, t, stride, n, omega, u, dvort, n_dvort);
     return hipGetLastError();
 }
 

@@ -118,6 +118,23 @@ int bve_load_panels(bve_ctx *ctx, int n_active, const double *x_active,
 int bve_rk4_step(bve_ctx *ctx, double dt, int n_particles, double *x_particles,
                  const double *absvort, double *relvort, double *total_ke, double *ke);
 
+/* The active panels' absolute vorticity (n_active values, gathered order),
+ * which the forced step needs: activePanels%absVort of
+ * PanelVelocityParallel.f90:686.  Cached until the next bve_load_panels. */
+int bve_load_panel_absvort(bve_ctx *ctx, const double *absvort_active);
+
+/* StratosphereRK4(aParticles, aPanels, t, dt, procRank, numProcs),
+ * PanelVelocityParallel.f90:661-919: the RK4 step with the Juckes forcing
+ * (TracerAndVorticityDistributions.f90:1223-1231).  Each stage's active
+ * vorticity is absVort - 2 Omega z - F(x, t_s) at the stage positions,
+ * t_s = t, t + dt/2, t + dt/2, t + dt; afterwards the active panels' and,
+ * if absvort is given, the particles' relative vorticity is
+ * absVort - 2 Omega z - F(x, t + dt) (:906-915).  Arguments otherwise as
+ * bve_rk4_step; needs bve_load_panel_absvort. */
+int bve_stratosphere_rk4_step(bve_ctx *ctx, double t, double dt, int n_particles,
+                              double *x_particles, const double *absvort, double *relvort,
+                              double *total_ke, double *ke);
+
 /* ScatterPanels payload (ParticlesEdgesPanels2.f90:1622-1637): current
  * active positions and relative vorticity and passive positions.  Any
  * pointer may be NULL. */

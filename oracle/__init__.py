@@ -39,6 +39,10 @@ def lib():
         L.bvo_voronoi_velocity.argtypes = [_i, _p, _p, _p, _p, _p, _i, _i, _d, _p, _p, _p, _p]
         L.bvo_rk4_step.argtypes = [_i, _p, _p, _p, _i, _p, _i, _p, _p, _p, _d, _d, _d, _p, _p]
         L.bvo_renormalize.argtypes = [_i, _p]
+        L.bvo_juckes_forcing.argtypes = [_p, _d]
+        L.bvo_juckes_forcing.restype = _d
+        L.bvo_stratosphere_rk4_step.argtypes = [_i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _d,
+                                                _d, _d, _d, _p, _p]
         L.bvo_velocity_q.argtypes = [_i, _p, _i, _p, _p, _i, _d, _p]
         L.bvo_jvp_active.argtypes = [_i, _p, _p, _p, _p, _p, _d, _p, _p]
         L.bvo_tlm_rk4.argtypes = [_i, _p, _p, _p, _p, _p, _d, _d, _i]
@@ -115,6 +119,29 @@ def rk4_step(active_x, active_vort, area, passive_x, particles_x, particles_absv
     tot = ctypes.c_double(0.0)
     lib().bvo_rk4_step(na, _ptr(ax), _ptr(av), _ptr(ar), npp, _ptr(px), np_, _ptr(qx), _ptr(qrel),
                        _ptr(qabs), dt, omega, smooth, _ptr(ke), ctypes.byref(tot))
+    return dict(active_x=ax, active_relvort=av, passive_x=px, particles_x=qx,
+                particles_relvort=qrel, ke=ke, total_ke=tot.value)
+
+
+def juckes_forcing(x, t):
+    """Juckes_Forcing at points x (N, 3) and time t."""
+    x = _c(x).reshape(-1, 3)
+    return np.array([lib().bvo_juckes_forcing(_ptr(np.ascontiguousarray(p)), t) for p in x])
+
+
+def stratosphere_rk4_step(active_x, active_absvort, area, passive_x, particles_x,
+                          particles_absvort, t, dt, omega, smooth):
+    """One StratosphereRK4 step from time t.  Returns a dict like rk4_step's;
+    active_relvort and particles_relvort are absVort - 2 Omega z - F(x, t+dt)."""
+    ax, aabs, ar = _c(active_x).copy(), _c(active_absvort), _c(area)
+    px, qx = _c(passive_x).copy(), _c(particles_x).copy()
+    qabs = _c(particles_absvort)
+    na, npp, np_ = len(aabs), len(px), len(qx)
+    av, qrel, ke = np.zeros(na), np.zeros(np_), np.zeros(na)
+    tot = ctypes.c_double(0.0)
+    lib().bvo_stratosphere_rk4_step(na, _ptr(ax), _ptr(av), _ptr(aabs), _ptr(ar), npp, _ptr(px),
+                                    np_, _ptr(qx), _ptr(qrel), _ptr(qabs), t, dt, omega, smooth,
+                                    _ptr(ke), ctypes.byref(tot))
     return dict(active_x=ax, active_relvort=av, passive_x=px, particles_x=qx,
                 particles_relvort=qrel, ke=ke, total_ke=tot.value)
 

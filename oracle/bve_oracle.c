@@ -218,6 +218,126 @@ void bvo_rk4_step(int na, double *ax, double *avort, const double *area,
     for (int s = 0; s < 4; ++s) { free(ka[s]); free(kv[s]); free(kp[s]); free(kq[s]); }
 }
 
+/* atan4, SphereGeom.f90:213-248: atan2 with values in [0, 2 PI), by the
+ * reference's case split (axes first, then the quadrant of atan2(|y|,|x|)). */
+static double bvo_atan4(double y, double x)
+{
+    if (x == 0.0) {
+        if (y > 0.0) return BVO_PI / 2.0;
+        if (y < 0.0) return 3.0 * BVO_PI / 2.0;
+        return 0.0;
+    }
+    if (y == 0.0) return x > 0.0 ? 0.0 : BVO_PI;
+    const double theta = atan2(fabs(y), fabs(x));
+    if (x > 0.0 && y > 0.0) return theta;
+    if (x < 0.0 && y > 0.0) return BVO_PI - theta;
+    if (x < 0.0 && y < 0.0) return BVO_PI + theta;
+    return 2.0 * BVO_PI - theta;
+}
+
+/* Juckes_A, TracerAndVorticityDistributions.f90:550-568: forcing ramp,
+ * up over [0, 4), 1 on [4, 11), down over [11, 15), 0 outside. */
+static double bvo_juckes_a(double t)
+{
+    const double tfull = 4.0, tend = 15.0, b = BVO_PI / tfull;
+    if (0.0 <= t && t < tfull) return 0.5 * (1.0 - cos(b * t));
+    if (tfull <= t && t < tend - tfull) return 1.0;
+    if (tend - tfull <= t && t < tend) return 0.5 * (1.0 - cos(b * (t - (tend - tfull)) + BVO_PI));
+    return 0.0;
+}
+
+/* Juckes_Forcing, TracerAndVorticityDistributions.f90:1223-1231, with
+ * Juckes_B (:606-617), Latitude (SphereGeom.f90:198-203) and Longitude
+ * (:176-181): F0 A(t) B(lat) cos(lon), F0 = 0.3 * 4 PI,
+ * B = tan^2(th1)/tan^2(lat) exp(1 - tan^2(th1)/tan^2(lat)) for lat > 0. */
+double bvo_juckes_forcing(const double *xyz, double t)
+{
+    const double f0 = 0.3 * 4.0 * BVO_PI;
+    const double lat = atan2(xyz[2], sqrt(xyz[0] * xyz[0] + xyz[1] * xyz[1]));
+    const double lon = bvo_atan4(xyz[1], xyz[0]);
+    double b = 0.0;
+    if (lat > 0.0) {
+        const double th1 = BVO_PI / 3.0;
+        const double tan1sq = tan(th1) * tan(th1);
+        b = tan1sq / (tan(lat) * tan(lat)) * exp(1.0 - tan1sq / (tan(lat) * tan(lat)));
+    }
+    return f0 * bvo_juckes_a(t) * b * cos(lon);
+}
+
+/* StratosphereRK4, PanelVelocityParallel.f90:661-919: the panel RK4 step
+ * with forcing.  Every stage's active vorticity is recomputed from the
+ * stage positions, absVort - 2 Omega z - F(x, t_s), at t_s = t, t + dt/2,
+ * t + dt/2, t + dt (:686-689, :744-747, :792-795, :837-840); the vorticity
+ * tendencies are not integrated.  After the update the active panels and
+ * the particles get relVort = absVort - 2 Omega z - F(x, t + dt)
+ * (:906-915); parent panels keep theirs.  Arrays as in bvo_rk4_step, plus
+ * the active panels' absolute vorticity aabs; avort is output only. */
+void bvo_stratosphere_rk4_step(int na, double *ax, double *avort, const double *aabs,
+                               const double *area, int npp, double *px, int np, double *qx,
+                               double *qrel, const double *qabs, double t, double dt,
+                               double omega, double smooth, double *ke, double *total_ke)
+{
+    const size_t a3 = 3 * (size_t)na, p3 = 3 * (size_t)npp, q3 = 3 * (size_t)np;
+    double *ain = malloc(sizeof(double) * a3), *vin = malloc(sizeof(double) * na);
+    double *pin = malloc(sizeof(double) * p3), *qin = malloc(sizeof(double) * q3);
+    double *dv = malloc(sizeof(double) * na);
+    double *ka[4], *kp[4], *kq[4];
+    for (int s = 0; s < 4; ++s) {
+        ka[s] = malloc(sizeof(double) * a3);
+        kp[s] = malloc(sizeof(double) * p3);
+        kq[s] = malloc(sizeof(double) * q3);
+    }
+    static const double stage_a[4] = {0.0, 0.5, 0.5, 1.0};
+    static const double stage_t[4] = {0.0, 0.5, 0.5, 1.0};
+    for (int s = 0; s < 4; ++s) {
+        if (s == 0) {
+            memcpy(ain, ax, sizeof(double) * a3);
+            memcpy(pin, px, sizeof(double) * p3);
+            memcpy(qin, qx, sizeof(double) * q3);
+        } else if (s < 3) {
+            for (size_t i = 0; i < a3; ++i) ain[i] = ax[i] + stage_a[s] * ka[s - 1][i];
+            for (size_t i = 0; i < q3; ++i) qin[i] = qx[i] + stage_a[s] * kq[s - 1][i];
+            for (size_t i = 0; i < p3; ++i) pin[i] = px[i] + stage_a[s] * kp[s - 1][i];
+        } else {
+            for (size_t i = 0; i < a3; ++i) ain[i] = ax[i] + ka[2][i];
+            for (size_t i = 0; i < q3; ++i) qin[i] = qx[i] + kq[2][i];
+            for (size_t i = 0; i < p3; ++i) pin[i] = px[i] + kp[2][i];
+        }
+        const double ts = s == 0 ? t : t + stage_t[s] * dt;
+        for (int j = 0; j < na; ++j)
+            vin[j] = aabs[j] - 2.0 * omega * ain[3 * j + 2] - bvo_juckes_forcing(ain + 3 * j, ts);
+        bvo_active_rhs(na, ain, vin, area, 0, na, omega, ka[s], dv);
+        bvo_passive_rhs(np, qin, na, ain, vin, area, 0, np, kq[s]);
+        bvo_smooth_velocity(npp, pin, na, ain, vin, area, 0, npp, smooth, kp[s]);
+        if (s == 0) {
+            double tot = 0.0;
+            for (int j = 0; j < na; ++j) {
+                const double *u = ka[0] + 3 * j;
+                double e = (u[0] * u[0] + u[1] * u[1]) + u[2] * u[2];
+                if (ke) ke[j] = e;
+                tot = tot + e * area[j];
+            }
+            if (total_ke) *total_ke = 0.5 * tot;
+        }
+        for (size_t i = 0; i < a3; ++i) ka[s][i] = dt * ka[s][i];
+        for (size_t i = 0; i < q3; ++i) kq[s][i] = dt * kq[s][i];
+        for (size_t i = 0; i < p3; ++i) kp[s][i] = dt * kp[s][i];
+    }
+    for (size_t i = 0; i < q3; ++i)
+        qx[i] = qx[i] + kq[0][i] / 6.0 + kq[1][i] / 3.0 + kq[2][i] / 3.0 + kq[3][i] / 6.0;
+    for (size_t i = 0; i < a3; ++i)
+        ax[i] = ax[i] + ka[0][i] / 6.0 + ka[1][i] / 3.0 + ka[2][i] / 3.0 + ka[3][i] / 6.0;
+    for (size_t i = 0; i < p3; ++i)
+        px[i] = px[i] + kp[0][i] / 6.0 + kp[1][i] / 3.0 + kp[2][i] / 3.0 + kp[3][i] / 6.0;
+    for (int j = 0; j < na; ++j)
+        avort[j] = aabs[j] - 2.0 * omega * ax[3 * j + 2] - bvo_juckes_forcing(ax + 3 * j, t + dt);
+    if (qrel && qabs)
+        for (int j = 0; j < np; ++j)
+            qrel[j] = qabs[j] - 2.0 * omega * qx[3 * j + 2] - bvo_juckes_forcing(qx + 3 * j, t + dt);
+    free(ain); free(vin); free(pin); free(qin); free(dv);
+    for (int s = 0; s < 4; ++s) { free(ka[s]); free(kp[s]); free(kq[s]); }
+}
+
 /* RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702: x / sqrt(x.x). */
 void bvo_renormalize(int n, double *x)
 {

@@ -20,6 +20,12 @@ def golden_manifest():
         return json.load(f)["cases"]
 
 
+def forced_manifest():
+    """Golden cases of the reference's StratosphereRK4 (forced step)."""
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        return json.load(f)["forced_cases"]
+
+
 def load_golden(name):
     """Arrays of one golden case (written by tests/golden/make_golden.py)."""
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:

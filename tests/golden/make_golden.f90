@@ -3,11 +3,13 @@
 ! cubed-sphere grid built by the reference's own grid code, and dumps the
 ! inputs and outputs of its RK4 step as raw little-endian float64.
 !
-! Usage:  make_golden LEVEL IC NSTEPS DT SMOOTH OUTFILE
+! Usage:  make_golden LEVEL IC NSTEPS DT SMOOTH OUTFILE [T0]
 !   IC = 0  the reference's Rossby-Haurwitz wave 4 (InitRH4Wave)
 !   IC = 1  RH4 with omega = K = 7.848e-6*86400, amplitude -30*K (the
 !           benchmark's parameterisation), assigned to panels and particles
 !   IC = 2  the reference's Gaussian vortex (InitGaussianVortex)
+!   IC = 3  the reference's stratosphere model (InitStratosphere), stepped
+!           with StratosphereRK4 (Juckes forcing) from time T0
 !
 ! Output layout (stream access, native little-endian):
 !   int32  nActive, nParticles, nPassive, nSteps
@@ -15,6 +17,7 @@
 !   state(0), state(1), state(nSteps)       (see write_state)
 !   float64 kineticEnergy(nActive)          stage-1 |u|^2 of step 1
 !   float64 totalKE(nSteps)                 per step
+!   IC = 3 only: float64 T0, active absVort(nActive)
 ! where a state is, in the order GatherPanels uses (panel index order):
 !   active  x(3,nActive) relVort(nActive) area(nActive)
 !   passive x(3,nPassive)
@@ -31,7 +34,7 @@ program make_golden
 	type(Edges), pointer :: ge
 	type(Panels), pointer :: gpan
 	integer(kint) :: level, ic, nSteps, step, ierr, rank, nprocs, j, nA, nP, nPP
-	real(kreal) :: dt, smoothArg, lambda, z, rhOmega
+	real(kreal) :: dt, smoothArg, lambda, z, rhOmega, t0
 	real(kreal), allocatable :: keStep1(:), keTotal(:)
 	character(len=512) :: arg, outFile
 	integer, parameter :: u = 21
@@ -46,6 +49,10 @@ program make_golden
 	call get_command_argument(4, arg); read(arg, *) dt
 	call get_command_argument(5, arg); read(arg, *) smoothArg
 	call get_command_argument(6, outFile)
+	t0 = 0.0_kreal
+	if (command_argument_count() >= 7) then
+		call get_command_argument(7, arg); read(arg, *) t0
+	endif
 
 	call New(gp, ge, gpan, QUAD_PANEL, level, 0, 3, BVE_SOLVER)
 	call SetSmooth(smoothArg)
@@ -69,6 +76,8 @@ program make_golden
 		enddo
 	case (2)
 		call InitGaussianVortex(gp, gpan)
+	case (3)
+		call InitStratosphere(gp, gpan)
 	case default
 		print *, 'make_golden: unknown IC ', ic
 		stop 1
@@ -86,7 +95,11 @@ program make_golden
 
 	call InitializeMPIRK4(gp, gpan, rank, nprocs)
 	do step = 1, nSteps
-		call BVERK4(gp, gpan, dt, rank, nprocs)
+		if (ic == 3) then
+			call StratosphereRK4(gp, gpan, t0 + real(step - 1, kreal)*dt, dt, rank, nprocs)
+		else
+			call BVERK4(gp, gpan, dt, rank, nprocs)
+		endif
 		keTotal(step) = totalKE
 		if (step == 1) then
 			keStep1 = kineticEnergy
@@ -96,6 +109,12 @@ program make_golden
 	call write_state()
 	write(u) keStep1
 	write(u) keTotal
+	if (ic == 3) then
+		write(u) t0
+		do j = 1, gpan%N
+			if (.not. gpan%hasChildren(j)) write(u) gpan%absVort(j)
+		enddo
+	endif
 	close(u)
 
 	call MPI_FINALIZE(ierr)

@@ -6,9 +6,13 @@ for a fixed list of cases and stores each case's inputs and outputs as one
 compressed ``.npz`` (float64 / int32 arrays only, loadable with
 ``allow_pickle=False``) plus ``manifest.json``.
 
+It also runs the reference's own drivers (RHWavePanelsMPI.f90,
+StratospherePanels.f90; unmodified, built by the same makefile's `drivers`
+target) on the committed namelists and keeps their .dat tables.
+
 Run from the repository root, in a container that has the reference:
 
-    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+    make -f oracle/Makefile.ref all drivers && python tests/golden/make_golden.py
 """
 import json
 import os
@@ -34,8 +38,47 @@ CASES = {
                      "cubed sphere L5, RH4 with omega=K=7.848e-6*86400, one RK4 step"),
 }
 
+# forced (StratosphereRK4) cases: name: (level, nsteps, dt, smooth, t0, description)
+FORCED_CASES = {
+    "strat_L4_3steps_t1": (4, 3, 0.01, 0.05, 1.0,
+                           "cubed sphere L4, InitStratosphere, 3 StratosphereRK4 steps from t=1 "
+                           "(forcing ramping up)"),
+    "strat_L3_2steps_t399": (3, 2, 0.01, 0.05, 3.99,
+                             "cubed sphere L3, InitStratosphere, 2 StratosphereRK4 steps from "
+                             "t=3.99 (stage times cross the end of the ramp at t=4)"),
+    "strat_L3_2steps_t5": (3, 2, 0.05, 0.10, 5.0,
+                           "cubed sphere L3, InitStratosphere, 2 StratosphereRK4 steps dt=0.05 "
+                           "from t=5 (full forcing), smooth=0.10"),
+}
 
-def read_dump(path):
+
+# reference drivers run unmodified on a committed namelist: name: (binary in
+# oracle/_ref, the namelist file name the driver opens); the namelist is
+# tests/golden/<name>.namelist and the driver's .dat becomes <name>.dat
+DRIVER_CASES = {
+    "rh4_driver_L4_noremesh": ("rh4_ref", "RHWavePanels.namelist"),
+    "rh4_driver_L4_remesh5": ("rh4_ref", "RHWavePanels.namelist"),
+    "strat_driver_L4": ("strat_ref", "StratospherePanels.namelist"),
+}
+
+
+def run_driver(tmp, name, binary, namelist):
+    """Run a reference driver (built by make -f oracle/Makefile.ref drivers)
+    in a scratch folder and copy its .dat table to tests/golden/<name>.dat."""
+    import glob
+    import shutil
+    work = os.path.join(tmp, name)
+    os.makedirs(os.path.join(work, "vtkOut"))
+    shutil.copy(os.path.join(HERE, name + ".namelist"), os.path.join(work, namelist))
+    subprocess.run([os.path.join(ROOT, "oracle", "_ref", binary)], check=True, cwd=work,
+                   stdout=subprocess.DEVNULL, timeout=600)
+    dat = glob.glob(os.path.join(work, "*.dat"))
+    assert len(dat) == 1, dat
+    shutil.copy(dat[0], os.path.join(HERE, name + ".dat"))
+    print(name, os.path.basename(dat[0]))
+
+
+def read_dump(path, forced=False):
     """Parse the stream file written by make_golden.f90."""
     raw = open(path, "rb").read()
     off = 0
@@ -64,44 +107,69 @@ def read_dump(path):
     s0, s1, sn = state(), state(), state()
     ke1 = take(na)
     ke_total = take(nsteps)
+    extra = {}
+    if forced:
+        extra["t0"] = take(1)
+        extra["active_absvort"] = take(na)
     assert off == len(raw), (off, len(raw))
     return dict(na=int(na), npart=int(npart), npp=int(npp), nsteps=int(nsteps),
                 dt=float(dt), smooth=float(smooth), omega=float(omega),
-                s0=s0, s1=s1, sn=sn, ke1=ke1, ke_total=ke_total)
+                s0=s0, s1=s1, sn=sn, ke1=ke1, ke_total=ke_total, extra=extra)
+
+
+def run_case(tmp, name, level, ic, nsteps, dt, smooth, t0=None):
+    """Run the harness for one case; write its .npz; return the parsed dump."""
+    out = os.path.join(tmp, name + ".bin")
+    cmd = [HARNESS, str(level), str(ic), str(nsteps), repr(dt), repr(smooth), out]
+    if t0 is not None:
+        cmd.append(repr(t0))
+    subprocess.run(cmd, check=True, cwd=tmp, stdout=subprocess.DEVNULL)
+    d = read_dump(out, forced=t0 is not None)
+    arrays = {}
+    for tag, st in (("s0", d["s0"]), ("s1", d["s1"]), ("sn", d["sn"])):
+        if tag == "sn" and d["nsteps"] == 1:
+            continue
+        for k, v in st.items():
+            arrays[f"{tag}_{k}"] = v
+    arrays["ke_step1"] = d["ke1"]
+    arrays["total_ke"] = d["ke_total"]
+    arrays.update(d["extra"])
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    print(name, d["na"], d["npart"], d["npp"])
+    return d
 
 
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -f oracle/Makefile.ref")
-    manifest = {}
+    manifest, forced = {}, {}
     with tempfile.TemporaryDirectory() as tmp:
         for name, (level, ic, nsteps, dt, smooth, desc) in CASES.items():
-            out = os.path.join(tmp, name + ".bin")
-            subprocess.run([HARNESS, str(level), str(ic), str(nsteps), repr(dt),
-                            repr(smooth), out], check=True, cwd=tmp,
-                           stdout=subprocess.DEVNULL)
-            d = read_dump(out)
-            arrays = {}
-            for tag, st in (("s0", d["s0"]), ("s1", d["s1"]), ("sn", d["sn"])):
-                if tag == "sn" and d["nsteps"] == 1:
-                    continue
-                for k, v in st.items():
-                    arrays[f"{tag}_{k}"] = v
-            arrays["ke_step1"] = d["ke1"]
-            arrays["total_ke"] = d["ke_total"]
-            np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+            d = run_case(tmp, name, level, ic, nsteps, dt, smooth)
             manifest[name] = dict(level=level, ic=ic, nsteps=d["nsteps"], dt=d["dt"],
                                   smooth=d["smooth"], omega=d["omega"],
                                   n_active=d["na"], n_particles=d["npart"],
                                   n_passive=d["npp"], description=desc)
-            print(name, d["na"], d["npart"], d["npp"])
+        for name, (level, nsteps, dt, smooth, t0, desc) in FORCED_CASES.items():
+            d = run_case(tmp, name, level, 3, nsteps, dt, smooth, t0)
+            forced[name] = dict(level=level, ic=3, nsteps=d["nsteps"], dt=d["dt"],
+                                smooth=d["smooth"], omega=d["omega"], t0=t0,
+                                n_active=d["na"], n_particles=d["npart"],
+                                n_passive=d["npp"], description=desc)
+        for name, (binary, namelist) in DRIVER_CASES.items():
+            run_driver(tmp, name, binary, namelist)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(dict(
             source="reference PanelVelocityParallelModule::BVERK4 (PanelVelocityParallel.f90:435-658), "
                    "grid from ParticlesEdgesPanelsModule::New (ParticlesEdgesPanels2.f90:362), "
                    "compiled with ROCm flang -O2 + MPICH, run on 1 rank",
             states="s0 = initial, s1 = after step 1, sn = after the last step (absent when nsteps == 1)",
-            cases=manifest), f, indent=1)
+            cases=manifest,
+            forced_source="reference PanelVelocityParallelModule::StratosphereRK4 "
+                          "(PanelVelocityParallel.f90:661-919) from InitStratosphere "
+                          "(TracerAndVorticityDistributions.f90:793-825); extra arrays t0 and "
+                          "active_absvort",
+            forced_cases=forced), f, indent=1)
 
 
 if __name__ == "__main__":

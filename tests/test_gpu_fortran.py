@@ -48,10 +48,10 @@ def test_fortran_host_matches_oracle(tmp_path):
     assert np.abs(got["particles_relvort"] - ref["particles_relvort"]).max() < 1e-12
 
 
-def run_dropin(tmp_path, namelist):
-    shutil.copy(os.path.join(GOLDEN, namelist), tmp_path / "RHWavePanels.namelist")
+def run_dropin(tmp_path, namelist, binary=DROPIN, namelist_name="RHWavePanels.namelist"):
+    shutil.copy(os.path.join(GOLDEN, namelist), tmp_path / namelist_name)
     (tmp_path / "vtkOut").mkdir()
-    r = subprocess.run([DROPIN], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    r = subprocess.run([binary], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, (r.stdout + r.stderr)[-2000:]
     dat = list(tmp_path.glob("*.dat"))
     assert len(dat) == 1
@@ -90,3 +90,20 @@ def test_reference_driver_remesh_on_the_dropin(tmp_path):
     assert np.allclose(got[5:, :2], ref[5:, :2], rtol=1e-8, atol=0)
     # error norms relative to their own size (~1e-3 .. 1e-6)
     assert np.allclose(got[5:, 2:], ref[5:, 2:], rtol=1e-5, atol=1e-12)
+
+
+STRAT_DROPIN = os.path.join(ROOT, "oracle", "_ref", "strat_gpu")
+
+
+@pytest.mark.skipif(not os.path.exists(STRAT_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_stratosphere_driver_runs_on_the_dropin(tmp_path):
+    """StratospherePanels (level 4, no AMR, 12 StratosphereRK4 steps at
+    dt = 0.01 from t = 0) on the GPU module reproduces the reference's .dat
+    table -- totalKE and enstrophy of every step -- to the printed 15
+    decimals."""
+    got = run_dropin(tmp_path, "strat_driver_L4.namelist", STRAT_DROPIN,
+                     "StratospherePanels.namelist")
+    ref = read_dat(os.path.join(GOLDEN, "strat_driver_L4.dat"))
+    assert got.shape == ref.shape
+    assert np.allclose(got, ref, rtol=1e-12, atol=2e-15), np.abs(got - ref).max(axis=0)

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import golden_manifest, load_golden, normwise_rel
+from conftest import forced_manifest, golden_manifest, load_golden, normwise_rel
 from thesiscode_amd import grid as gridmod
 from thesiscode_amd import initial_conditions as ic
 
@@ -207,3 +207,79 @@ def test_tlm_matches_central_differences_of_rk4(l4_state):
     assert normwise_rel(dxT, (xp - xm) / (2 * eps)) < 1e-7
     assert np.abs(dzT - (zp - zm) / (2 * eps)).max() < 1e-7 * np.abs(dzT).max()
     assert np.abs(xT - run(z)[0]).max() < 1e-14
+
+
+# ---- StratosphereRK4 (forced step) ----------------------------------------
+
+FORCED = sorted(forced_manifest())
+
+
+def run_forced_case(name, c, t_shift=0.0):
+    g = load_golden(name)
+    st = dict(ax=g["s0_active_x"], px=g["s0_passive_x"], qx=g["s0_particles_x"])
+    t0 = float(g["t0"][0]) + t_shift
+    out = []
+    for n in range(c["nsteps"]):
+        r = oracle.stratosphere_rk4_step(st["ax"], g["active_absvort"], g["s0_active_area"],
+                                         st["px"], st["qx"], g["s0_particles_absvort"],
+                                         t0 + n * c["dt"], c["dt"], c["omega"], c["smooth"])
+        st.update(ax=r["active_x"], px=r["passive_x"], qx=r["particles_x"])
+        out.append(r)
+    return g, out
+
+
+@pytest.mark.parametrize("name", FORCED)
+def test_forced_oracle_matches_reference_bitwise(name):
+    """bvo_stratosphere_rk4_step restates StratosphereRK4 (with Juckes_Forcing)
+    in the reference's rounding order: bit for bit, over every step."""
+    c = forced_manifest()[name]
+    g, out = run_forced_case(name, c)
+    r1, rn = out[0], out[-1]
+    for key in ("active_x", "active_relvort", "passive_x", "particles_x", "particles_relvort"):
+        assert np.array_equal(r1[key], g["s1_" + key]), (name, key)
+        assert np.array_equal(rn[key], g["sn_" + key]), (name, key)
+    assert np.array_equal(r1["ke"], g["ke_step1"])
+    assert np.array_equal([r["total_ke"] for r in out], g["total_ke"])
+
+
+def test_forced_golden_is_sensitive_to_the_stage_times():
+    """Planted mistake: the forcing evaluated 1e-3 later fails the golden
+    comparison, so the vectors pin the stage times t, t + dt/2, t + dt."""
+    name = "strat_L4_3steps_t1"
+    g, out = run_forced_case(name, forced_manifest()[name], t_shift=1e-3)
+    assert np.abs(out[0]["active_relvort"] - g["s1_active_relvort"]).max() > 1e-6
+
+
+def test_juckes_forcing_closed_form():
+    """F = 0.3 (4 PI) A(t) B(lat) cos(lon): B(60 N) = 1, A = 1 on [4, 11),
+    A(2) = A(13) = 1/2, A = 0 outside [0, 15), B = 0 south of the equator."""
+    lat = np.pi / 3
+    p = np.array([[np.cos(lat), 0, np.sin(lat)], [-np.cos(lat), 0, np.sin(lat)],
+                  [0, np.cos(lat), np.sin(lat)], [np.cos(lat), 0, -np.sin(lat)]])
+    f0 = 0.3 * 4 * np.pi
+    assert np.allclose(oracle.juckes_forcing(p, 5.0), [f0, -f0, 0.0, 0.0], rtol=1e-14, atol=1e-15)
+    for t, a in ((2.0, 0.5), (13.0, 0.5), (4.0, 1.0), (10.99, 1.0), (0.0, 0.0), (15.0, 0.0),
+                 (-1.0, 0.0)):
+        assert abs(oracle.juckes_forcing(p[:1], t)[0] - f0 * a) < 1e-14, t
+
+
+def test_unforced_stratosphere_step_is_the_bve_step():
+    """With the forcing switched off (t beyond the ramp, A = 0), recomputing
+    the stage vorticity as absVort - 2 Omega z equals carrying zeta + a k_zeta
+    (k_zeta = -2 Omega k_z exactly): the forced step reproduces BVERK4 to
+    rounding."""
+    g = gridmod.cubed_sphere(3)
+    omega = 2 * np.pi
+    z = ic.rh4_wave(g.active_x)
+    aabs = z + 2 * omega * g.active_x[:, 2]
+    qa = ic.rh4_wave(g.particles_x) + 2 * omega * g.particles_x[:, 2]
+    f = oracle.stratosphere_rk4_step(g.active_x, aabs, g.active_area, g.passive_x, g.particles_x,
+                                     qa, 20.0, 0.01, omega, 0.05)
+    r = oracle.rk4_step(g.active_x, z, g.active_area, g.passive_x, g.particles_x, qa, 0.01,
+                        omega, 0.05)
+    disp = np.abs(r["active_x"] - g.active_x).max()
+    for key in ("active_x", "passive_x", "particles_x"):
+        assert np.abs(f[key] - r[key]).max() < 1e-13 * disp, key
+    assert np.abs(f["active_relvort"] - r["active_relvort"]).max() < 1e-13 * np.abs(z).max()
+    assert np.abs(f["particles_relvort"] - r["particles_relvort"]).max() < 1e-13 * np.abs(z).max()
+    assert abs(f["total_ke"] - r["total_ke"]) <= 1e-14 * r["total_ke"]

@@ -33,6 +33,8 @@ SIGNATURES = {
     "bve_set_chunk": (_i, [_p, _i]),
     "bve_load_panels": (_i, [_p, _i, _p, _p, _p, _i, _p]),
     "bve_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
+    "bve_load_panel_absvort": (_i, [_p, _p]),
+    "bve_stratosphere_rk4_step": (_i, [_p, _d, _d, _i, _p, _p, _p, _p, _p]),
     "bve_store_panels": (_i, [_p, _p, _p, _p]),
     "bve_reset": (_i, [_p]),
     "bve_load_particles": (_i, [_p, _i, _p, _p]),
@@ -179,6 +181,23 @@ class Context:
         ke = np.zeros(self.n_active) if want_ke else None
         self._check(self._L.bve_rk4_step(self._h, dt, n, _ptr(qx), _ptr(qa), _ptr(rel), _ptr(tot),
                                          _ptr(ke)), "bve_rk4_step")
+        return qx.reshape(n, 3), rel, float(tot[0]), ke
+
+    def load_panel_absvort(self, absvort_active):
+        a = _f64(absvort_active, self.n_active)
+        self._check(self._L.bve_load_panel_absvort(self._h, _ptr(a)), "bve_load_panel_absvort")
+
+    def stratosphere_rk4_step(self, t, dt, particles_x, absvort=None, want_ke=True):
+        """One forced (StratosphereRK4) step from time t; returns like rk4_step."""
+        n = len(particles_x)
+        qx = _f64(particles_x, 3 * n).copy()
+        qa = _f64(absvort, n) if absvort is not None else None
+        rel = np.zeros(n) if absvort is not None else None
+        tot = np.zeros(1)
+        ke = np.zeros(self.n_active) if want_ke else None
+        self._check(self._L.bve_stratosphere_rk4_step(self._h, t, dt, n, _ptr(qx), _ptr(qa),
+                                                      _ptr(rel), _ptr(tot), _ptr(ke)),
+                    "bve_stratosphere_rk4_step")
         return qx.reshape(n, 3), rel, float(tot[0]), ke
 
     def store_panels(self):

@@ -60,6 +60,7 @@ struct bve_ctx {
     double *x0 = nullptr, *xs = nullptr, *acc = nullptr;
     double *zeta0 = nullptr, *zacc = nullptr, *area = nullptr;
     double *absvort = nullptr, *relvort = nullptr;
+    double *panel_absvort = nullptr;   // active panels' absolute vorticity (forced steps)
     Source *src = nullptr;
     double *partial = nullptr;
     size_t partial_cap = 0;   // doubles
@@ -193,6 +194,7 @@ void free_state(bve_ctx *c)
     dev_free(c->area);
     dev_free(c->absvort);
     dev_free(c->relvort);
+    dev_free(c->panel_absvort);
     dev_free(c->src);
     dev_free(c->ke);
     dev_free(c->ke_part);
@@ -435,17 +437,25 @@ int pair_pass(bve_ctx *c, const PairArgs &a, int n_targets, int n_sources)
 // n_steps RK4 steps of the first n_targets targets ([active | passive |
 // particles] prefix) with launch plan `plan`.  If stage_store is given, the
 // source records of every stage input are copied there, stage_stride records
-// apart, in step-major order (the adjoint's recomputation).
+// apart, in step-major order (the adjoint's recomputation).  forced: the
+// StratosphereRK4 variant, step n starting at time t + n dt.
 int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPlan &plan,
-                 Source *stage_store, size_t stage_stride)
+                 Source *stage_store, size_t stage_stride, bool forced = false, double t = 0.0)
 {
     StepState st = step_state(c);
     st.n_targets = n_targets;
+    if (forced) {
+        st.panel_absvort = c->panel_absvort;
+        st.t_step = t;
+        st.omega = c->omega;
+    }
     const PairArgs pa = pair_args_of(c, plan);
     static const double a_next[4] = {0.5, 0.5, 1.0, 0.0};
     static const double divisor[4] = {6.0, 3.0, 3.0, 6.0};
+    static const double t_frac[4] = {0.5, 0.5, 1.0, 1.0};
     HIP_TRY(c, launch_prepare_step(st, c->stream));
     for (int n = 0; n < n_steps; ++n) {
+        const double tn = t + n * dt;
         for (int s = 0; s < 4; ++s) {
             if (stage_store)
                 HIP_TRY(c, hipMemcpyAsync(stage_store + (size_t)(4 * n + s) * stage_stride, c->src,
@@ -461,6 +471,8 @@ int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPl
             sp.a_next = a_next[s];
             sp.divisor = divisor[s];
             sp.renormalize = c->renormalize;
+            // t + dt/2, t + dt/2, t + dt, t + dt (PanelVelocityParallel.f90:746, :794, :839, :908)
+            sp.t_next = t_frac[s] == 1.0 ? tn + dt : tn + t_frac[s] * dt;
             HIP_TRY(c, launch_stage_update(st, sp, c->stream));
         }
     }
@@ -878,6 +890,45 @@ int bve_rk4_step(bve_ctx *c, double dt, int n_particles, double *x_particles,
     int rc = bve_load_particles(c, n_particles, x_particles, absvort);
     if (rc) return rc;
     rc = run_steps(c, dt, 1);
+    if (rc) return rc;
+    rc = bve_store_particles(c, x_particles, relvort);
+    if (rc) return rc;
+    return bve_get_kinetic_energy(c, total_ke, ke);
+}
+
+int bve_load_panel_absvort(bve_ctx *c, const double *absvort_active)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded)
+        return c->fail(BVE_ERR_STATE, "bve_load_panel_absvort: no panels loaded");
+    if (!absvort_active) return c->fail(BVE_ERR_ARGUMENT, "bve_load_panel_absvort: null array");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->panel_absvort) HIP_TRY(c, dev_alloc(&c->panel_absvort, (size_t)c->n_active));
+    HIP_TRY(c, hipMemcpyAsync(c->panel_absvort, absvort_active, sizeof(double) * c->n_active,
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BVE_OK;
+}
+
+int bve_stratosphere_rk4_step(bve_ctx *c, double t, double dt, int n_particles,
+                              double *x_particles, const double *absvort, double *relvort,
+                              double *total_ke, double *ke)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded)
+        return c->fail(BVE_ERR_STATE, "bve_stratosphere_rk4_step: no panels loaded");
+    if (!c->panel_absvort)
+        return c->fail(BVE_ERR_STATE, "bve_stratosphere_rk4_step: no panel absolute vorticity "
+                       "(bve_load_panel_absvort)");
+    if (n_particles < 0 || (n_particles > 0 && !x_particles))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_stratosphere_rk4_step: bad particle arguments "
+                       "(n=%d)", n_particles);
+    if (relvort && !absvort)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_stratosphere_rk4_step: relvort requested without "
+                       "absvort");
+    int rc = bve_load_particles(c, n_particles, x_particles, absvort);
+    if (rc) return rc;
+    rc = run_steps_on(c, dt, 1, c->n_targets, c->plan, nullptr, 0, true, t);
     if (rc) return rc;
     rc = bve_store_particles(c, x_particles, relvort);
     if (rc) return rc;

@@ -151,7 +151,6 @@ __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int
 {
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    const int stride = a.t_stride;
     const TargetClasses cls = a.cls;
 
     double nx[T], ny[T], nz[T], c0[T], sx[T], sy[T], sz[T];
@@ -508,6 +507,53 @@ __device__ __forceinline__ void mixed_target_prep_body(const double *__restrict_
 // next-stage inputs and source records, stage-1 kinetic energy.
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Juckes forcing of the stratosphere model (TracerAndVorticityDistributions.f90)
+// ---------------------------------------------------------------------------
+
+// Longitude in [0, 2 PI) by the case split of atan4 (SphereGeom.f90:213-248).
+__device__ __forceinline__ double longitude_0_2pi(double y, double x)
+{
+    if (x == 0.0) return y > 0.0 ? kPi / 2.0 : (y < 0.0 ? 3.0 * kPi / 2.0 : 0.0);
+    if (y == 0.0) return x > 0.0 ? 0.0 : kPi;
+    const double theta = atan2(fabs(y), fabs(x));
+    if (x > 0.0) return y > 0.0 ? theta : 2.0 * kPi - theta;
+    return y > 0.0 ? kPi - theta : kPi + theta;
+}
+
+// F0 A(t) B(lat) cos(lon) (Juckes_Forcing, :1223-1231): F0 = 0.3 * 4 PI; the
+// ramp A (:550-568) rises over [0, 4), holds on [4, 11), falls over
+// [11, 15); B (:606-617) = tan^2(PI/3)/tan^2(lat) exp(1 - tan^2(PI/3)/tan^2(lat))
+// north of the equator, 0 south of it.
+__device__ __forceinline__ double juckes_forcing(double x, double y, double z, double t)
+{
+    const double tfull = 4.0, tend = 15.0, b = kPi / tfull;
+    double a = 0.0;
+    if (0.0 <= t && t < tfull)
+        a = 0.5 * (1.0 - cos(b * t));
+    else if (tfull <= t && t < tend - tfull)
+        a = 1.0;
+    else if (tend - tfull <= t && t < tend)
+        a = 0.5 * (1.0 - cos(b * (t - (tend - tfull)) + kPi));
+    const double lat = atan2(z, sqrt(x * x + y * y));
+    double bl = 0.0;
+    if (lat > 0.0) {
+        const double th1 = kPi / 3.0;
+        const double tan1sq = tan(th1) * tan(th1);
+        const double tl = tan(lat);
+        bl = tan1sq / (tl * tl) * exp(1.0 - tan1sq / (tl * tl));
+    }
+    return 0.3 * 4.0 * kPi * a * bl * cos(longitude_0_2pi(y, x));
+}
+
+// Relative vorticity of a forced step at position (x, y, z) and time t:
+// absVort - 2 Omega z - F(x, t) (PanelVelocityParallel.f90:686-689, :906-915).
+__device__ __forceinline__ double forced_relvort(double absvort, double omega, double x, double y,
+                                                 double z, double t)
+{
+    return absvort - 2.0 * omega * z - juckes_forcing(x, y, z, t);
+}
+
 // Sum of the partials of target i of a pass with plan p, in a fixed order
 // (chunk 0 first, or the first segment touching i's tile first).
 __device__ __forceinline__ void reduce_partials(const double *__restrict__ partial,
@@ -561,13 +607,16 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
         const double ay = base[stride + i] + ky / sp.divisor;
         const double az = base[2 * stride + i] + kz / sp.divisor;
         const bool active = i < st.n_active;
+        const bool forced = st.panel_absvort != nullptr;
         double zeta_next = 0.0, zacc = 0.0;
         if (active) {
-            const double dvort = -(2.0 * sp.omega * uz);   // :1333
-            const double kv = sp.dt * dvort;
-            zacc = ((sp.stage == 0) ? st.zeta0[i] : st.zacc[i]) + kv / sp.divisor;
-            st.zacc[i] = zacc;
-            zeta_next = st.zeta0[i] + sp.a_next * kv;
+            if (!forced) {
+                const double dvort = -(2.0 * sp.omega * uz);   // :1333
+                const double kv = sp.dt * dvort;
+                zacc = ((sp.stage == 0) ? st.zeta0[i] : st.zacc[i]) + kv / sp.divisor;
+                st.zacc[i] = zacc;
+                zeta_next = st.zeta0[i] + sp.a_next * kv;
+            }
             if (sp.stage == 0) {
                 const double e = (ux * ux + uy * uy) + uz * uz;   // :495-497
                 st.ke[i] = e;
@@ -585,7 +634,11 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
             st.xs[i] = nx;
             st.xs[stride + i] = ny;
             st.xs[2 * stride + i] = nz;
-            if (active) st.src[i] = Source{nx, ny, nz, zeta_next * st.area[i]};
+            if (active) {
+                if (forced)   // stage inputs' vorticity (:744-747, :792-795, :837-840)
+                    zeta_next = forced_relvort(st.panel_absvort[i], sp.omega, nx, ny, nz, sp.t_next);
+                st.src[i] = Source{nx, ny, nz, zeta_next * st.area[i]};
+            }
         } else {
             double nx = ax, ny = ay, nz = az;
             if (sp.renormalize) {   // RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702
@@ -601,13 +654,18 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
             st.xs[stride + i] = ny;
             st.xs[2 * stride + i] = nz;
             if (active) {
+                if (forced)   // :906-911
+                    zacc = forced_relvort(st.panel_absvort[i], sp.omega, nx, ny, nz, sp.t_next);
                 st.zeta0[i] = zacc;
                 st.src[i] = Source{nx, ny, nz, zacc * st.area[i]};
             }
-            // particles carry absolute vorticity: relVort = absVort - 2 Omega z (:651)
+            // particles carry absolute vorticity: relVort = absVort - 2 Omega z (:651),
+            // minus the forcing in a forced step (:912-915)
             if (st.particle_absvort && i >= st.particle_begin && i < st.particle_end) {
                 const int p = i - st.particle_begin;
-                st.particle_relvort[p] = st.particle_absvort[p] - 2.0 * sp.omega * nz;
+                st.particle_relvort[p] =
+                    forced ? forced_relvort(st.particle_absvort[p], sp.omega, nx, ny, nz, sp.t_next)
+                           : st.particle_absvort[p] - 2.0 * sp.omega * nz;
             }
         }
     }
@@ -638,7 +696,13 @@ __device__ __forceinline__ void prepare_step_body(const StepState &st, int blk)
     st.xs[i] = x;
     st.xs[stride + i] = y;
     st.xs[2 * stride + i] = z;
-    if (i < st.n_active) st.src[i] = Source{x, y, z, st.zeta0[i] * st.area[i]};
+    if (i < st.n_active) {
+        // a forced step's first stage uses absVort - 2 Omega z - F(x, t) (:686-689)
+        const double w = st.panel_absvort
+                             ? forced_relvort(st.panel_absvort[i], st.omega, x, y, z, st.t_step)
+                             : st.zeta0[i];
+        st.src[i] = Source{x, y, z, w * st.area[i]};
+    }
 }
 
 }  // namespace bve

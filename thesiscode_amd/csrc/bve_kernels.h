@@ -81,6 +81,11 @@ struct StepState {
     double *ke_total;
     const double *particle_absvort;   // may be null
     double *particle_relvort;
+    // Forced step (StratosphereRK4) when panel_absvort is set: every stage's
+    // active vorticity is absVort - 2 Omega z - F(x, t) at the stage input.
+    const double *panel_absvort;
+    double t_step;    // time at the start of the step
+    double omega;
 };
 
 struct StageParams {
@@ -90,6 +95,7 @@ struct StageParams {
     double a_next;    // 0.5, 0.5, 1.0 (next stage input x + a k); unused at stage 3
     double divisor;   // 6, 3, 3, 6
     int renormalize;
+    double t_next;    // forced step: time of the next stage input (of the result at stage 3)
 };
 
 bool plan_supported(int targets_per_lane, int tile);

@@ -14,6 +14,7 @@
 !     relVort = absVort - 2 Omega z is set after it (:651);
 !   * kineticEnergy / totalKE are those of stage 1 (:495-501), copied to
 !     panel tracer 2 when there are at least two tracers.
+! StratosphereRK4 (:661-919, Juckes forcing) runs on the same cached panels.
 ! procRank and numProcs are accepted; one GPU computes every target.
 module PanelVelocityParallelModule
 
@@ -138,6 +139,8 @@ subroutine InitRK4(aParticles, aPanels)
 	call check(bve_set_smooth(ctx, smooth), 'InitRK4')
 	call check(bve_load_panels(ctx, int(nActive, c_int), activePanels%x, activePanels%relVort, &
 		activePanels%area, int(nPassive, c_int), passivePanels%x), 'InitRK4')
+	! the forced step (StratosphereRK4) works from the panels' absolute vorticity
+	call check(bve_load_panel_absvort(ctx, activePanels%absVort), 'InitRK4')
 	allocate(kineticEnergy(nActive))
 	kineticEnergy = 0.0_kreal
 	rk4IsReady = .True.
@@ -174,16 +177,30 @@ end subroutine
 subroutine NotOnGPU(name)
 	character(len=*), intent(in) :: name
 	call LogMessage(log, ERROR_LOGGING_LEVEL, logKey, name// &
-		' (analytic-wind / forced variants) is not provided by the GPU module.')
+		' (analytic-wind advection) is not provided by the GPU module.')
 	error stop 1
 end subroutine
 
+! StratosphereRK4 (PanelVelocityParallel.f90:661-919): the same cached
+! panels and stage loop on the GPU, with every stage's active vorticity
+! absVort - 2 Omega z - F(x, t_s) and the Juckes forcing F; afterwards the
+! active panels' and particles' relVort = absVort - 2 Omega z - F(x, t + dt).
 subroutine StratosphereRK4(aParticles, aPanels, t, dt, procRank, numProcs)
 	type(Particles), intent(inout) :: aParticles
 	type(Panels), intent(inout) :: aPanels
 	real(kreal), intent(in) :: t, dt
 	integer(kint), intent(in) :: procRank, numProcs
-	call NotOnGPU('StratosphereRK4')
+	integer(kint) :: n
+	if (.not. c_associated(ctx)) call InitializeMPIRK4(aParticles, aPanels, procRank, numProcs)
+	if (.not. rk4IsReady) call InitRK4(aParticles, aPanels)
+	n = aParticles%N
+	call check(bve_set_omega(ctx, Omega), 'StratosphereRK4')
+	call check(bve_stratosphere_rk4_step(ctx, t, dt, int(n, c_int), aParticles%x(:,1:n), &
+		aParticles%absVort(1:n), aParticles%relVort(1:n), totalKE, kineticEnergy), 'StratosphereRK4')
+	call check(bve_store_panels(ctx, activePanels%x, activePanels%relVort, passivePanels%x), &
+		'StratosphereRK4')
+	if (GetNTracer(aPanels) >= 2) activePanels%tracer(:,2) = kineticEnergy
+	call ScatterPanels(aPanels, activePanels, activeMap, passivePanels, passiveMap)
 end subroutine
 
 subroutine AdvectionRK4(aParticles, aPanels, t, dt, procRank, numProcs)

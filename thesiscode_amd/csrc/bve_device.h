@@ -344,19 +344,44 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
 #pragma unroll
         for (int t = 0; t < T; ++t) Sx[t] = Sy[t] = Sz[t] = Cx[t] = Cy[t] = Cz[t] = 0.f;
         if (!careful) {
-#pragma unroll 2
-            for (int k = 0; k < TILE; ++k) {
-                const float4 s = lds[k];
-                const float cj = ldsc[k];
+            // Four sources share one v_rcp_f32 (the slow transcendental
+            // path) of the product of their denominators:
+            // 1/D_a = D_b D_c D_d / (D_a D_b D_c D_d), at the price of ~2 ulp
+            // more rounding per term.  Far-field denominators lie in
+            // [r_nf^2 / 2, 2 + eps2] (and above ~1e-6 for any r_nf down to
+            // level-10 spacing), so products of four stay inside the float32
+            // range.
+            for (int k = 0; k < TILE; k += 4) {
+                float4 s[4];
+                float cj[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    s[r] = lds[k + r];
+                    cj[r] = ldsc[k + r];
+                }
 #pragma unroll
                 for (int t = 0; t < T; ++t) {
-                    const float dx = ax[t] - s.x, dy = ay[t] - s.y, dz = az[t] - s.z;
-                    const float h = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    const float D = __builtin_fmaf(0.5f, h, cif[t] + cj);
-                    const float q = s.w * __builtin_amdgcn_rcpf(D);
-                    add_f32<KAHAN>(Sx[t], Cx[t], -q * dx);
-                    add_f32<KAHAN>(Sy[t], Cy[t], -q * dy);
-                    add_f32<KAHAN>(Sz[t], Cz[t], -q * dz);
+                    float dx[4], dy[4], dz[4], D[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        dx[r] = ax[t] - s[r].x;
+                        dy[r] = ay[t] - s[r].y;
+                        dz[r] = az[t] - s[r].z;
+                        const float h =
+                            __builtin_fmaf(dz[r], dz[r], __builtin_fmaf(dy[r], dy[r], dx[r] * dx[r]));
+                        D[r] = __builtin_fmaf(0.5f, h, cif[t] + cj[r]);
+                    }
+                    const float p01 = D[0] * D[1], p23 = D[2] * D[3];
+                    const float rr = __builtin_amdgcn_rcpf(p01 * p23);
+                    const float r01 = rr * p23, r23 = rr * p01;
+                    const float q[4] = {s[0].w * (D[1] * r01), s[1].w * (D[0] * r01),
+                                        s[2].w * (D[3] * r23), s[3].w * (D[2] * r23)};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        add_f32<KAHAN>(Sx[t], Cx[t], -q[r] * dx[r]);
+                        add_f32<KAHAN>(Sy[t], Cy[t], -q[r] * dy[r]);
+                        add_f32<KAHAN>(Sz[t], Cz[t], -q[r] * dz[r]);
+                    }
                 }
             }
         } else {

@@ -59,6 +59,8 @@ DRIVER_CASES = {
     "rh4_driver_L4_noremesh": ("rh4_ref", "RHWavePanels.namelist"),
     "rh4_driver_L4_remesh5": ("rh4_ref", "RHWavePanels.namelist"),
     "strat_driver_L4": ("strat_ref", "StratospherePanels.namelist"),
+    "jet_driver_L4_amr2": ("jet_ref", "JetPanels.namelist"),
+    "jet_driver_L4_amr2_remesh5": ("jet_ref", "JetPanels.namelist"),
 }
 
 

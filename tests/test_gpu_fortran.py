@@ -1,6 +1,7 @@
 """GPU: the Fortran host over the ISO_C_BINDING interface, and the
-reference's own RH4 driver (RHWavePanelsMPI.f90, unmodified) linked against
-the Fortran drop-in PanelVelocityParallelModule, against the oracle and the
+reference's own drivers (RHWavePanelsMPI.f90, StratospherePanels.f90,
+JetPanels.f90 with adaptive refinement; unmodified) linked against the
+Fortran drop-in PanelVelocityParallelModule, against the oracle and the
 reference's own output."""
 import os
 import shutil
@@ -107,3 +108,36 @@ def test_reference_stratosphere_driver_runs_on_the_dropin(tmp_path):
     ref = read_dat(os.path.join(GOLDEN, "strat_driver_L4.dat"))
     assert got.shape == ref.shape
     assert np.allclose(got, ref, rtol=1e-12, atol=2e-15), np.abs(got - ref).max(axis=0)
+
+
+JET_DROPIN = os.path.join(ROOT, "oracle", "_ref", "jet_gpu")
+
+
+@pytest.mark.skipif(not os.path.exists(JET_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_amr_jet_driver_runs_on_the_dropin(tmp_path):
+    """JetPanels with adaptive refinement of the initial grid (level 4 plus
+    two AMR levels: 3,828 active panels of non-uniform size, 10 steps, no
+    remesh) on the GPU module reproduces the reference's .dat table to the
+    printed 15 decimals."""
+    got = run_dropin(tmp_path, "jet_driver_L4_amr2.namelist", JET_DROPIN, "JetPanels.namelist")
+    ref = read_dat(os.path.join(GOLDEN, "jet_driver_L4_amr2.dat"))
+    assert got.shape == ref.shape
+    assert np.allclose(got, ref, rtol=1e-12, atol=2e-15), np.abs(got - ref).max(axis=0)
+
+
+@pytest.mark.skipif(not os.path.exists(JET_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_amr_jet_driver_remesh_on_the_dropin(tmp_path):
+    """The same with an adaptive remesh every 5 steps (the grid is re-refined,
+    ResetRK4, new panel counts re-gathered onto the device).  Rows before the
+    first remesh match to the printed precision.  The remesh is host code
+    that can amplify rounding-level differences (see the RH4 remesh test);
+    here later rows measured within 6e-13, held to 1e-9."""
+    got = run_dropin(tmp_path, "jet_driver_L4_amr2_remesh5.namelist", JET_DROPIN,
+                     "JetPanels.namelist")
+    ref = read_dat(os.path.join(GOLDEN, "jet_driver_L4_amr2_remesh5.dat"))
+    assert got.shape == ref.shape
+    assert np.allclose(got[:5], ref[:5], rtol=1e-12, atol=2e-15)
+    print("post-remesh max relative difference", np.abs(got[5:] / ref[5:] - 1).max(axis=0))
+    assert np.allclose(got[5:], ref[5:], rtol=1e-9, atol=0)

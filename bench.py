@@ -251,6 +251,33 @@ def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all, sample=None):
                 traffic_bytes_per_launch=traffic)
 
 
+def host_roundtrip_measurement(ctx, grid, absv, dt, steps, pairs_primary, mode="fp64_fast"):
+    """The drop-in's per-step pattern (the Fortran BVERK4): particles and
+    their absolute vorticity uploaded, one RK4 step, particles, their
+    relative vorticity and KE downloaded (bve_rk4_step), then the panels
+    downloaded for ScatterPanels (bve_store_panels).  Host arrays are
+    ordinary (pageable) memory, as Fortran's are."""
+    from thesiscode_amd import _lib
+    ctx.set_precision(MODES[mode])
+    qx = np.ascontiguousarray(grid.particles_x).copy()
+    qa = np.ascontiguousarray(absv)
+    rel, ke = np.zeros(grid.n_particles), np.zeros(grid.n_active)
+    ax, av = np.zeros((grid.n_active, 3)), np.zeros(grid.n_active)
+    px = np.zeros((grid.n_passive, 3))
+    ctx.rk4_step_into(dt, qx, qa, rel, ke)
+    ctx.store_panels_into(ax, av, px)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.rk4_step_into(dt, qx, qa, rel, ke)
+        ctx.store_panels_into(ax, av, px)
+    el = (time.perf_counter() - t0) / steps
+    ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    up = 32.0 * grid.n_particles
+    down = 32.0 * grid.n_particles + 32.0 * grid.n_active + 24.0 * grid.n_passive + 8.0 * grid.n_active
+    return dict(mode=mode, ms_per_step=el * 1e3, pair_rate_pcie_inclusive=pairs_primary / el,
+                host_bytes_up_per_step=up, host_bytes_down_per_step=down, steps=steps)
+
+
 def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed_kahan"),
                             dt=0.002):
     """Large single solve: two co-rotating Gaussian vortices (amplitude 4 PI,
@@ -324,6 +351,9 @@ def main():
     for mode in [m for m in args.modes.split(",") if m]:
         results[mode] = measure(ctx, g, za, mode, args, pairs_primary, pairs_all, sample)
         print(f"# {mode}: {results[mode]}", file=sys.stderr, flush=True)
+    roundtrip = host_roundtrip_measurement(ctx, g, absv, args.dt, max(args.steps // 2, 1),
+                                           pairs_primary)
+    print(f"# host round trip: {roundtrip}", file=sys.stderr, flush=True)
     ctx.close()
 
     ok = [m for m in results if results[m]["rel_err_vs_fp64_oracle"] <= ERR_BAR]
@@ -348,6 +378,7 @@ def main():
             "dt": args.dt, "mode": best,
             "pairs_per_step": pairs_primary, "pairs_per_step_all_classes": pairs_all,
             "modes": results,
+            "host_roundtrip": roundtrip,
         },
         "roofline": {
             "bound": "valu",

@@ -367,3 +367,28 @@ def test_errors_are_reported(ctx):
         ctx.set_plan(3, 256, 0)
     with pytest.raises(_lib.BVEError, match="bad target classes"):
         ctx.velocity(np.zeros((4, 3)), np.zeros((2, 3)), np.ones(2), np.ones(2), self_end=3)
+
+
+def test_step_into_caller_arrays_matches_step(ctx):
+    """The caller-owned-array step (pinned staging, one wait per call)
+    equals the allocating one bit for bit, over several steps."""
+    g = gridmod.cubed_sphere(4)
+    z = ic.rh4_wave(g.active_x)
+    qa = ic.rh4_wave(g.particles_x) + 4 * np.pi * g.particles_x[:, 2]
+    ctx.reset()
+    ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+    qx1 = g.particles_x
+    for _ in range(3):
+        qx1, rel1, tot1, ke1 = ctx.rk4_step(0.01, qx1, qa)
+    a1 = ctx.store_panels()
+    ctx.reset()
+    ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+    qx2 = np.ascontiguousarray(g.particles_x).copy()
+    rel2, ke2 = np.zeros(g.n_particles), np.zeros(g.n_active)
+    ax, av, px = np.zeros((g.n_active, 3)), np.zeros(g.n_active), np.zeros((g.n_passive, 3))
+    for _ in range(3):
+        tot2 = ctx.rk4_step_into(0.01, qx2, qa, rel2, ke2)
+        ctx.store_panels_into(ax, av, px)
+    assert np.array_equal(qx1, qx2) and np.array_equal(rel1, rel2) and tot1 == tot2
+    assert np.array_equal(ke1, ke2)
+    assert all(np.array_equal(a, b) for a, b in zip(a1, (ax, av, px)))

@@ -183,6 +183,28 @@ class Context:
                                          _ptr(ke)), "bve_rk4_step")
         return qx.reshape(n, 3), rel, float(tot[0]), ke
 
+    def rk4_step_into(self, dt, particles_x, absvort, relvort, ke):
+        """rk4_step on caller-owned arrays (the Fortran drop-in's pattern):
+        particles_x (n, 3) float64 C-contiguous is updated in place, relvort
+        (n) and ke (n_active) are written; returns totalKE."""
+        n = len(particles_x)
+        for a, size in ((particles_x, 3 * n), (absvort, n), (relvort, n), (ke, self.n_active)):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != size:
+                raise ValueError("rk4_step_into needs C-contiguous float64 arrays of the right size")
+        tot = np.zeros(1)
+        self._check(self._L.bve_rk4_step(self._h, dt, n, _ptr(particles_x), _ptr(absvort),
+                                         _ptr(relvort), _ptr(tot), _ptr(ke)), "bve_rk4_step")
+        return float(tot[0])
+
+    def store_panels_into(self, active_x, active_relvort, passive_x):
+        """store_panels into caller-owned C-contiguous float64 arrays."""
+        for a, size in ((active_x, 3 * self.n_active), (active_relvort, self.n_active),
+                        (passive_x, 3 * self.n_passive)):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.size != size:
+                raise ValueError("store_panels_into needs C-contiguous float64 arrays")
+        self._check(self._L.bve_store_panels(self._h, _ptr(active_x), _ptr(active_relvort),
+                                             _ptr(passive_x)), "bve_store_panels")
+
     def load_panel_absvort(self, absvort_active):
         a = _f64(absvort_active, self.n_active)
         self._check(self._L.bve_load_panel_absvort(self._h, _ptr(a)), "bve_load_panel_absvort")

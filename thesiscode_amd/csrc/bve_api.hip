@@ -51,6 +51,18 @@ struct bve_ctx {
     int n_cu = 256;              // compute units of the device
     unsigned *queue = nullptr;   // segment queue of the segmented pair pass (2 counters)
 
+    // Pinned host arena for the per-step host transfers: uploads are copied
+    // in and sent asynchronously; downloads land here and are copied to the
+    // caller's arrays after the one synchronisation of the call (host_sync).
+    char *pinned = nullptr;
+    size_t pinned_cap = 0, pinned_used = 0;
+    struct Download {
+        void *host;
+        const char *pinned;
+        size_t bytes;
+    };
+    std::vector<Download> downloads;
+
     // problem sizes
     bool panels_loaded = false;
     int n_active = 0, n_passive = 0, n_particles = 0;
@@ -251,9 +263,17 @@ int update_plan(bve_ctx *c)
 }
 
 // Resizes the target arrays for n_particles particles, keeping the panel part.
+int pinned_reserve(bve_ctx *c, size_t bytes);
+
 int ensure_particles(bve_ctx *c, int n_particles)
 {
     if (n_particles == c->n_particles && c->x0) return BVE_OK;
+    // one step's host traffic through the pinned arena: particles and their
+    // absolute vorticity up; particles, relative vorticity, the panels and
+    // the kinetic energy down (plus per-transfer alignment)
+    int rc = pinned_reserve(c, 64 * (size_t)n_particles + 48 * (size_t)c->n_active +
+                                   24 * (size_t)c->n_passive + 4096);
+    if (rc) return rc;
     const int n_panel = c->n_active + c->n_passive;
     const int nt = n_panel + n_particles;
     const int stride = round_up(std::max(nt, 1), kTargetAlign);
@@ -582,27 +602,102 @@ int adjoint_reverse_step(bve_ctx *c, const Source *stages, const PairPlan &vjp)
     return BVE_OK;
 }
 
+// Waits for the context's stream, then completes the pending downloads and
+// frees the pinned arena for reuse.
+int host_sync(bve_ctx *c)
+{
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (const auto &d : c->downloads) std::memcpy(d.host, d.pinned, d.bytes);
+    c->downloads.clear();
+    c->pinned_used = 0;
+    return BVE_OK;
+}
+
+// `bytes` of the pinned arena; when it is full, the transfers in flight
+// are completed first and the arena grown to hold everything that was
+// asked for since the last wait, so a repeated call pattern (one step's
+// uploads and downloads) soon fits without waiting midway.
+int pinned_take(bve_ctx *c, size_t bytes, char **out)
+{
+    const size_t need = (bytes + 255) / 256 * 256;
+    if (c->pinned_used + need > c->pinned_cap) {
+        const size_t demand = c->pinned_used + need;
+        int rc = host_sync(c);
+        if (rc) return rc;
+        if (c->pinned_cap < demand) {
+            if (c->pinned) (void)hipHostFree(c->pinned);
+            c->pinned = nullptr;
+            const size_t cap = std::max({demand, 2 * c->pinned_cap, (size_t)4 << 20});
+            c->pinned_cap = 0;
+            HIP_TRY(c, hipHostMalloc((void **)&c->pinned, cap, hipHostMallocDefault));
+            c->pinned_cap = cap;
+        }
+    }
+    *out = c->pinned + c->pinned_used;
+    c->pinned_used += need;
+    return BVE_OK;
+}
+
+// Makes the pinned arena hold `bytes` without growing later (pinning host
+// memory takes milliseconds: done when the problem size is set, not per step).
+int pinned_reserve(bve_ctx *c, size_t bytes)
+{
+    if (c->pinned_cap >= bytes) return BVE_OK;
+    int rc = host_sync(c);
+    if (rc) return rc;
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_cap = 0;
+    HIP_TRY(c, hipHostMalloc((void **)&c->pinned, bytes, hipHostMallocDefault));
+    c->pinned_cap = bytes;
+    return BVE_OK;
+}
+
+// Host -> device through the pinned arena (the host array is free for the
+// caller as soon as this returns).
+int host_up(bve_ctx *c, void *dev, const void *host, size_t bytes)
+{
+    if (bytes == 0) return BVE_OK;
+    char *p = nullptr;
+    int rc = pinned_take(c, bytes, &p);
+    if (rc) return rc;
+    std::memcpy(p, host, bytes);
+    HIP_TRY(c, hipMemcpyAsync(dev, p, bytes, hipMemcpyHostToDevice, c->stream));
+    return BVE_OK;
+}
+
+// Device -> host through the pinned arena; the host array is written by the
+// next host_sync.
+int host_down(bve_ctx *c, void *host, const void *dev, size_t bytes)
+{
+    if (bytes == 0 || !host) return BVE_OK;
+    char *p = nullptr;
+    int rc = pinned_take(c, bytes, &p);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(p, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    c->downloads.push_back({host, p, bytes});
+    return BVE_OK;
+}
+
 int upload_aos(bve_ctx *c, const double *host, int n, double *dst_rows, int stride, int off)
 {
     if (n <= 0) return BVE_OK;
     int rc = ensure_staging(c, 3 * (size_t)n);
     if (rc) return rc;
-    HIP_TRY(c, hipMemcpyAsync(c->staging, host, 3 * sizeof(double) * n, hipMemcpyHostToDevice,
-                              c->stream));
+    rc = host_up(c, c->staging, host, 3 * sizeof(double) * n);
+    if (rc) return rc;
     HIP_TRY(c, launch_aos_to_soa(c->staging, n, dst_rows, stride, off, c->stream));
     return BVE_OK;
 }
 
+// Enqueues the download; the host array is written by the next host_sync.
 int download_aos(bve_ctx *c, const double *src_rows, int stride, int off, int n, double *host)
 {
     if (n <= 0 || !host) return BVE_OK;
     int rc = ensure_staging(c, 3 * (size_t)n);
     if (rc) return rc;
     HIP_TRY(c, launch_soa_to_aos(src_rows, stride, off, n, c->staging, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(host, c->staging, 3 * sizeof(double) * n, hipMemcpyDeviceToHost,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return BVE_OK;
+    return host_down(c, host, c->staging, 3 * sizeof(double) * n);
 }
 
 }  // namespace
@@ -675,6 +770,7 @@ int bve_destroy(bve_ctx *c)
     dev_free(c->tile_box);
     dev_free(c->group_box);
     if (c->queue) (void)hipFree(c->queue);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     for (auto &ev : c->pending) c->event_pool.push_back(ev);
     for (auto &ev : c->event_pool) {
         (void)hipEventDestroy(ev.first);
@@ -776,7 +872,9 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
         return c->fail(BVE_ERR_ARGUMENT, "bve_load_panels: bad sizes or null arrays "
                        "(n_active=%d, n_passive=%d)", n_active, n_passive);
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->downloads.clear();
+    int rc = host_sync(c);   // uploads in flight into the old arrays
+    if (rc) return rc;
     free_state(c);
     free_adjoint(c);
     c->n_active = n_active;
@@ -789,18 +887,18 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
     HIP_TRY(c, dev_alloc(&c->ke_total, 1));
     HIP_TRY(c, dev_alloc(&c->src, c->n_src_pad));
     HIP_TRY(c, hipMemsetAsync(c->src, 0, sizeof(Source) * c->n_src_pad, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->zeta0, relvort, sizeof(double) * n_active,
-                              hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->area, area, sizeof(double) * n_active, hipMemcpyHostToDevice,
-                              c->stream));
+    rc = host_up(c, c->zeta0, relvort, sizeof(double) * n_active);
+    if (!rc) rc = host_up(c, c->area, area, sizeof(double) * n_active);
+    if (rc) return rc;
     c->n_particles = -1;   // forces the target arrays to be built
-    int rc = ensure_particles(c, 0);
+    rc = ensure_particles(c, 0);
     if (rc) return rc;
     rc = upload_aos(c, x_active, n_active, c->x0, c->t_stride, 0);
     if (rc) return rc;
     rc = upload_aos(c, x_passive, n_passive, c->x0, c->t_stride, n_active);
     if (rc) return rc;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    rc = host_sync(c);
+    if (rc) return rc;
     c->panels_loaded = true;
     return BVE_OK;
 }
@@ -818,10 +916,11 @@ int bve_load_particles(bve_ctx *c, int n_particles, const double *x, const doubl
     rc = upload_aos(c, x, n_particles, c->x0, c->t_stride, off);
     if (rc) return rc;
     c->particles_have_absvort = absvort != nullptr;
-    if (absvort && n_particles > 0)
-        HIP_TRY(c, hipMemcpyAsync(c->absvort, absvort, sizeof(double) * n_particles,
-                                  hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (absvort && n_particles > 0) {
+        rc = host_up(c, c->absvort, absvort, sizeof(double) * n_particles);
+        if (rc) return rc;
+    }
+    // no wait: the host arrays are already copied into the pinned arena
     return BVE_OK;
 }
 
@@ -834,21 +933,59 @@ int bve_advance(bve_ctx *c, double dt, int n_steps)
     return run_steps(c, dt, n_steps);
 }
 
-int bve_store_particles(bve_ctx *c, double *x, double *relvort)
+}  // extern "C"
+
+namespace {
+
+// Downloads of the particles (positions, relative vorticity) and of the
+// stage-1 kinetic energy, completed by the caller's host_sync.
+int enqueue_store_particles(bve_ctx *c, double *x, double *relvort)
 {
-    if (!c) return BVE_ERR_ARGUMENT;
-    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_store_particles: no panels loaded");
-    HIP_TRY(c, hipSetDevice(c->device));
     const int off = c->n_active + c->n_passive;
     int rc = download_aos(c, c->x0, c->t_stride, off, c->n_particles, x);
     if (rc) return rc;
     if (relvort && c->n_particles > 0) {
         if (!c->particles_have_absvort)
             return c->fail(BVE_ERR_STATE, "bve_store_particles: particles were loaded without absvort");
-        HIP_TRY(c, hipMemcpyAsync(relvort, c->relvort, sizeof(double) * c->n_particles,
-                                  hipMemcpyDeviceToHost, c->stream));
+        rc = host_down(c, relvort, c->relvort, sizeof(double) * c->n_particles);
     }
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+int enqueue_kinetic_energy(bve_ctx *c, double *total_ke, double *ke)
+{
+    int rc = BVE_OK;
+    if (total_ke) rc = host_down(c, total_ke, c->ke_total, sizeof(double));
+    if (!rc && ke) rc = host_down(c, ke, c->ke, sizeof(double) * c->n_active);
+    return rc;
+}
+
+// End of a host-driven step: particles, their relative vorticity and the
+// kinetic energy back to the caller with one wait.
+int finish_step(bve_ctx *c, double *x_particles, double *relvort, double *total_ke, double *ke)
+{
+    c->downloads.clear();
+    int rc = enqueue_store_particles(c, x_particles, relvort);
+    if (!rc) rc = enqueue_kinetic_energy(c, total_ke, ke);
+    if (!rc) rc = host_sync(c);
+    if (rc) return rc;
+    collect_profile(c);
+    return BVE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bve_store_particles(bve_ctx *c, double *x, double *relvort)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_store_particles: no panels loaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->downloads.clear();
+    int rc = enqueue_store_particles(c, x, relvort);
+    if (!rc) rc = host_sync(c);
+    if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
 }
@@ -858,13 +995,10 @@ int bve_get_kinetic_energy(bve_ctx *c, double *total_ke, double *ke)
     if (!c) return BVE_ERR_ARGUMENT;
     if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_get_kinetic_energy: no panels loaded");
     HIP_TRY(c, hipSetDevice(c->device));
-    if (total_ke)
-        HIP_TRY(c, hipMemcpyAsync(total_ke, c->ke_total, sizeof(double), hipMemcpyDeviceToHost,
-                                  c->stream));
-    if (ke)
-        HIP_TRY(c, hipMemcpyAsync(ke, c->ke, sizeof(double) * c->n_active, hipMemcpyDeviceToHost,
-                                  c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->downloads.clear();
+    int rc = enqueue_kinetic_energy(c, total_ke, ke);
+    if (!rc) rc = host_sync(c);
+    if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
 }
@@ -873,7 +1007,8 @@ int bve_synchronize(bve_ctx *c)
 {
     if (!c) return BVE_ERR_ARGUMENT;
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int rc = host_sync(c);
+    if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
 }
@@ -891,9 +1026,7 @@ int bve_rk4_step(bve_ctx *c, double dt, int n_particles, double *x_particles,
     if (rc) return rc;
     rc = run_steps(c, dt, 1);
     if (rc) return rc;
-    rc = bve_store_particles(c, x_particles, relvort);
-    if (rc) return rc;
-    return bve_get_kinetic_energy(c, total_ke, ke);
+    return finish_step(c, x_particles, relvort, total_ke, ke);
 }
 
 int bve_load_panel_absvort(bve_ctx *c, const double *absvort_active)
@@ -904,10 +1037,7 @@ int bve_load_panel_absvort(bve_ctx *c, const double *absvort_active)
     if (!absvort_active) return c->fail(BVE_ERR_ARGUMENT, "bve_load_panel_absvort: null array");
     HIP_TRY(c, hipSetDevice(c->device));
     if (!c->panel_absvort) HIP_TRY(c, dev_alloc(&c->panel_absvort, (size_t)c->n_active));
-    HIP_TRY(c, hipMemcpyAsync(c->panel_absvort, absvort_active, sizeof(double) * c->n_active,
-                              hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return BVE_OK;
+    return host_up(c, c->panel_absvort, absvort_active, sizeof(double) * c->n_active);
 }
 
 int bve_stratosphere_rk4_step(bve_ctx *c, double t, double dt, int n_particles,
@@ -930,9 +1060,7 @@ int bve_stratosphere_rk4_step(bve_ctx *c, double t, double dt, int n_particles,
     if (rc) return rc;
     rc = run_steps_on(c, dt, 1, c->n_targets, c->plan, nullptr, 0, true, t);
     if (rc) return rc;
-    rc = bve_store_particles(c, x_particles, relvort);
-    if (rc) return rc;
-    return bve_get_kinetic_energy(c, total_ke, ke);
+    return finish_step(c, x_particles, relvort, total_ke, ke);
 }
 
 int bve_store_panels(bve_ctx *c, double *x_active, double *relvort, double *x_passive)
@@ -940,22 +1068,21 @@ int bve_store_panels(bve_ctx *c, double *x_active, double *relvort, double *x_pa
     if (!c) return BVE_ERR_ARGUMENT;
     if (!c->panels_loaded) return c->fail(BVE_ERR_STATE, "bve_store_panels: no panels loaded");
     HIP_TRY(c, hipSetDevice(c->device));
+    c->downloads.clear();
     int rc = download_aos(c, c->x0, c->t_stride, 0, c->n_active, x_active);
-    if (rc) return rc;
-    rc = download_aos(c, c->x0, c->t_stride, c->n_active, c->n_passive, x_passive);
-    if (rc) return rc;
-    if (relvort)
-        HIP_TRY(c, hipMemcpyAsync(relvort, c->zeta0, sizeof(double) * c->n_active,
-                                  hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return BVE_OK;
+    if (!rc) rc = download_aos(c, c->x0, c->t_stride, c->n_active, c->n_passive, x_passive);
+    if (!rc && relvort) rc = host_down(c, relvort, c->zeta0, sizeof(double) * c->n_active);
+    if (!rc) rc = host_sync(c);
+    return rc;
 }
 
 int bve_reset(bve_ctx *c)
 {
     if (!c) return BVE_ERR_ARGUMENT;
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->downloads.clear();
+    const int rc = host_sync(c);
+    if (rc) return rc;
     collect_profile(c);
     free_state(c);
     free_adjoint(c);
@@ -1121,14 +1248,13 @@ int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every
                                    nullptr, 0);
         if (rc) return rc;
     }
+    c->downloads.clear();
     rc = download_aos(c, c->x0, c->t_stride, 0, c->n_active, x_T);
+    if (!rc && zeta_T) rc = host_down(c, zeta_T, c->zeta0, sizeof(double) * c->n_active);
     if (rc) return rc;
-    if (zeta_T)
-        HIP_TRY(c, hipMemcpyAsync(zeta_T, c->zeta0, sizeof(double) * c->n_active,
-                                  hipMemcpyDeviceToHost, c->stream));
     rc = copy_active_state(c, c->adj_ckpt, false);   // the resident state is left at t = 0
+    if (!rc) rc = host_sync(c);
     if (rc) return rc;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     collect_profile(c);
     c->adj_dt = dt;
     c->adj_steps = n_steps;
@@ -1171,14 +1297,13 @@ int bve_adjoint_backward(bve_ctx *c, const double *lambda_x, const double *lambd
             if (rc) return rc;
         }
     }
+    c->downloads.clear();
     rc = download_aos(c, c->adj_lam, c->adj_pad, 0, n, grad_x0);
+    if (!rc && grad_zeta0) rc = host_down(c, grad_zeta0, c->adj_lam + 3 * P, sizeof(double) * n);
     if (rc) return rc;
-    if (grad_zeta0)
-        HIP_TRY(c, hipMemcpyAsync(grad_zeta0, c->adj_lam + 3 * P, sizeof(double) * n,
-                                  hipMemcpyDeviceToHost, c->stream));
     rc = copy_active_state(c, c->adj_ckpt, false);
+    if (!rc) rc = host_sync(c);
     if (rc) return rc;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     collect_profile(c);
     return BVE_OK;
 }
@@ -1576,6 +1701,9 @@ int bve_ensemble_load_member(bve_ensemble *e, int member, int n_active, const do
     ens_free_tables(e);
     int rc = bve_load_panels(c, n_active, x_active, relvort, area, n_passive, x_passive);
     if (!rc) rc = bve_load_particles(c, n_particles, x_particles, absvort);
+    // the batched launches run on the ensemble's stream: finish the
+    // member's uploads (issued on its own stream) first
+    if (!rc) rc = host_sync(c);
     if (rc) return e->fail(rc, "member %d: %s", member, c->error.c_str());
     return BVE_OK;
 }

@@ -135,6 +135,15 @@ int bve_stratosphere_rk4_step(bve_ctx *ctx, double t, double dt, int n_particles
                               double *x_particles, const double *absvort, double *relvort,
                               double *total_ke, double *ke);
 
+/* AdvectionRK4 (divergent = 0) and DivergentAdvectionRK4 (divergent = 1)
+ * (aParticles, aPanels, t, dt, procRank, numProcs),
+ * PanelVelocityParallel.f90:922-1279: one RK4 step of every cached panel
+ * and of the n_particles particles (read from x_particles and overwritten)
+ * in the analytic NonDivergentWind / DivergentWind (:1415-1459), stages at
+ * t, t + dt/2, t + dt/2, t + dt.  Vorticity and KE are not touched. */
+int bve_advection_rk4_step(bve_ctx *ctx, double t, double dt, int divergent, int n_particles,
+                           double *x_particles);
+
 /* ScatterPanels payload (ParticlesEdgesPanels2.f90:1622-1637): current
  * active positions and relative vorticity and passive positions.  Any
  * pointer may be NULL. */

@@ -39,6 +39,8 @@ def lib():
         L.bvo_voronoi_velocity.argtypes = [_i, _p, _p, _p, _p, _p, _i, _i, _d, _p, _p, _p, _p]
         L.bvo_rk4_step.argtypes = [_i, _p, _p, _p, _i, _p, _i, _p, _p, _p, _d, _d, _d, _p, _p]
         L.bvo_renormalize.argtypes = [_i, _p]
+        L.bvo_wind.argtypes = [_i, _p, _d, _p]
+        L.bvo_advection_rk4_step.argtypes = [_i, _i, _p, _d, _d]
         L.bvo_juckes_forcing.argtypes = [_p, _d]
         L.bvo_juckes_forcing.restype = _d
         L.bvo_stratosphere_rk4_step.argtypes = [_i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _d,
@@ -121,6 +123,22 @@ def rk4_step(active_x, active_vort, area, passive_x, particles_x, particles_absv
                        _ptr(qabs), dt, omega, smooth, _ptr(ke), ctypes.byref(tot))
     return dict(active_x=ax, active_relvort=av, passive_x=px, particles_x=qx,
                 particles_relvort=qrel, ke=ke, total_ke=tot.value)
+
+
+def wind(x, t, divergent=False):
+    """NonDivergentWind / DivergentWind at points x (N, 3) and time t."""
+    x = _c(x).reshape(-1, 3)
+    out = np.zeros_like(x)
+    for i in range(len(x)):
+        lib().bvo_wind(int(divergent), _ptr(np.ascontiguousarray(x[i])), t, _ptr(out[i]))
+    return out
+
+
+def advection_rk4_step(x, t, dt, divergent=False):
+    """One AdvectionRK4 / DivergentAdvectionRK4 step of the points x (N, 3)."""
+    x = _c(x).copy()
+    lib().bvo_advection_rk4_step(int(divergent), len(x), _ptr(x), t, dt)
+    return x
 
 
 def juckes_forcing(x, t):

@@ -338,6 +338,64 @@ void bvo_stratosphere_rk4_step(int na, double *ax, double *avort, const double *
     for (int s = 0; s < 4; ++s) { free(ka[s]); free(kp[s]); free(kq[s]); }
 }
 
+/* NonDivergentWind (divergent = 0) and DivergentWind (divergent = 1),
+ * PanelVelocityParallel.f90:1415-1459: the analytic deformational winds
+ * (R = 1, T = 5) at xyz and time t, zero within 1e-14 of the axis. */
+void bvo_wind(int divergent, const double *xyz, double t, double *out)
+{
+    const double RR = 1.0, TT = 5.0, zero_tol = 1e-14;
+    const double x = xyz[0], y = xyz[1], z = xyz[2];
+    const double lat = atan2(z, sqrt(x * x + y * y));
+    const double lon = bvo_atan4(y, x);
+    const double raxis = sqrt(x * x + y * y);
+    if (!(raxis > zero_tol)) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    double u, v;
+    if (!divergent) {
+        u = 10.0 * RR / TT * sin(lon - 2.0 * BVO_PI * t / TT) * sin(lon - 2.0 * BVO_PI * t / TT) *
+                sin(2.0 * lat) * cos(BVO_PI * t / TT) +
+            2.0 * BVO_PI * RR / TT * cos(lat);
+        v = 10.0 * RR / TT * sin(2.0 * (lon - 2.0 * BVO_PI * t / TT)) * cos(lat) * cos(BVO_PI * t / TT);
+    } else {
+        u = -5.0 * RR / TT * sin((lon - 2.0 * BVO_PI * t / TT) / 2.0) *
+                sin((lon - 2.0 * BVO_PI * t / TT) / 2.0) * sin(2.0 * lat) * cos(lat) * cos(lat) *
+                cos(BVO_PI * t / TT) +
+            2.0 * BVO_PI * RR * cos(lat) / TT;
+        v = 5.0 * RR / (2.0 * TT) * sin(lon - 2.0 * BVO_PI * t / TT) * cos(lat) * cos(lat) * cos(lat) *
+            cos(BVO_PI * t / TT);
+    }
+    out[0] = -u * y / raxis - v * x * z / raxis;
+    out[1] = u * x / raxis - v * y * z / raxis;
+    out[2] = v * raxis;
+}
+
+/* AdvectionRK4 / DivergentAdvectionRK4, PanelVelocityParallel.f90:922-1099
+ * and :1102-1279: classical RK4 of the n points x (AoS, in place) in the
+ * analytic wind, stages at t, t + dt/2, t + dt/2, t + dt; every point class
+ * is moved the same way and vorticity is untouched. */
+void bvo_advection_rk4_step(int divergent, int n, double *x, double t, double dt)
+{
+    for (int j = 0; j < n; ++j) {
+        double *p = x + 3 * (size_t)j;
+        double k1[3], k2[3], k3[3], k4[3], in[3];
+        bvo_wind(divergent, p, t, k1);
+        for (int c = 0; c < 3; ++c) k1[c] = dt * k1[c];
+        for (int c = 0; c < 3; ++c) in[c] = p[c] + 0.5 * k1[c];
+        bvo_wind(divergent, in, t + 0.5 * dt, k2);
+        for (int c = 0; c < 3; ++c) k2[c] = dt * k2[c];
+        for (int c = 0; c < 3; ++c) in[c] = p[c] + 0.5 * k2[c];
+        bvo_wind(divergent, in, t + 0.5 * dt, k3);
+        for (int c = 0; c < 3; ++c) k3[c] = dt * k3[c];
+        for (int c = 0; c < 3; ++c) in[c] = p[c] + k3[c];
+        bvo_wind(divergent, in, t + dt, k4);
+        for (int c = 0; c < 3; ++c) k4[c] = dt * k4[c];
+        for (int c = 0; c < 3; ++c)
+            p[c] = p[c] + k1[c] / 6.0 + k2[c] / 3.0 + k3[c] / 3.0 + k4[c] / 6.0;
+    }
+}
+
 /* RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702: x / sqrt(x.x). */
 void bvo_renormalize(int n, double *x)
 {

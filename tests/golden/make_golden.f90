@@ -10,6 +10,8 @@
 !   IC = 2  the reference's Gaussian vortex (InitGaussianVortex)
 !   IC = 3  the reference's stratosphere model (InitStratosphere), stepped
 !           with StratosphereRK4 (Juckes forcing) from time T0
+!   IC = 4  the RH4 wave, moved by AdvectionRK4 (NonDivergentWind) from T0
+!   IC = 5  the RH4 wave, moved by DivergentAdvectionRK4 (DivergentWind) from T0
 !
 ! Output layout (stream access, native little-endian):
 !   int32  nActive, nParticles, nPassive, nSteps
@@ -17,7 +19,7 @@
 !   state(0), state(1), state(nSteps)       (see write_state)
 !   float64 kineticEnergy(nActive)          stage-1 |u|^2 of step 1
 !   float64 totalKE(nSteps)                 per step
-!   IC = 3 only: float64 T0, active absVort(nActive)
+!   IC >= 3 only: float64 T0, active absVort(nActive)
 ! where a state is, in the order GatherPanels uses (panel index order):
 !   active  x(3,nActive) relVort(nActive) area(nActive)
 !   passive x(3,nPassive)
@@ -78,6 +80,8 @@ program make_golden
 		call InitGaussianVortex(gp, gpan)
 	case (3)
 		call InitStratosphere(gp, gpan)
+	case (4, 5)
+		call InitRH4Wave(gp, gpan)
 	case default
 		print *, 'make_golden: unknown IC ', ic
 		stop 1
@@ -97,6 +101,10 @@ program make_golden
 	do step = 1, nSteps
 		if (ic == 3) then
 			call StratosphereRK4(gp, gpan, t0 + real(step - 1, kreal)*dt, dt, rank, nprocs)
+		elseif (ic == 4) then
+			call AdvectionRK4(gp, gpan, t0 + real(step - 1, kreal)*dt, dt, rank, nprocs)
+		elseif (ic == 5) then
+			call DivergentAdvectionRK4(gp, gpan, t0 + real(step - 1, kreal)*dt, dt, rank, nprocs)
 		else
 			call BVERK4(gp, gpan, dt, rank, nprocs)
 		endif
@@ -109,7 +117,7 @@ program make_golden
 	call write_state()
 	write(u) keStep1
 	write(u) keTotal
-	if (ic == 3) then
+	if (ic >= 3) then
 		write(u) t0
 		do j = 1, gpan%N
 			if (.not. gpan%hasChildren(j)) write(u) gpan%absVort(j)

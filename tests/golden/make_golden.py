@@ -7,8 +7,9 @@ compressed ``.npz`` (float64 / int32 arrays only, loadable with
 ``allow_pickle=False``) plus ``manifest.json``.
 
 It also runs the reference's own drivers (RHWavePanelsMPI.f90,
-StratospherePanels.f90; unmodified, built by the same makefile's `drivers`
-target) on the committed namelists and keeps their .dat tables.
+StratospherePanels.f90, JetPanels.f90, NonDivAdvection.f90; unmodified,
+built by the same makefile's `drivers` target) on the committed namelists
+and keeps their .dat tables.
 
 Run from the repository root, in a container that has the reference:
 
@@ -52,6 +53,20 @@ FORCED_CASES = {
 }
 
 
+# advection (AdvectionRK4 ic 4, DivergentAdvectionRK4 ic 5) cases:
+# name: (level, ic, nsteps, dt, t0, description)
+ADVECTION_CASES = {
+    "adv_nondiv_L4_4steps_t1": (4, 4, 4, 0.05, 1.0,
+                                "cubed sphere L4, AdvectionRK4 (NonDivergentWind), 4 steps dt=0.05 "
+                                "from t=1"),
+    "adv_div_L4_4steps_t1": (4, 5, 4, 0.05, 1.0,
+                             "cubed sphere L4, DivergentAdvectionRK4 (DivergentWind), 4 steps "
+                             "dt=0.05 from t=1"),
+    "adv_nondiv_L3_3steps_t24": (3, 4, 3, 0.05, 2.4,
+                                 "cubed sphere L3, AdvectionRK4 across t=2.5 (wind reduces to a "
+                                 "solid-body rotation there)"),
+}
+
 # reference drivers run unmodified on a committed namelist: name: (binary in
 # oracle/_ref, the namelist file name the driver opens); the namelist is
 # tests/golden/<name>.namelist and the driver's .dat becomes <name>.dat
@@ -61,6 +76,7 @@ DRIVER_CASES = {
     "strat_driver_L4": ("strat_ref", "StratospherePanels.namelist"),
     "jet_driver_L4_amr2": ("jet_ref", "JetPanels.namelist"),
     "jet_driver_L4_amr2_remesh5": ("jet_ref", "JetPanels.namelist"),
+    "adv_driver_L3": ("adv_ref", "AdvectGaussHills.namelist"),
 }
 
 
@@ -144,7 +160,7 @@ def run_case(tmp, name, level, ic, nsteps, dt, smooth, t0=None):
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -f oracle/Makefile.ref")
-    manifest, forced = {}, {}
+    manifest, forced, advection = {}, {}, {}
     with tempfile.TemporaryDirectory() as tmp:
         for name, (level, ic, nsteps, dt, smooth, desc) in CASES.items():
             d = run_case(tmp, name, level, ic, nsteps, dt, smooth)
@@ -158,6 +174,11 @@ def main():
                                 smooth=d["smooth"], omega=d["omega"], t0=t0,
                                 n_active=d["na"], n_particles=d["npart"],
                                 n_passive=d["npp"], description=desc)
+        for name, (level, ic, nsteps, dt, t0, desc) in ADVECTION_CASES.items():
+            d = run_case(tmp, name, level, ic, nsteps, dt, 0.05, t0)
+            advection[name] = dict(level=level, ic=ic, divergent=ic == 5, nsteps=d["nsteps"],
+                                   dt=d["dt"], t0=t0, n_active=d["na"], n_particles=d["npart"],
+                                   n_passive=d["npp"], description=desc)
         for name, (binary, namelist) in DRIVER_CASES.items():
             run_driver(tmp, name, binary, namelist)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
@@ -171,7 +192,11 @@ def main():
                           "(PanelVelocityParallel.f90:661-919) from InitStratosphere "
                           "(TracerAndVorticityDistributions.f90:793-825); extra arrays t0 and "
                           "active_absvort",
-            forced_cases=forced), f, indent=1)
+            forced_cases=forced,
+            advection_source="reference AdvectionRK4 / DivergentAdvectionRK4 "
+                             "(PanelVelocityParallel.f90:922-1279, winds :1415-1459); only the "
+                             "positions change",
+            advection_cases=advection), f, indent=1)
 
 
 if __name__ == "__main__":

@@ -1,6 +1,7 @@
 """GPU: the Fortran host over the ISO_C_BINDING interface, and the
 reference's own drivers (RHWavePanelsMPI.f90, StratospherePanels.f90,
-JetPanels.f90 with adaptive refinement; unmodified) linked against the
+JetPanels.f90 with adaptive refinement, NonDivAdvection.f90; unmodified)
+linked against the
 Fortran drop-in PanelVelocityParallelModule, against the oracle and the
 reference's own output."""
 import os
@@ -141,3 +142,35 @@ def test_reference_amr_jet_driver_remesh_on_the_dropin(tmp_path):
     assert np.allclose(got[:5], ref[:5], rtol=1e-12, atol=2e-15)
     print("post-remesh max relative difference", np.abs(got[5:] / ref[5:] - 1).max(axis=0))
     assert np.allclose(got[5:], ref[5:], rtol=1e-9, atol=0)
+
+
+ADV_DROPIN = os.path.join(ROOT, "oracle", "_ref", "adv_gpu")
+
+
+def read_advection_dat(path):
+    """NonDivAdvection's .dat: three error lines, a blank line, then the
+    time / totalMass table."""
+    lines = open(path).read().splitlines()
+    errors = np.array([float(l.split(":")[-1]) for l in lines[:3]])
+    table = np.array([[float(v) for v in l.split()] for l in lines[5:] if l.strip()])
+    return errors, table
+
+
+@pytest.mark.skipif(not os.path.exists(ADV_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_advection_driver_runs_on_the_dropin(tmp_path):
+    """NonDivAdvection (Gaussian hills, level 3, AdvectionRK4 over one full
+    period, 100 steps, no remesh) on the GPU module: the tracer errors at
+    t = 5 (~2e-5) agree with the reference's to 1e-8 relative (positions
+    differ at the 1e-15 level), the mass history exactly."""
+    shutil.copy(os.path.join(GOLDEN, "adv_driver_L3.namelist"),
+                tmp_path / "AdvectGaussHills.namelist")
+    (tmp_path / "vtkOut").mkdir()
+    r = subprocess.run([ADV_DROPIN], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout + r.stderr)[-2000:]
+    dat = list(tmp_path.glob("*.dat"))
+    assert len(dat) == 1
+    got_err, got_tab = read_advection_dat(dat[0])
+    ref_err, ref_tab = read_advection_dat(os.path.join(GOLDEN, "adv_driver_L3.dat"))
+    assert np.allclose(got_err, ref_err, rtol=1e-8, atol=0), (got_err, ref_err)
+    assert np.array_equal(got_tab, ref_tab)

@@ -283,3 +283,66 @@ def test_unforced_stratosphere_step_is_the_bve_step():
     assert np.abs(f["active_relvort"] - r["active_relvort"]).max() < 1e-13 * np.abs(z).max()
     assert np.abs(f["particles_relvort"] - r["particles_relvort"]).max() < 1e-13 * np.abs(z).max()
     assert abs(f["total_ke"] - r["total_ke"]) <= 1e-14 * r["total_ke"]
+
+
+# ---- AdvectionRK4 / DivergentAdvectionRK4 (analytic winds) ----------------
+
+def advection_manifest():
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        return json.load(f)["advection_cases"]
+
+
+@pytest.mark.parametrize("name", sorted(advection_manifest()))
+def test_advection_oracle_matches_reference_bitwise(name):
+    """bvo_advection_rk4_step restates AdvectionRK4 / DivergentAdvectionRK4
+    and their winds in the reference's rounding order: bit for bit for
+    every point class and every step."""
+    c = advection_manifest()[name]
+    g = load_golden(name)
+    t0 = float(g["t0"][0])
+    pts = {k: g["s0_" + k] for k in ("active_x", "passive_x", "particles_x")}
+    for n in range(c["nsteps"]):
+        pts = {k: oracle.advection_rk4_step(v, t0 + n * c["dt"], c["dt"], c["divergent"])
+               for k, v in pts.items()}
+        if n == 0:
+            for k in pts:
+                assert np.array_equal(pts[k], g["s1_" + k]), (name, k)
+    for k in pts:
+        assert np.array_equal(pts[k], g["sn_" + k]), (name, k)
+
+
+@pytest.mark.parametrize("divergent", [False, True])
+def test_winds_closed_form(divergent):
+    """At t = T/2 = 2.5 both winds reduce to the solid-body rotation
+    (2 PI / 5) e_z x x; at any time they are tangent to the sphere, and the
+    non-divergent one is divergence free on the sphere (checked through the
+    area change of a small triangle over a short time)."""
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(200, 3))
+    x /= np.linalg.norm(x, axis=1)[:, None]
+    w = oracle.wind(x, 2.5, divergent)
+    solid = 2 * np.pi / 5 * np.stack([-x[:, 1], x[:, 0], np.zeros(len(x))], axis=1)
+    assert np.abs(w - solid).max() < 1e-14
+    w = oracle.wind(x, 1.3, divergent)
+    assert np.abs(np.einsum("ij,ij->i", w, x)).max() < 1e-14
+    # a small spherical triangle carried by the flow: its area is conserved
+    # by the non-divergent wind (to the time step's order) and not by the
+    # divergent one
+    c = np.array([0.3, 0.5, 0.8]) / np.linalg.norm([0.3, 0.5, 0.8])
+    e1 = np.cross(c, [0, 0, 1.0]); e1 /= np.linalg.norm(e1)
+    e2 = np.cross(c, e1)
+    tri = np.array([c, c + 1e-3 * e1, c + 1e-3 * e2])
+    tri /= np.linalg.norm(tri, axis=1)[:, None]
+
+    def area(p):
+        return 0.5 * np.linalg.norm(np.cross(p[1] - p[0], p[2] - p[0]))
+
+    a0 = area(tri)
+    moved = tri.copy()
+    for n in range(20):
+        moved = oracle.advection_rk4_step(moved, 0.7 + n * 0.01, 0.01, divergent)
+    change = abs(area(moved) / a0 - 1)
+    assert (change > 1e-3) if divergent else (change < 1e-4)

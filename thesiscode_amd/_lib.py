@@ -34,6 +34,7 @@ SIGNATURES = {
     "bve_load_panels": (_i, [_p, _i, _p, _p, _p, _i, _p]),
     "bve_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
     "bve_load_panel_absvort": (_i, [_p, _p]),
+    "bve_advection_rk4_step": (_i, [_p, _d, _d, _i, _i, _p]),
     "bve_stratosphere_rk4_step": (_i, [_p, _d, _d, _i, _p, _p, _p, _p, _p]),
     "bve_store_panels": (_i, [_p, _p, _p, _p]),
     "bve_reset": (_i, [_p]),
@@ -204,6 +205,14 @@ class Context:
                 raise ValueError("store_panels_into needs C-contiguous float64 arrays")
         self._check(self._L.bve_store_panels(self._h, _ptr(active_x), _ptr(active_relvort),
                                              _ptr(passive_x)), "bve_store_panels")
+
+    def advection_rk4_step(self, t, dt, particles_x, divergent=False):
+        """One AdvectionRK4 / DivergentAdvectionRK4 step; returns the particles."""
+        n = len(particles_x)
+        qx = _f64(particles_x, 3 * n).copy()
+        self._check(self._L.bve_advection_rk4_step(self._h, t, dt, int(divergent), n, _ptr(qx)),
+                    "bve_advection_rk4_step")
+        return qx.reshape(n, 3)
 
     def load_panel_absvort(self, absvort_active):
         a = _f64(absvort_active, self.n_active)

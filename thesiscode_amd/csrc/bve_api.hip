@@ -1040,6 +1040,24 @@ int bve_load_panel_absvort(bve_ctx *c, const double *absvort_active)
     return host_up(c, c->panel_absvort, absvort_active, sizeof(double) * c->n_active);
 }
 
+int bve_advection_rk4_step(bve_ctx *c, double t, double dt, int divergent, int n_particles,
+                           double *x_particles)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded)
+        return c->fail(BVE_ERR_STATE, "bve_advection_rk4_step: no panels loaded");
+    if (n_particles < 0 || (n_particles > 0 && !x_particles) || (divergent != 0 && divergent != 1))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_advection_rk4_step: bad arguments (n=%d, "
+                       "divergent=%d)", n_particles, divergent);
+    int rc = bve_load_particles(c, n_particles, x_particles, nullptr);
+    if (rc) return rc;
+    HIP_TRY(c, launch_advection_step(step_state(c), t, dt, divergent, c->stream));
+    c->downloads.clear();
+    rc = enqueue_store_particles(c, x_particles, nullptr);
+    if (!rc) rc = host_sync(c);
+    return rc;
+}
+
 int bve_stratosphere_rk4_step(bve_ctx *c, double t, double dt, int n_particles,
                               double *x_particles, const double *absvort, double *relvort,
                               double *total_ke, double *ke)

@@ -579,6 +579,69 @@ __device__ __forceinline__ double forced_relvort(double absvort, double omega, d
     return absvort - 2.0 * omega * z - juckes_forcing(x, y, z, t);
 }
 
+// ---------------------------------------------------------------------------
+// Analytic winds of the advection tests (PanelVelocityParallel.f90:1415-1459)
+// ---------------------------------------------------------------------------
+
+// NonDivergentWind (divergent = 0) or DivergentWind (divergent = 1) at
+// (x, y, z), time t: R = 1, T = 5, zero within 1e-14 of the polar axis.
+__device__ __forceinline__ void analytic_wind(int divergent, double x, double y, double z,
+                                              double t, double &wx, double &wy, double &wz)
+{
+    const double RR = 1.0, TT = 5.0;
+    const double raxis = sqrt(x * x + y * y);
+    if (!(raxis > 1e-14)) {
+        wx = wy = wz = 0.0;
+        return;
+    }
+    const double lat = atan2(z, sqrt(x * x + y * y));
+    const double lon = longitude_0_2pi(y, x);
+    const double phase = lon - 2.0 * kPi * t / TT;
+    double u, v;
+    if (!divergent) {
+        u = 10.0 * RR / TT * sin(phase) * sin(phase) * sin(2.0 * lat) * cos(kPi * t / TT) +
+            2.0 * kPi * RR / TT * cos(lat);
+        v = 10.0 * RR / TT * sin(2.0 * phase) * cos(lat) * cos(kPi * t / TT);
+    } else {
+        u = -5.0 * RR / TT * sin(phase / 2.0) * sin(phase / 2.0) * sin(2.0 * lat) * cos(lat) *
+                cos(lat) * cos(kPi * t / TT) +
+            2.0 * kPi * RR * cos(lat) / TT;
+        v = 5.0 * RR / (2.0 * TT) * sin(phase) * cos(lat) * cos(lat) * cos(lat) * cos(kPi * t / TT);
+    }
+    wx = -u * y / raxis - v * x * z / raxis;
+    wy = u * x / raxis - v * y * z / raxis;
+    wz = v * raxis;
+}
+
+// One AdvectionRK4 / DivergentAdvectionRK4 step of target i
+// (PanelVelocityParallel.f90:922-1279): the pointwise wind needs no other
+// target, so the four stages run in registers; an active panel's source
+// record follows its position (its vorticity is unchanged).
+__device__ __forceinline__ void advection_step_body(const StepState &st, int i, double t,
+                                                    double dt, int divergent)
+{
+    const int stride = st.t_stride;
+    const double p[3] = {st.x0[i], st.x0[stride + i], st.x0[2 * stride + i]};
+    double k[4][3], in[3];
+    analytic_wind(divergent, p[0], p[1], p[2], t, k[0][0], k[0][1], k[0][2]);
+    static constexpr double a[3] = {0.5, 0.5, 1.0};
+    for (int s = 0; s < 3; ++s) {
+        for (int c = 0; c < 3; ++c) k[s][c] = dt * k[s][c];
+        for (int c = 0; c < 3; ++c) in[c] = a[s] == 1.0 ? p[c] + k[s][c] : p[c] + a[s] * k[s][c];
+        const double ts = s < 2 ? t + 0.5 * dt : t + dt;
+        analytic_wind(divergent, in[0], in[1], in[2], ts, k[s + 1][0], k[s + 1][1], k[s + 1][2]);
+    }
+    for (int c = 0; c < 3; ++c) k[3][c] = dt * k[3][c];
+    double q[3];
+    for (int c = 0; c < 3; ++c)
+        q[c] = p[c] + k[0][c] / 6.0 + k[1][c] / 3.0 + k[2][c] / 3.0 + k[3][c] / 6.0;
+    for (int c = 0; c < 3; ++c) {
+        st.x0[c * stride + i] = q[c];
+        st.xs[c * stride + i] = q[c];
+    }
+    if (i < st.n_active) st.src[i] = Source{q[0], q[1], q[2], st.zeta0[i] * st.area[i]};
+}
+
 // Sum of the partials of target i of a pass with plan p, in a fixed order
 // (chunk 0 first, or the first segment touching i's tile first).
 __device__ __forceinline__ void reduce_partials(const double *__restrict__ partial,

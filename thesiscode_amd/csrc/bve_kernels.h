@@ -106,6 +106,9 @@ hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s);
 hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s);
 hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s);
 hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s);
+// One AdvectionRK4 (divergent = 0) / DivergentAdvectionRK4 (1) step of every target.
+hipError_t launch_advection_step(const StepState &st, double t, double dt, int divergent,
+                                 hipStream_t s);
 hipError_t launch_prepare_step(const StepState &st, hipStream_t s);
 int ke_partial_count(int n_targets);
 

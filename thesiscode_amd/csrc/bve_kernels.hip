@@ -116,6 +116,22 @@ __global__ __launch_bounds__(256) void stage_update_kernel(StepState st, StagePa
     stage_update_body(st, sp, blockIdx.x, sh);
 }
 
+__global__ __launch_bounds__(256) void advection_step_kernel(StepState st, double t, double dt,
+                                                             int divergent)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < st.n_targets) advection_step_body(st, i, t, dt, divergent);
+}
+
+hipError_t launch_advection_step(const StepState &st, double t, double dt, int divergent,
+                                 hipStream_t s)
+{
+    if (st.n_targets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(advection_step_kernel, dim3(ceil_div(st.n_targets, 256)), dim3(256), 0, s,
+                       st, t, dt, divergent);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void ke_total_kernel(const double *__restrict__ part, int n,
                                                        double *__restrict__ out)
 {

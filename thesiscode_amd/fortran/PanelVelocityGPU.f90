@@ -14,7 +14,9 @@
 !     relVort = absVort - 2 Omega z is set after it (:651);
 !   * kineticEnergy / totalKE are those of stage 1 (:495-501), copied to
 !     panel tracer 2 when there are at least two tracers.
-! StratosphereRK4 (:661-919, Juckes forcing) runs on the same cached panels.
+! StratosphereRK4 (:661-919, Juckes forcing) and AdvectionRK4 /
+! DivergentAdvectionRK4 (:922-1279, analytic winds) run on the same cached
+! panels.
 ! procRank and numProcs are accepted; one GPU computes every target.
 module PanelVelocityParallelModule
 
@@ -174,13 +176,6 @@ subroutine BVERK4(aParticles, aPanels, dt, procRank, numProcs)
 	call ScatterPanels(aPanels, activePanels, activeMap, passivePanels, passiveMap)
 end subroutine
 
-subroutine NotOnGPU(name)
-	character(len=*), intent(in) :: name
-	call LogMessage(log, ERROR_LOGGING_LEVEL, logKey, name// &
-		' (analytic-wind advection) is not provided by the GPU module.')
-	error stop 1
-end subroutine
-
 ! StratosphereRK4 (PanelVelocityParallel.f90:661-919): the same cached
 ! panels and stage loop on the GPU, with every stage's active vorticity
 ! absVort - 2 Omega z - F(x, t_s) and the Juckes forcing F; afterwards the
@@ -203,12 +198,15 @@ subroutine StratosphereRK4(aParticles, aPanels, t, dt, procRank, numProcs)
 	call ScatterPanels(aPanels, activePanels, activeMap, passivePanels, passiveMap)
 end subroutine
 
+! AdvectionRK4 / DivergentAdvectionRK4 (PanelVelocityParallel.f90:922-1279):
+! every cached panel and the particles move in the analytic wind; vorticity,
+! tracers and the kinetic energy are left as they are.
 subroutine AdvectionRK4(aParticles, aPanels, t, dt, procRank, numProcs)
 	type(Particles), intent(inout) :: aParticles
 	type(Panels), intent(inout) :: aPanels
 	real(kreal), intent(in) :: t, dt
 	integer(kint), intent(in) :: procRank, numProcs
-	call NotOnGPU('AdvectionRK4')
+	call AdvectWithWind(aParticles, aPanels, t, dt, 0_c_int, procRank, numProcs)
 end subroutine
 
 subroutine DivergentAdvectionRK4(aParticles, aPanels, t, dt, procRank, numProcs)
@@ -216,7 +214,24 @@ subroutine DivergentAdvectionRK4(aParticles, aPanels, t, dt, procRank, numProcs)
 	type(Panels), intent(inout) :: aPanels
 	real(kreal), intent(in) :: t, dt
 	integer(kint), intent(in) :: procRank, numProcs
-	call NotOnGPU('DivergentAdvectionRK4')
+	call AdvectWithWind(aParticles, aPanels, t, dt, 1_c_int, procRank, numProcs)
+end subroutine
+
+subroutine AdvectWithWind(aParticles, aPanels, t, dt, divergent, procRank, numProcs)
+	type(Particles), intent(inout) :: aParticles
+	type(Panels), intent(inout) :: aPanels
+	real(kreal), intent(in) :: t, dt
+	integer(c_int), intent(in) :: divergent
+	integer(kint), intent(in) :: procRank, numProcs
+	integer(kint) :: n
+	if (.not. c_associated(ctx)) call InitializeMPIRK4(aParticles, aPanels, procRank, numProcs)
+	if (.not. rk4IsReady) call InitRK4(aParticles, aPanels)
+	n = aParticles%N
+	call check(bve_advection_rk4_step(ctx, t, dt, divergent, int(n, c_int), aParticles%x(:,1:n)), &
+		'AdvectionRK4')
+	call check(bve_store_panels(ctx, activePanels%x, activePanels%relVort, passivePanels%x), &
+		'AdvectionRK4')
+	call ScatterPanels(aPanels, activePanels, activeMap, passivePanels, passiveMap)
 end subroutine
 
 end module PanelVelocityParallelModule

@@ -9,7 +9,7 @@ module BVEInterfaceModule
 	public :: BVE_OK, BVE_PRECISION_FP64, BVE_PRECISION_MIXED, BVE_PRECISION_FP64_FAST
 	public :: bve_create, bve_destroy, bve_set_omega, bve_set_smooth, bve_set_precision
 	public :: bve_set_renormalize, bve_load_panels, bve_rk4_step, bve_store_panels, bve_reset
-	public :: bve_load_panel_absvort, bve_stratosphere_rk4_step
+	public :: bve_load_panel_absvort, bve_stratosphere_rk4_step, bve_advection_rk4_step
 	public :: bve_load_particles, bve_advance, bve_store_particles, bve_synchronize
 	public :: bve_voronoi_rk4_step, bve_error_message
 
@@ -95,6 +95,15 @@ module BVEInterfaceModule
 			real(c_double), intent(inout) :: xParticles(3,*)
 			real(c_double), intent(in) :: absVort(*)
 			real(c_double), intent(out) :: relVort(*), totalKE, ke(*)
+		end function
+
+		integer(c_int) function bve_advection_rk4_step(ctx, t, dt, divergent, nParticles, &
+				xParticles) bind(C, name="bve_advection_rk4_step")
+			import :: c_int, c_ptr, c_double
+			type(c_ptr), value :: ctx
+			real(c_double), value :: t, dt
+			integer(c_int), value :: divergent, nParticles
+			real(c_double), intent(inout) :: xParticles(3,*)
 		end function
 
 		integer(c_int) function bve_store_panels(ctx, xActive, relVort, xPassive) &

@@ -31,7 +31,8 @@ enum {
     BVE_ERR_ARGUMENT = 1, /* bad size, null pointer or out-of-range value */
     BVE_ERR_STATE = 2,    /* call out of order (e.g. step before panels are loaded) */
     BVE_ERR_DEVICE = 3,   /* a HIP call failed */
-    BVE_ERR_MEMORY = 4    /* device allocation failed */
+    BVE_ERR_MEMORY = 4,   /* device allocation failed */
+    BVE_ERR_IO = 5        /* a file could not be opened or written */
 };
 
 enum {
@@ -48,7 +49,8 @@ enum {
  * (its state is module SAVE data, PanelVelocityParallel.f90:25-89). */
 int bve_create(bve_ctx **ctx, int device);
 int bve_destroy(bve_ctx *ctx);
-/* Message of the last failed call on ctx (or of bve_create when ctx is NULL). */
+/* Message of the last failed call on ctx; with ctx NULL, of the last failed
+ * call that takes no context (bve_create, bve_vtk_write_frame) on this thread. */
 const char *bve_last_error(const bve_ctx *ctx);
 
 /* ---- parameters ------------------------------------------------------ */
@@ -243,6 +245,39 @@ int bve_ensemble_store_member(bve_ensemble *ens, int member, double *x_active,
 int bve_ensemble_set_profiling(bve_ensemble *ens, int on);
 int bve_ensemble_get_profile(bve_ensemble *ens, long long *launches, double *total_ms);
 int bve_ensemble_pairs_per_step(const bve_ensemble *ens, double *pairs);
+
+/* ---- output ----------------------------------------------------------- */
+
+/* One legacy-VTK ASCII POLYDATA frame, byte for byte the file the
+ * reference's vtkOutputUnif writes (ParticlesEdgesPanels2.f90:2116-2260,
+ * reached through its vtkOutput interface, :133-137): particle positions as
+ * POINTS (F24.18), the undivided panels' corner lists as POLYGONS (I8; for
+ * Voronoi panels every panel, list-directed), then per-particle tracers,
+ * relVort and absVort (POINT_DATA) and per-undivided-panel tracers,
+ * relVort and absVort (CELL_DATA).  Host memory only, no device work.
+ *
+ * The arrays are the reference types' own storage (column-major):
+ *   particles_x        x(3, N_Max) of Particles; the first n_particles columns
+ *   particles_relvort, particles_absvort   (N_Max) or NULL when absent
+ *                      (advection problems carry no vorticity)
+ *   particles_tracer   tracer(ld_particles, n_tracer) or NULL (no tracers);
+ *                      n_tracer is the reference's nTracers and also sets
+ *                      how many panel tracers are written
+ *   panel_kind         3 (TRI_PANEL), 4 (QUAD_PANEL) or 10 (VORONOI_PANEL)
+ *   n_panels, n_active aPanels%N, aPanels%N_Active
+ *   vertices           vertices(panel_kind, N_Max), 1-based particle indices
+ *   has_children       (N_Max) Fortran default logicals, nonzero = divided;
+ *                      NULL = no panel is divided
+ *   panels_tracer      tracer(ld_panels, n_tracer) or NULL
+ *   panels_relvort, panels_absvort   (N_Max) or NULL
+ * Returns BVE_ERR_IO when the file cannot be written (the reference stops). */
+int bve_vtk_write_frame(const char *filename, int n_particles, int ld_particles,
+                        const double *particles_x, const double *particles_relvort,
+                        const double *particles_absvort, int n_tracer,
+                        const double *particles_tracer, int panel_kind, int n_panels,
+                        int n_active, int ld_panels, const int *vertices, const int *has_children,
+                        const double *panels_tracer, const double *panels_relvort,
+                        const double *panels_absvort);
 
 /* ---- measurement ------------------------------------------------------ */
 

@@ -14,6 +14,9 @@ and keeps their .dat tables.
 Run from the repository root, in a container that has the reference:
 
     make -f oracle/Makefile.ref all drivers && python tests/golden/make_golden.py
+
+It also writes VTK frames with the reference's vtkOutput (make_vtk_golden.f90)
+together with the arrays they were written from.
 """
 import json
 import os
@@ -26,6 +29,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "make_golden")
+VTK_HARNESS = os.path.join(ROOT, "oracle", "_ref", "make_vtk_golden")
 
 # name: (level, ic, nsteps, dt, smooth, description)
 CASES = {
@@ -78,6 +82,56 @@ DRIVER_CASES = {
     "jet_driver_L4_amr2_remesh5": ("jet_ref", "JetPanels.namelist"),
     "adv_driver_L3": ("adv_ref", "AdvectGaussHills.namelist"),
 }
+
+
+# VTK frames written by the reference's vtkOutput (tests/golden/make_vtk_golden.f90):
+# name: (harness case, description); stored as <name>.vtk plus <name>.npz
+VTK_CASES = {
+    "vtk_quad_L2_amr": (1, "cubed sphere L2 with 8 panels divided once more, RH4 wave after one "
+                           "BVERK4 step, 3 tracers"),
+    "vtk_tri_L2_advection": (2, "icosahedral triangles L2, advection problem (no vorticity), "
+                                "2 tracers"),
+    "vtk_voronoi_special": (3, "Voronoi panel kind, synthetic: wrapped list-directed corner lists; "
+                               "signed zeros, ties, sub-normals, too-wide values, NaN, Inf"),
+}
+
+
+def read_vtk_dump(path):
+    """Parse the stream file written by make_vtk_golden.f90."""
+    raw = open(path, "rb").read()
+    kind, n, npan, nact, nt, has_vort = (int(v) for v in np.frombuffer(raw, "<i4", 6, 0))
+    off = 24
+
+    def take(count, dtype="<f8"):
+        nonlocal off
+        a = np.frombuffer(raw, dtype, count, off).copy()
+        off += a.itemsize * count
+        return a
+
+    d = dict(panel_kind=np.array([kind], np.int32), n_active=np.array([nact], np.int32))
+    d["particles_x"] = take(3 * n).reshape(n, 3)
+    if has_vort:
+        d["particles_relvort"] = take(n)
+        d["particles_absvort"] = take(n)
+    d["particles_tracer"] = take(n * nt).reshape(nt, n).T.copy()
+    d["vertices"] = take(kind * npan, "<i4").reshape(npan, kind)      # 1-based, as the reference
+    d["has_children"] = take(npan, "<i4")
+    d["panels_tracer"] = take(npan * nt).reshape(nt, npan).T.copy()
+    if has_vort:
+        d["panels_relvort"] = take(npan)
+        d["panels_absvort"] = take(npan)
+    assert off == len(raw), (off, len(raw))
+    return d
+
+
+def run_vtk_case(tmp, name, case):
+    import shutil
+    vtk, dump = os.path.join(tmp, name + ".vtk"), os.path.join(tmp, name + ".bin")
+    subprocess.run([VTK_HARNESS, str(case), vtk, dump], check=True, cwd=tmp,
+                   stdout=subprocess.DEVNULL)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **read_vtk_dump(dump))
+    shutil.copy(vtk, os.path.join(HERE, name + ".vtk"))
+    print(name, os.path.getsize(vtk))
 
 
 def run_driver(tmp, name, binary, namelist):
@@ -181,6 +235,8 @@ def main():
                                    n_passive=d["npp"], description=desc)
         for name, (binary, namelist) in DRIVER_CASES.items():
             run_driver(tmp, name, binary, namelist)
+        for name, (case, desc) in VTK_CASES.items():
+            run_vtk_case(tmp, name, case)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(dict(
             source="reference PanelVelocityParallelModule::BVERK4 (PanelVelocityParallel.f90:435-658), "
@@ -196,7 +252,11 @@ def main():
             advection_source="reference AdvectionRK4 / DivergentAdvectionRK4 "
                              "(PanelVelocityParallel.f90:922-1279, winds :1415-1459); only the "
                              "positions change",
-            advection_cases=advection), f, indent=1)
+            advection_cases=advection,
+            vtk_source="reference vtkOutput / vtkOutputUnif (ParticlesEdgesPanels2.f90:2116-2260) "
+                       "run by make_vtk_golden.f90; <name>.vtk is the frame, <name>.npz the arrays "
+                       "it was written from (vertices 1-based)",
+            vtk_cases={k: v[1] for k, v in VTK_CASES.items()}), f, indent=1)
 
 
 if __name__ == "__main__":

@@ -14,12 +14,14 @@ import pytest
 import oracle
 from conftest import GOLDEN, ROOT
 from fortran_io import read_dat, read_host_output, write_host_input
+from vtk_io import read_frame, rewrite
 from thesiscode_amd import grid as gridmod
 from thesiscode_amd import initial_conditions as ic
 
 pytestmark = pytest.mark.gpu
 HOST = os.path.join(ROOT, "thesiscode_amd", "bve_host.exe")
 DROPIN = os.path.join(ROOT, "oracle", "_ref", "rh4_gpu")
+REFERENCE = os.path.join(ROOT, "oracle", "_ref", "rh4_ref")
 
 
 @pytest.mark.skipif(not os.path.exists(HOST), reason="Fortran host not built (no flang)")
@@ -51,6 +53,8 @@ def test_fortran_host_matches_oracle(tmp_path):
 
 
 def run_dropin(tmp_path, namelist, binary=DROPIN, namelist_name="RHWavePanels.namelist"):
+    """Run a driver in tmp_path on a committed namelist; returns its .dat
+    table (its frames are left under tmp_path / vtkOut)."""
     shutil.copy(os.path.join(GOLDEN, namelist), tmp_path / namelist_name)
     (tmp_path / "vtkOut").mkdir()
     r = subprocess.run([binary], cwd=tmp_path, capture_output=True, text=True, timeout=300)
@@ -92,6 +96,42 @@ def test_reference_driver_remesh_on_the_dropin(tmp_path):
     assert np.allclose(got[5:, :2], ref[5:, :2], rtol=1e-8, atol=0)
     # error norms relative to their own size (~1e-3 .. 1e-6)
     assert np.allclose(got[5:, 2:], ref[5:, 2:], rtol=1e-5, atol=1e-12)
+
+
+@pytest.mark.skipif(not (os.path.exists(DROPIN) and os.path.exists(REFERENCE)),
+                    reason="the drivers are built only where the reference sources are")
+def test_reference_driver_frames_match_reference_frames(tmp_path):
+    """The 13 VTK frames RHWavePanelsMPI writes (one per step: particle
+    positions, tracers incl. stage-1 kinetic energy, particle and panel
+    vorticity) from the drop-in against those of the reference build run
+    here on the host, field by field; and every drop-in frame read back and
+    written again by libbve's native writer is byte for byte the file the
+    reference's vtkOutput wrote."""
+    dirs = {}
+    for tag, binary in (("gpu", DROPIN), ("ref", REFERENCE)):
+        d = tmp_path / tag
+        d.mkdir()
+        run_dropin(d, "rh4_driver_L4_noremesh.namelist", binary)
+        dirs[tag] = d / "vtkOut"
+    names = sorted(os.listdir(dirs["ref"]))
+    assert len(names) == 13 and names == sorted(os.listdir(dirs["gpu"]))
+    worst = {}
+    for name in names:
+        got, ref = read_frame(dirs["gpu"] / name), read_frame(dirs["ref"] / name)
+        assert got["cells"] == ref["cells"]
+        assert np.abs(got["points"] - ref["points"]).max() < 1e-13
+        for sec in ("point_data", "cell_data"):
+            assert list(got[sec]) == list(ref[sec])
+            for key, r in ref[sec].items():
+                # relative to the field's size; the error tracer (Tracer3, a
+                # small error normalised by max |zeta|) relative to 1
+                err = np.abs(got[sec][key] - r).max() / max(np.abs(r).max(), 1.0)
+                worst[key] = max(worst.get(key, 0.0), err)
+                assert err < 1e-11, (name, key, err)
+        out = tmp_path / ("native_" + name)
+        rewrite(got, out)
+        assert open(out, "rb").read() == open(dirs["gpu"] / name, "rb").read(), name
+    print("worst normwise relative difference per field", worst)
 
 
 STRAT_DROPIN = os.path.join(ROOT, "oracle", "_ref", "strat_gpu")

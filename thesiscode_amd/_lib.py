@@ -66,6 +66,8 @@ SIGNATURES = {
     "bve_get_plan": (_i, [_p, _p, _p, _p, _p, _p]),
     "bve_get_launch": (_i, [_p, _p, _p, _p, _p]),
     "bve_pairs_per_step": (_i, [_p, _dp]),
+    "bve_vtk_write_frame": (_i, [ctypes.c_char_p, _i, _i, _p, _p, _p, _i, _p, _i, _i, _i, _i,
+                                 _p, _p, _p, _p, _p]),
 }
 
 _lib = None

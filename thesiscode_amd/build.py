@@ -15,6 +15,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbve.so")
 SOURCES = ["bve_kernels.hip", "bve_mixed.hip", "bve_adjoint.hip", "bve_ensemble.hip", "bve_api.hip"]
+# host-only C++ (no device code): compiled by the host compiler
+CXX_SOURCES = ["bve_vtk.cpp"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall"]
 HEADERS = ["bve_common.h", "bve_kernels.h"]
 ARCH = os.environ.get("BVE_OFFLOAD_ARCH", "gfx950")
 
@@ -33,6 +36,13 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def cxx():
+    for cand in (os.environ.get("CXX"), shutil.which("g++"), shutil.which("c++")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("no host C++ compiler found")
+
+
 def build(verbose=False, extra_flags=()):
     """Compile every HIP source and link libbve.so (always rebuilds)."""
     objs = []
@@ -46,8 +56,16 @@ def build(verbose=False, extra_flags=()):
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
+    for src in CXX_SOURCES:
+        obj = os.path.join(build_dir, src.replace(".cpp", ".o"))
+        cmd = [cxx(), *CXX_FLAGS, "-I", CSRC, "-I", os.path.join(ROOT, "include"),
+               "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-pthread", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

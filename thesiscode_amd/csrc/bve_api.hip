@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "bve_common.h"
+#include "bve_host_error.h"
 #include "bve_kernels.h"
 #include "../../include/bve.h"
 
@@ -15,7 +16,6 @@ using namespace bve;
 
 namespace {
 
-thread_local std::string g_create_error;
 
 template <class P>
 hipError_t dev_alloc(P **p, size_t count)
@@ -711,30 +711,30 @@ extern "C" {
 int bve_create(bve_ctx **out, int device)
 {
     if (!out) {
-        g_create_error = "bve_create: null output pointer";
+        context_free_error() = "bve_create: null output pointer";
         return BVE_ERR_ARGUMENT;
     }
     *out = nullptr;
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n == 0) {
-        g_create_error = std::string("bve_create: no HIP device: ") + hipGetErrorString(e);
+        context_free_error() = std::string("bve_create: no HIP device: ") + hipGetErrorString(e);
         return BVE_ERR_DEVICE;
     }
     if (device < 0 || device >= n) {
-        g_create_error = "bve_create: device index out of range";
+        context_free_error() = "bve_create: device index out of range";
         return BVE_ERR_ARGUMENT;
     }
     e = hipSetDevice(device);
     if (e != hipSuccess) {
-        g_create_error = std::string("bve_create: hipSetDevice: ") + hipGetErrorString(e);
+        context_free_error() = std::string("bve_create: hipSetDevice: ") + hipGetErrorString(e);
         return BVE_ERR_DEVICE;
     }
     bve_ctx *c = new bve_ctx;
     c->device = device;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        g_create_error = std::string("bve_create: hipStreamCreate: ") + hipGetErrorString(e);
+        context_free_error() = std::string("bve_create: hipStreamCreate: ") + hipGetErrorString(e);
         delete c;
         return BVE_ERR_DEVICE;
     }
@@ -746,7 +746,7 @@ int bve_create(bve_ctx **out, int device)
     e = hipMalloc(&c->queue, 2 * sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->queue, 0, 2 * sizeof(unsigned));
     if (e != hipSuccess) {
-        g_create_error = std::string("bve_create: queue: ") + hipGetErrorString(e);
+        context_free_error() = std::string("bve_create: queue: ") + hipGetErrorString(e);
         if (c->queue) (void)hipFree(c->queue);
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -783,7 +783,7 @@ int bve_destroy(bve_ctx *c)
 
 const char *bve_last_error(const bve_ctx *c)
 {
-    return c ? c->error.c_str() : g_create_error.c_str();
+    return c ? c->error.c_str() : context_free_error().c_str();
 }
 
 int bve_set_omega(bve_ctx *c, double omega)
@@ -1638,7 +1638,7 @@ extern "C" {
 int bve_ensemble_create(bve_ensemble **out, int device, int n_members)
 {
     if (!out || n_members < 1) {
-        g_create_error = "bve_ensemble_create: bad arguments";
+        context_free_error() = "bve_ensemble_create: bad arguments";
         return BVE_ERR_ARGUMENT;
     }
     *out = nullptr;
@@ -1661,7 +1661,7 @@ int bve_ensemble_create(bve_ensemble **out, int device, int n_members)
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         for (bve_ctx *d : e->members) bve_destroy(d);
         delete e;
-        g_create_error = "bve_ensemble_create: hipStreamCreate failed";
+        context_free_error() = "bve_ensemble_create: hipStreamCreate failed";
         return BVE_ERR_DEVICE;
     }
     *out = e;
@@ -1687,7 +1687,7 @@ int bve_ensemble_destroy(bve_ensemble *e)
 
 const char *bve_ensemble_last_error(const bve_ensemble *e)
 {
-    return e ? e->error.c_str() : g_create_error.c_str();
+    return e ? e->error.c_str() : context_free_error().c_str();
 }
 
 int bve_ensemble_set_params(bve_ensemble *e, double omega, double smooth, int precision,

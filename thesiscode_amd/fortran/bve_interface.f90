@@ -11,7 +11,7 @@ module BVEInterfaceModule
 	public :: bve_set_renormalize, bve_load_panels, bve_rk4_step, bve_store_panels, bve_reset
 	public :: bve_load_panel_absvort, bve_stratosphere_rk4_step, bve_advection_rk4_step
 	public :: bve_load_particles, bve_advance, bve_store_particles, bve_synchronize
-	public :: bve_voronoi_rk4_step, bve_error_message
+	public :: bve_voronoi_rk4_step, bve_error_message, bve_vtk_write_frame
 
 	integer(c_int), parameter :: BVE_OK = 0
 	integer(c_int), parameter :: BVE_PRECISION_FP64 = 0, BVE_PRECISION_MIXED = 1, &
@@ -154,6 +154,19 @@ module BVEInterfaceModule
 			real(c_double), intent(inout) :: x(*), y(*), z(*), relVort(*)
 			real(c_double), intent(in) :: area(*)
 			real(c_double), intent(out) :: totalKE, ke(*)
+		end function
+
+		! One VTK frame in vtkOutputUnif's format; arrays that may be absent
+		! (vorticity, tracers, hasChildren) are passed as c_loc(...) or c_null_ptr.
+		integer(c_int) function bve_vtk_write_frame(filename, nParticles, ldParticles, xParticles, &
+				relVort, absVort, nTracer, tracer, panelKind, nPanels, nActive, ldPanels, vertices, &
+				hasChildren, panelTracer, panelRelVort, panelAbsVort) bind(C, name="bve_vtk_write_frame")
+			import :: c_int, c_ptr, c_char
+			character(kind=c_char), intent(in) :: filename(*)
+			integer(c_int), value :: nParticles, ldParticles, nTracer, panelKind, nPanels, nActive, &
+				ldPanels
+			type(c_ptr), value :: xParticles, relVort, absVort, tracer, vertices, hasChildren, &
+				panelTracer, panelRelVort, panelAbsVort
 		end function
 	end interface
 

@@ -55,6 +55,8 @@ class Panels:
     absVort: np.ndarray                 # (N,)
     x0: Optional[np.ndarray] = None
     tracer: Optional[np.ndarray] = None  # (N, nTracer)
+    vertices: Optional[np.ndarray] = None  # (N, panelKind) 0-based particle indices (corners)
+    panelKind: int = 4                     # QUAD_PANEL; 3 = TRI_PANEL
 
     @property
     def N(self):
@@ -69,15 +71,17 @@ def panels_from_grid(grid, relvort_active, relvort_passive=None, omega=OMEGA_EAR
     """Panels of a uniform cubed sphere (thesiscode_amd.grid) in tree order:
     the passive (divided) panels of levels 0..L-1 come first in the
     reference's numbering, then the leaves; GatherPanels only needs the
-    active/passive split, which this preserves."""
+    active/passive split, which this preserves.  Corner lists are kept for
+    the leaves only (-1 for divided panels, which no frame writes)."""
     na, npp = grid.n_active, grid.n_passive
     x = np.concatenate([grid.passive_x, grid.active_x])
     area = np.concatenate([np.zeros(npp), grid.active_area])
     has = np.concatenate([np.ones(npp, bool), np.zeros(na, bool)])
     rv = np.concatenate([np.zeros(npp) if relvort_passive is None else relvort_passive,
                          relvort_active])
+    verts = np.concatenate([np.full((npp, 4), -1, np.int64), grid.active_verts])
     return Panels(x=x, area=area, hasChildren=has, relVort=rv, absVort=rv + 2.0 * omega * x[:, 2],
-                  x0=x.copy(), tracer=np.zeros((na + npp, n_tracer)))
+                  x0=x.copy(), tracer=np.zeros((na + npp, n_tracer)), vertices=verts)
 
 
 def gather_panels(panels):

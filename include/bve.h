@@ -40,7 +40,12 @@ enum {
     BVE_PRECISION_MIXED = 1,      /* float32 far field, float64 near field and reductions;
                                      float32 sums of each 256-source tile carried in float64 */
     BVE_PRECISION_FP64_FAST = 2,  /* float64 pairs, one reciprocal shared by 2 or 4 pairs */
-    BVE_PRECISION_MIXED_KAHAN = 3 /* as MIXED with Kahan-compensated float32 sums */
+    BVE_PRECISION_MIXED_KAHAN = 3, /* as MIXED with Kahan-compensated float32 sums */
+    BVE_PRECISION_REFERENCE = 4   /* float64 in the reference's own order and rounding: one
+                                     target per lane, sources in index order, each term
+                                     ((x_i x x_k) zeta_k) A_k / d_ik with IEEE divisions, no
+                                     FMA contraction; BVERK4 results (panel and Voronoi) equal
+                                     the reference's bit for bit */
 };
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -63,9 +68,13 @@ int bve_set_omega(bve_ctx *ctx, double omega);
 int bve_set_smooth(bve_ctx *ctx, double smooth);
 int bve_get_smooth(const bve_ctx *ctx, double *smooth);
 /* Arithmetic of the pair sum: BVE_PRECISION_FP64 (default),
- * BVE_PRECISION_FP64_FAST (float64, relative error per pair ~1e-15) or
+ * BVE_PRECISION_FP64_FAST (float64, relative error per pair ~1e-15),
  * BVE_PRECISION_MIXED (float32 far field, float64 for pairs closer than
- * nearfield_radius, float64 target reduction).  No reference counterpart. */
+ * nearfield_radius, float64 target reduction) or BVE_PRECISION_REFERENCE
+ * (the reference's rounding, bit for bit; about 6x the cost of FP64, for
+ * runs that must reproduce a reference run exactly, e.g. through host
+ * re-triangulation, which is sensitive to the last bits of the positions).
+ * No reference counterpart. */
 int bve_set_precision(bve_ctx *ctx, int precision, double nearfield_radius);
 /* 1 = project every position back onto the unit sphere after each step
  * (RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702, applied on the

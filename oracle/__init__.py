@@ -38,6 +38,7 @@ def lib():
         L.bvo_smooth_velocity.argtypes = [_i, _p, _i, _p, _p, _p, _i, _i, _d, _p]
         L.bvo_voronoi_velocity.argtypes = [_i, _p, _p, _p, _p, _p, _i, _i, _d, _p, _p, _p, _p]
         L.bvo_rk4_step.argtypes = [_i, _p, _p, _p, _i, _p, _i, _p, _p, _p, _d, _d, _d, _p, _p]
+        L.bvo_voronoi_rk4_step.argtypes = [_i, _p, _p, _p, _p, _p, _p, _d, _d, _p, _p]
         L.bvo_renormalize.argtypes = [_i, _p]
         L.bvo_wind.argtypes = [_i, _p, _d, _p]
         L.bvo_advection_rk4_step.argtypes = [_i, _i, _p, _d, _d]
@@ -123,6 +124,20 @@ def rk4_step(active_x, active_vort, area, passive_x, particles_x, particles_absv
                        _ptr(qabs), dt, omega, smooth, _ptr(ke), ctypes.byref(tot))
     return dict(active_x=ax, active_relvort=av, passive_x=px, particles_x=qx,
                 particles_relvort=qrel, ke=ke, total_ke=tot.value)
+
+
+def voronoi_rk4_step(x, y, z, relvort, sum_area, cur_area, dt, omega):
+    """One Voronoi BVERK4 step (SoA) with NormalizeGrid.  sum_area weighs the
+    sources, cur_area enters only totalKE.  Returns (x, y, z, relvort, ke,
+    total_ke)."""
+    x, y, z, rv = (_c(a).copy() for a in (x, y, z, relvort))
+    sa, ca = _c(sum_area), _c(cur_area)
+    n = len(x)
+    ke = np.zeros(n)
+    tot = ctypes.c_double(0.0)
+    lib().bvo_voronoi_rk4_step(n, _ptr(x), _ptr(y), _ptr(z), _ptr(rv), _ptr(sa), _ptr(ca), dt,
+                               omega, _ptr(ke), ctypes.byref(tot))
+    return x, y, z, rv, ke, tot.value
 
 
 def wind(x, t, divergent=False):

@@ -20,6 +20,8 @@
  *       sum_i w_i u_i = 0, and a __float128 brute force (bvo_velocity_q).
  *   bvo_voronoi_velocity: equals bvo_active_rhs on SoA input (same sum,
  *       the reference's sequential-subtraction denominator), tangency.
+ *   bvo_voronoi_rk4_step: golden vectors from the compiled reference's
+ *       Voronoi BVERK4 on icosahedral Voronoi grids (bitwise), |x| = 1.
  *   bvo_rk4_step: RK4 polynomial on a solid-body flow, zeta + 2 Omega z
  *       invariant, golden vectors over 12 steps.
  *   bvo_renormalize: |x| = 1 within 1 ulp.
@@ -216,6 +218,67 @@ void bvo_rk4_step(int na, double *ax, double *avort, const double *area,
         for (int i = 0; i < np; ++i) qrel[i] = qabs[i] - 2.0 * omega * qx[3 * i + 2];
     free(ain); free(vin); free(pin); free(qin);
     for (int s = 0; s < 4; ++s) { free(ka[s]); free(kv[s]); free(kp[s]); free(kq[s]); }
+}
+
+/* One BVERK4 step of the Voronoi module, VoronoiVelocityParallel.f90:304-483,
+ * on one rank, on n generators in SoA form (in/out x, y, z, relvort).
+ * sum_area (n) weighs the sources (inputArea: the saved or the current
+ * areas, :331-335); cur_area (n) is aPanels%area, which only the kinetic
+ * energy integral uses (:358-361).  Stage inputs x + a k (:372-375,
+ * :399-402, :427-430), k_s = dt u_s (:363-366), update x + k1/6 + k2/3 +
+ * k3/3 + k4/6 left to right (:455-467), then NormalizeGrid (:480,
+ * VoronoiPanels.f90:395-406): x / sqrt(x x + y y + z z).
+ * Out: ke (n) = |u|^2 at stage 1, *total_ke = 1/2 sum ke * cur_area. */
+void bvo_voronoi_rk4_step(int n, double *x, double *y, double *z, double *relvort,
+                          const double *sum_area, const double *cur_area, double dt,
+                          double omega, double *ke, double *total_ke)
+{
+    const size_t nb = sizeof(double) * (size_t)n;
+    double *in[4], *k[4][4];
+    for (int c = 0; c < 4; ++c) {
+        in[c] = malloc(nb);
+        for (int s = 0; s < 4; ++s) k[s][c] = malloc(nb);
+    }
+    double *state[4] = {x, y, z, relvort};
+    static const double stage_a[4] = {0.0, 0.5, 0.5, 1.0};
+    for (int s = 0; s < 4; ++s) {
+        for (int c = 0; c < 4; ++c) {
+            if (s == 0)
+                memcpy(in[c], state[c], nb);
+            else if (s < 3)
+                for (int i = 0; i < n; ++i) in[c][i] = state[c][i] + stage_a[s] * k[s - 1][c][i];
+            else
+                for (int i = 0; i < n; ++i) in[c][i] = state[c][i] + k[2][c][i];
+        }
+        bvo_voronoi_velocity(n, in[0], in[1], in[2], in[3], sum_area, 0, n, omega, k[s][0],
+                             k[s][1], k[s][2], k[s][3]);
+        if (s == 0) {
+            double tot = 0.0;
+            for (int j = 0; j < n; ++j) {
+                const double e = (k[0][0][j] * k[0][0][j] + k[0][1][j] * k[0][1][j]) +
+                                 k[0][2][j] * k[0][2][j];
+                if (ke) ke[j] = e;
+                tot = tot + e * cur_area[j];
+            }
+            if (total_ke) *total_ke = 0.5 * tot;
+        }
+        for (int c = 0; c < 4; ++c)
+            for (int i = 0; i < n; ++i) k[s][c][i] = dt * k[s][c][i];
+    }
+    for (int c = 0; c < 4; ++c)
+        for (int i = 0; i < n; ++i)
+            state[c][i] = state[c][i] + k[0][c][i] / 6.0 + k[1][c][i] / 3.0 + k[2][c][i] / 3.0 +
+                          k[3][c][i] / 6.0;
+    for (int j = 0; j < n; ++j) {
+        const double r = sqrt(x[j] * x[j] + y[j] * y[j] + z[j] * z[j]);
+        x[j] = x[j] / r;
+        y[j] = y[j] / r;
+        z[j] = z[j] / r;
+    }
+    for (int c = 0; c < 4; ++c) {
+        free(in[c]);
+        for (int s = 0; s < 4; ++s) free(k[s][c]);
+    }
 }
 
 /* atan4, SphereGeom.f90:213-248: atan2 with values in [0, 2 PI), by the

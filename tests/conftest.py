@@ -26,6 +26,21 @@ def forced_manifest():
         return json.load(f)["forced_cases"]
 
 
+def voronoi_manifest():
+    """Golden cases of the reference's Voronoi BVERK4."""
+    with open(os.path.join(GOLDEN, "manifest.json")) as f:
+        return json.load(f)["voronoi_cases"]
+
+
+def voronoi_step_inputs(g, c, s):
+    """Inputs of step s (1-based) of a Voronoi golden case: the state after
+    step s-1, the areas the sum weighs (the initial tessellation's with
+    UseSavedArea, else the current ones) and the current areas."""
+    p = f"s{s - 1}_"
+    sum_area = g["s0_area"] if c["saved_area"] else g[p + "area"]
+    return g[p + "x"], g[p + "y"], g[p + "z"], g[p + "relvort"], sum_area, g[p + "area"]
+
+
 def load_golden(name):
     """Arrays of one golden case (written by tests/golden/make_golden.py)."""
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:

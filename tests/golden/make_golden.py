@@ -9,7 +9,8 @@ compressed ``.npz`` (float64 / int32 arrays only, loadable with
 It also runs the reference's own drivers (RHWavePanelsMPI.f90,
 StratospherePanels.f90, JetPanels.f90, NonDivAdvection.f90; unmodified,
 built by the same makefile's `drivers` target) on the committed namelists
-and keeps their .dat tables.
+and keeps their .dat tables.  ``make_golden.py voronoi`` redoes only the
+Voronoi cases (make_voronoi_golden.f90 and the two Voronoi drivers).
 
 Run from the repository root, in a container that has the reference:
 
@@ -82,6 +83,67 @@ DRIVER_CASES = {
     "jet_driver_L4_amr2_remesh5": ("jet_ref", "JetPanels.namelist"),
     "adv_driver_L3": ("adv_ref", "AdvectGaussHills.namelist"),
 }
+# the reference's Voronoi Gaussian-vortex driver (GaussVortVoronoi.f90, MPI
+# module) and its serial RH4 Voronoi driver (RHWaveVoronoi.f90, serial module)
+VORONOI_DRIVER_CASES = {
+    "vor_driver_L3": ("vor_ref", "GaussVortVoronoi.namelist"),
+    "vors_driver_L3": ("vors_ref", "RHWaveVoronoi.namelist"),
+}
+
+
+# Voronoi BVERK4 (VoronoiVelocityParallel.f90:304-483) on icosahedral Voronoi
+# grids (tests/golden/make_voronoi_golden.f90):
+# name: (initNest, ic, nsteps, dt, saved_area, description)
+VORONOI_CASES = {
+    "vor_rh4_L3_1step": (3, 0, 1, 0.01, 0,
+                         "icosahedral Voronoi grid, 642 generators, reference RH4 wave, one step"),
+    "vor_rh4_L4_2steps": (4, 0, 2, 0.01, 0,
+                          "icosahedral Voronoi grid, 2562 generators, reference RH4 wave, two "
+                          "steps, areas re-tessellated after each"),
+    "vor_rh4_L3_3steps_saved": (3, 0, 3, 0.01, 1,
+                                "642 generators, reference RH4 wave, three steps summed with the "
+                                "initial areas (UseSavedArea)"),
+    "vor_gauss_L3_2steps": (3, 1, 2, 0.02, 0,
+                            "642 generators, reference Gaussian vortex, two steps dt=0.02"),
+}
+VORONOI_HARNESS = os.path.join(ROOT, "oracle", "_ref", "make_voronoi_golden")
+
+
+def read_voronoi_dump(path):
+    """Parse the stream file written by make_voronoi_golden.f90."""
+    raw = open(path, "rb").read()
+    n, nsteps, saved = (int(v) for v in np.frombuffer(raw, "<i4", 3, 0))
+    dt, omega = np.frombuffer(raw, "<f8", 2, 12)
+    off = 28
+
+    def take(count):
+        nonlocal off
+        a = np.frombuffer(raw, "<f8", count, off).copy()
+        off += 8 * count
+        return a
+
+    d = dict(n=n, nsteps=nsteps, saved=saved, dt=float(dt), omega=float(omega))
+    arrays = {}
+    for s in range(nsteps + 1):
+        for key in ("x", "y", "z", "relvort", "area"):
+            arrays[f"s{s}_{key}"] = take(n)
+    for s in range(1, nsteps + 1):
+        arrays[f"ke_step{s}"] = take(n)
+        arrays[f"total_ke_step{s}"] = take(1)
+    assert off == len(raw), (off, len(raw))
+    return d, arrays
+
+
+def run_voronoi_case(tmp, name, nest, ic, nsteps, dt, saved):
+    out = os.path.join(tmp, name + ".bin")
+    # the reference's STRIPACK can loop for ever on a degenerate point set,
+    # hence the time limit
+    subprocess.run([VORONOI_HARNESS, str(nest), str(ic), str(nsteps), repr(dt), str(saved), out],
+                   check=True, cwd=tmp, stdout=subprocess.DEVNULL, timeout=120)
+    d, arrays = read_voronoi_dump(out)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+    print(name, d["n"])
+    return d
 
 
 # VTK frames written by the reference's vtkOutput (tests/golden/make_vtk_golden.f90):
@@ -211,6 +273,31 @@ def run_case(tmp, name, level, ic, nsteps, dt, smooth, t0=None):
     return d
 
 
+def main_voronoi():
+    """Only the Voronoi cases; merged into an existing manifest.json."""
+    if not os.path.exists(VORONOI_HARNESS):
+        sys.exit("build the harness first: make -f oracle/Makefile.ref")
+    cases = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, (nest, ic, nsteps, dt, saved, desc) in VORONOI_CASES.items():
+            d = run_voronoi_case(tmp, name, nest, ic, nsteps, dt, saved)
+            cases[name] = dict(init_nest=nest, ic=ic, nsteps=nsteps, dt=d["dt"], omega=d["omega"],
+                               saved_area=bool(saved), n=d["n"], description=desc)
+        for name, (binary, namelist) in VORONOI_DRIVER_CASES.items():
+            run_driver(tmp, name, binary, namelist)
+    path = os.path.join(HERE, "manifest.json")
+    with open(path) as f:
+        manifest = json.load(f)
+    manifest["voronoi_source"] = (
+        "reference VoronoiVelocityParallelModule::BVERK4 (VoronoiVelocityParallel.f90:304-483) on "
+        "a grid from VoronoiPanelsModule::New (VoronoiPanels.f90:245-315, STRIPACK), one rank; "
+        "s<k>_* = state after step k (s0 initial; area = that state's tessellation), ke_step<k> / "
+        "total_ke_step<k> = the stage-1 kinetic energy of step k")
+    manifest["voronoi_cases"] = cases
+    with open(path, "w") as f:
+        json.dump(manifest, f, indent=1)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -f oracle/Makefile.ref")
@@ -260,4 +347,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["voronoi"]:
+        main_voronoi()
+    else:
+        main()
+        main_voronoi()

@@ -52,16 +52,26 @@ def test_fortran_host_matches_oracle(tmp_path):
     assert np.abs(got["particles_relvort"] - ref["particles_relvort"]).max() < 1e-12
 
 
-def run_dropin(tmp_path, namelist, binary=DROPIN, namelist_name="RHWavePanels.namelist"):
+def run_dropin(tmp_path, namelist, binary=DROPIN, namelist_name="RHWavePanels.namelist",
+               precision=None, raw=False):
     """Run a driver in tmp_path on a committed namelist; returns its .dat
-    table (its frames are left under tmp_path / vtkOut)."""
+    table (its frames are left under tmp_path / vtkOut), or the file's bytes
+    with raw.  precision sets BVE_PRECISION for the drop-in modules.  The
+    time limit is short: a driver that stops progressing (the reference's
+    STRIPACK can loop for ever on some inputs) fails the test."""
     shutil.copy(os.path.join(GOLDEN, namelist), tmp_path / namelist_name)
     (tmp_path / "vtkOut").mkdir()
-    r = subprocess.run([binary], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    env = dict(os.environ)
+    env.pop("BVE_PRECISION", None)
+    if precision:
+        env["BVE_PRECISION"] = precision
+    print("driver", os.path.basename(binary), namelist, precision, flush=True)
+    r = subprocess.run([binary], cwd=tmp_path, capture_output=True, text=True, timeout=120,
+                       env=env)
     assert r.returncode == 0, (r.stdout + r.stderr)[-2000:]
     dat = list(tmp_path.glob("*.dat"))
     assert len(dat) == 1
-    return read_dat(dat[0])
+    return open(dat[0], "rb").read() if raw else read_dat(dat[0])
 
 
 @pytest.mark.skipif(not os.path.exists(DROPIN),
@@ -214,3 +224,118 @@ def test_reference_advection_driver_runs_on_the_dropin(tmp_path):
     ref_err, ref_tab = read_advection_dat(os.path.join(GOLDEN, "adv_driver_L3.dat"))
     assert np.allclose(got_err, ref_err, rtol=1e-8, atol=0), (got_err, ref_err)
     assert np.array_equal(got_tab, ref_tab)
+
+
+VOR_DROPIN = os.path.join(ROOT, "oracle", "_ref", "vor_gpu")
+
+
+@pytest.mark.skipif(not os.path.exists(VOR_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_voronoi_driver_runs_on_the_dropin(tmp_path):
+    """GaussVortVoronoi (icosahedral Voronoi grid, 642 generators, 20 steps
+    at dt = 0.01 summed with the saved initial areas, the reference's
+    STRIPACK re-triangulation after every step) linked against the drop-in
+    VoronoiVelocityParallelModule, in float64 (BVE_PRECISION=fp64),
+    reproduces the reference's .dat table -- totalKE and enstrophy of every
+    step -- to the printed 15 decimals."""
+    got = run_dropin(tmp_path, "vor_driver_L3.namelist", VOR_DROPIN, "GaussVortVoronoi.namelist",
+                     precision="fp64")
+    ref = read_dat(os.path.join(GOLDEN, "vor_driver_L3.dat"))
+    assert got.shape == ref.shape
+    assert np.allclose(got, ref, rtol=1e-12, atol=2e-15), np.abs(got - ref).max(axis=0)
+
+
+VORS_DROPIN = os.path.join(ROOT, "oracle", "_ref", "vors_gpu")
+
+
+@pytest.mark.skipif(not os.path.exists(VORS_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_serial_voronoi_driver_runs_on_the_dropin(tmp_path):
+    """RHWaveVoronoi (the serial driver: 642 generators, 20 steps at dt = 0.01,
+    saved areas) linked against the drop-in VoronoiVelocitySerialModule, in
+    the Voronoi modules' default reference-order arithmetic, writes the same
+    .dat file, byte for byte, as the driver linked against the reference's
+    VoronoiVelocitySerial2.f90: the positions handed to STRIPACK are the
+    reference's bit for bit.  (With float64 rounding instead, the reference's
+    STRIPACK loops for ever after step 9 of this trajectory: shown on the CPU
+    with the GPU kernel's arithmetic form.)"""
+    got = run_dropin(tmp_path, "vors_driver_L3.namelist", VORS_DROPIN, "RHWaveVoronoi.namelist",
+                     raw=True)
+    assert got == open(os.path.join(GOLDEN, "vors_driver_L3.dat"), "rb").read()
+
+
+RH4S_DROPIN = os.path.join(ROOT, "oracle", "_ref", "rh4s_gpu")
+
+
+def check_serial_panel_table(got, ref, exact):
+    """The serial RH4 driver (RHWavePanels.f90) records the l1 / l2 / linf
+    vorticity errors of every step but, unlike the MPI driver, never stores
+    totalKE or the enstrophy after step 0 (its arrays keep their zeros):
+    columns 3-5 and row 0 are compared with the MPI driver's table."""
+    assert got.shape == ref.shape
+    assert np.array_equal(got[1:, :2], np.zeros_like(got[1:, :2]))
+    assert np.array_equal(got[0], ref[0])
+    if exact:
+        assert np.array_equal(got[:, 2:], ref[:, 2:]), np.abs(got - ref)[:, 2:].max(axis=0)
+    else:
+        assert np.allclose(got[:, 2:], ref[:, 2:], rtol=1e-12, atol=2e-15), \
+            np.abs(got - ref)[:, 2:].max(axis=0)
+
+
+@pytest.mark.skipif(not os.path.exists(RH4S_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_serial_panel_driver_runs_on_the_dropin(tmp_path):
+    """RHWavePanels (the serial panel driver, which the reference's own
+    PanelVelocitySerial.f90 cannot be compiled for here) linked against the
+    drop-in PanelVelocitySerialModule in float64, on the namelist of the MPI
+    driver's golden run (level 4, 12 steps, no remesh): the RH4 error norms
+    of every step equal those the reference's RHWavePanelsMPI wrote with the
+    reference's PanelVelocityParallelModule, to the printed 15 decimals."""
+    got = run_dropin(tmp_path, "rh4_driver_L4_noremesh.namelist", RH4S_DROPIN, precision="fp64")
+    check_serial_panel_table(got, read_dat(os.path.join(GOLDEN, "rh4_driver_L4_noremesh.dat")),
+                             exact=False)
+
+
+@pytest.mark.skipif(not os.path.exists(RH4S_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_serial_panel_driver_bitwise_in_reference_mode(tmp_path):
+    """As above with BVE_PRECISION=reference: every printed digit of the
+    error norms equals the reference MPI driver's."""
+    got = run_dropin(tmp_path, "rh4_driver_L4_noremesh.namelist", RH4S_DROPIN,
+                     precision="reference")
+    check_serial_panel_table(got, read_dat(os.path.join(GOLDEN, "rh4_driver_L4_noremesh.dat")),
+                             exact=True)
+
+
+@pytest.mark.skipif(not os.path.exists(VOR_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_voronoi_driver_bitwise_in_reference_mode(tmp_path):
+    """GaussVortVoronoi on the drop-in in its default reference-order
+    arithmetic: the .dat file equals the reference run's byte for byte."""
+    got = run_dropin(tmp_path, "vor_driver_L3.namelist", VOR_DROPIN, "GaussVortVoronoi.namelist",
+                     raw=True)
+    assert got == open(os.path.join(GOLDEN, "vor_driver_L3.dat"), "rb").read()
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+@pytest.mark.parametrize("name", ["rh4_driver_L4_noremesh", "rh4_driver_L4_remesh5"])
+def test_reference_driver_bitwise_in_reference_mode(tmp_path, name):
+    """RHWavePanelsMPI on the drop-in with BVE_PRECISION=reference writes the
+    reference run's .dat file byte for byte -- also across the remesh every
+    5 steps, where the float64 mode can only follow the reference to the
+    interpolation-error scale."""
+    got = run_dropin(tmp_path, name + ".namelist", precision="reference", raw=True)
+    assert got == open(os.path.join(GOLDEN, name + ".dat"), "rb").read()
+
+
+@pytest.mark.skipif(not os.path.exists(JET_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+@pytest.mark.parametrize("name", ["jet_driver_L4_amr2", "jet_driver_L4_amr2_remesh5"])
+def test_reference_amr_jet_driver_bitwise_in_reference_mode(tmp_path, name):
+    """JetPanels (adaptively refined grid, with and without adaptive remesh)
+    on the drop-in with BVE_PRECISION=reference: the reference run's .dat
+    file byte for byte."""
+    got = run_dropin(tmp_path, name + ".namelist", JET_DROPIN, "JetPanels.namelist",
+                     precision="reference", raw=True)
+    assert got == open(os.path.join(GOLDEN, name + ".dat"), "rb").read()

@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import forced_manifest, golden_manifest, load_golden, normwise_rel
+from conftest import (forced_manifest, golden_manifest, load_golden, normwise_rel,
+                      voronoi_manifest, voronoi_step_inputs)
 from thesiscode_amd import grid as gridmod
 from thesiscode_amd import initial_conditions as ic
 
@@ -346,3 +347,51 @@ def test_winds_closed_form(divergent):
         moved = oracle.advection_rk4_step(moved, 0.7 + n * 0.01, 0.01, divergent)
     change = abs(area(moved) / a0 - 1)
     assert (change > 1e-3) if divergent else (change < 1e-4)
+
+
+VORONOI = sorted(voronoi_manifest())
+
+
+@pytest.mark.parametrize("name", VORONOI)
+def test_voronoi_oracle_matches_reference_bitwise(name):
+    """bvo_voronoi_rk4_step restates the Voronoi BVERK4 (SoA sum with the
+    sequential-subtraction denominator, RK4, NormalizeGrid) in the reference's
+    rounding order: every step of every case, from the reference's state
+    before it (the re-tessellated areas come from the reference's STRIPACK),
+    is reproduced bit for bit, and so are KE and totalKE."""
+    c = voronoi_manifest()[name]
+    g = load_golden(name)
+    for s in range(1, c["nsteps"] + 1):
+        x, y, z, rv, ke, tot = oracle.voronoi_rk4_step(*voronoi_step_inputs(g, c, s), c["dt"],
+                                                       c["omega"])
+        for key, got in (("x", x), ("y", y), ("z", z), ("relvort", rv)):
+            assert np.array_equal(got, g[f"s{s}_{key}"]), (name, s, key)
+        assert np.array_equal(ke, g[f"ke_step{s}"]), (name, s)
+        assert tot == g[f"total_ke_step{s}"][0], (name, s)
+
+
+def test_voronoi_golden_pins_the_saved_area():
+    """The saved-area case differs from summing with the current areas (so the
+    bitwise test above pins which areas the sum weighs), and the panel-form
+    denominator 1 - (x.y) differs from the Voronoi one in the last bits."""
+    c = voronoi_manifest()["vor_rh4_L3_3steps_saved"]
+    g = load_golden("vor_rh4_L3_3steps_saved")
+    x, y, z, rv, sa, ca = voronoi_step_inputs(g, c, 3)
+    assert not np.array_equal(sa, ca)
+    wrong = oracle.voronoi_rk4_step(x, y, z, rv, ca, ca, c["dt"], c["omega"])
+    assert not np.array_equal(wrong[0], g["s3_x"])
+    # the panel restatement (other denominator rounding, then renormalise)
+    pts = np.stack([x, y, z], 1)
+    r = oracle.rk4_step(pts, rv, sa, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), c["dt"],
+                        c["omega"], 0.05)
+    q = oracle.renormalize(r["active_x"])
+    exact = np.stack([g["s3_x"], g["s3_y"], g["s3_z"]], 1)
+    assert not np.array_equal(q, exact)
+    assert np.abs(q - exact).max() < 1e-14
+
+
+def test_voronoi_step_keeps_generators_on_the_sphere():
+    c = voronoi_manifest()["vor_rh4_L4_2steps"]
+    g = load_golden("vor_rh4_L4_2steps")
+    x, y, z, *_ = oracle.voronoi_rk4_step(*voronoi_step_inputs(g, c, 1), c["dt"], c["omega"])
+    assert np.abs(np.sqrt(x * x + y * y + z * z) - 1.0).max() <= 2.3e-16

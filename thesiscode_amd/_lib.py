@@ -16,6 +16,7 @@ BVE_PRECISION_FP64 = 0
 BVE_PRECISION_MIXED = 1
 BVE_PRECISION_FP64_FAST = 2
 BVE_PRECISION_MIXED_KAHAN = 3
+BVE_PRECISION_REFERENCE = 4
 
 # every symbol include/bve.h declares, with its ctypes signature
 _d, _i, _p, _ll = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong

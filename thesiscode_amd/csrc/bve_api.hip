@@ -45,6 +45,7 @@ struct bve_ctx {
     int precision = BVE_PRECISION_FP64;
     double nearfield = 0.02;
     int renormalize = 0;
+    int voronoi = 0;   // 1 while a Voronoi step runs (its module's denominator form)
     int plan_t = 0, plan_tile = 0, plan_wg = 0, plan_recip = 0;
     int plan_chunk = 0;   // explicit source chunk length (ensemble members), 0 = derived
     bool chunked_only = false;   // ensemble members: the batched launch runs chunked plans
@@ -224,8 +225,25 @@ int recip_for(const bve_ctx *c)
     return c->plan_recip ? c->plan_recip : 4;
 }
 
+// Reference-order pass: one chunk of all sources, 256 targets per workgroup.
+PairPlan make_exact_plan(int n_targets, int n_src_pad)
+{
+    PairPlan p{};
+    p.targets_per_lane = 1;
+    p.tile = kBlock;
+    p.recip = 1;
+    p.n_tiles = std::max(1, ceil_div(n_targets, kBlock));
+    p.chunk = n_src_pad;
+    p.n_chunks = 1;
+    p.n_work = p.n_tiles;
+    p.grid = p.n_tiles;
+    p.exact = 1;
+    return p;
+}
+
 PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
 {
+    if (c->precision == BVE_PRECISION_REFERENCE) return make_exact_plan(n_targets, n_src_pad);
     const bool mixed = c->precision == BVE_PRECISION_MIXED ||
                        c->precision == BVE_PRECISION_MIXED_KAHAN;
     // mixed: at most 2 targets per lane (register pressure of the two paths)
@@ -336,6 +354,7 @@ StepState step_state(bve_ctx *c)
     st.ke_total = c->ke_total;
     st.particle_absvort = c->particles_have_absvort ? c->absvort : nullptr;
     st.particle_relvort = c->relvort;
+    st.unweighted = c->precision == BVE_PRECISION_REFERENCE;
     return st;
 }
 
@@ -419,18 +438,23 @@ PairArgs pair_args_of(const bve_ctx *c, const PairPlan &plan)
     pa.cls.smooth_begin = c->n_active;
     pa.cls.smooth_end = c->n_active + c->n_passive;
     pa.cls.eps2 = c->smooth * c->smooth;
+    pa.cls.voronoi = c->voronoi;
     pa.src = c->src;
     pa.n_src_pad = c->n_src_pad;
     pa.plan = plan;
     pa.partial = c->partial;
     pa.queue = c->queue;
+    pa.area = c->area;
     return pa;
 }
 
 // One pair-sum pass over n_targets targets and n_sources sources in the
 // context's arithmetic (the mixed pass first builds its per-stage records).
-int pair_pass(bve_ctx *c, const PairArgs &a, int n_targets, int n_sources)
+int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
 {
+    PairArgs a = a_in;
+    a.n_targets = n_targets;
+    a.n_sources = n_sources;
     MixedArgs ma{};
     if (is_mixed(c)) {
         int rc = ensure_mixed(c, a.n_src_pad, a.t_stride);
@@ -445,6 +469,8 @@ int pair_pass(bve_ctx *c, const PairArgs &a, int n_targets, int n_sources)
     }
     if (is_mixed(c))
         HIP_TRY(c, launch_pair_sum_mixed(ma, c->stream));
+    else if (a.plan.exact)
+        HIP_TRY(c, launch_pair_sum_exact(a, c->stream));
     else
         HIP_TRY(c, launch_pair_sum_f64(a, c->stream));
     if (c->profiling) {
@@ -811,7 +837,8 @@ int bve_set_precision(bve_ctx *c, int precision, double nearfield_radius)
 {
     if (!c) return BVE_ERR_ARGUMENT;
     if (precision != BVE_PRECISION_FP64 && precision != BVE_PRECISION_MIXED &&
-        precision != BVE_PRECISION_FP64_FAST && precision != BVE_PRECISION_MIXED_KAHAN)
+        precision != BVE_PRECISION_FP64_FAST && precision != BVE_PRECISION_MIXED_KAHAN &&
+        precision != BVE_PRECISION_REFERENCE)
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: unknown precision %d", precision);
     if (!(nearfield_radius >= 0.0))
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: nearfield_radius must be >= 0");
@@ -1154,7 +1181,8 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
         VTRY(hipMemcpyAsync(xs, x_sources, 3 * sizeof(double) * n_sources, hipMemcpyHostToDevice, c->stream));
         VTRY(hipMemcpyAsync(vv, vort, sizeof(double) * n_sources, hipMemcpyHostToDevice, c->stream));
         VTRY(hipMemcpyAsync(aa, area, sizeof(double) * n_sources, hipMemcpyHostToDevice, c->stream));
-        VTRY(launch_make_sources(xs, vv, aa, n_sources, src, c->stream));
+        VTRY(launch_make_sources(xs, vv, aa, n_sources, c->precision == BVE_PRECISION_REFERENCE,
+                                 src, c->stream));
     }
     rc = ensure_staging(c, 3 * (size_t)n_targets);
     if (rc) { cleanup(); return rc; }
@@ -1177,6 +1205,7 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
     pa.plan = plan;
     pa.partial = c->partial;
     pa.queue = c->queue;
+    pa.area = aa;
     rc = pair_pass(c, pa, n_targets, n_sources);
     if (rc) { (void)hipStreamSynchronize(c->stream); cleanup(); return rc; }
     if (dvort && self_end > 0) VTRY(dev_alloc(&dv, self_end));
@@ -1209,8 +1238,10 @@ int bve_voronoi_rk4_step(bve_ctx *c, double dt, int n, double *x, double *y, dou
     rc = ensure_particles(c, 0);
     if (rc) return rc;
     c->renormalize = 1;   // NormalizeGrid after every Voronoi step (VoronoiVelocityParallel.f90:480)
+    c->voronoi = 1;
     rc = run_steps(c, dt, 1);
     c->renormalize = saved_renorm;
+    c->voronoi = 0;
     if (rc) return rc;
     rc = bve_store_panels(c, aos.data(), relvort, nullptr);
     if (rc) return rc;
@@ -1694,6 +1725,9 @@ int bve_ensemble_set_params(bve_ensemble *e, double omega, double smooth, int pr
                             double nearfield_radius)
 {
     if (!e) return BVE_ERR_ARGUMENT;
+    if (precision == BVE_PRECISION_REFERENCE)
+        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_set_params: the reference-order mode "
+                       "runs per context, not in the batched launch");
     for (bve_ctx *c : e->members) {
         int rc = bve_set_omega(c, omega);
         if (!rc) rc = bve_set_smooth(c, smooth);

@@ -28,6 +28,10 @@ struct TargetClasses {
     int smooth_begin;
     int smooth_end;
     double eps2;
+    // reference-order pass only: 1 = the Voronoi module's denominator
+    // ((1 - x_j x_k) - y_j y_k) - z_j z_k (VoronoiVelocityParallel.f90:502),
+    // 0 = the panel module's 1 - ((x_k x_j + y_k y_j) + z_k z_j) (:1320)
+    int voronoi;
 };
 
 // Launch geometry of one pair-sum pass (fixed per context and size).
@@ -48,6 +52,11 @@ struct PairPlan {
     int segments;
     int n_src_tiles;        // n_src_pad / tile
     int seg_len;
+    // Reference-order pass (exact = 1): one target per lane, every source in
+    // index order, the reference's own expression and rounding per pair; one
+    // chunk, whose rows hold the sums of the cross-product terms themselves
+    // (u = S / (-4 PI)) rather than the factored S = sum (w/d) x.
+    int exact;
 };
 
 __host__ __device__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }

@@ -63,6 +63,92 @@ __device__ __forceinline__ bool diag_tile(int lo, int self_end, int jt, int tile
 }
 
 // ---------------------------------------------------------------------------
+// Reference-order pair body
+// ---------------------------------------------------------------------------
+
+// The sums of BVEActiveRHS / BVEPassiveRHS / BVESmoothVelocity
+// (PanelVelocityParallel.f90:1287-1412) and of the Voronoi BVEVelocity
+// (VoronoiVelocityParallel.f90:487-515) in the reference's own order and
+// rounding: target i (one per lane) adds, for k = 0, 1, ... in index order
+// (skipping k = i for the self-excluded targets), the terms
+//     ((x_i,b x_k,c - x_i,c x_k,b) zeta_k) A_k / denom
+// each divided separately (IEEE division), into sums that start at 0, with
+// no contraction into FMA: the float64 results equal the reference's
+// (compiled without FMA) bit for bit.  The records carry w = zeta_k and the
+// areas come from a.area; sources stream through LDS 256 at a time, and all
+// lanes of a wave read the same source (an LDS broadcast).
+__device__ __forceinline__ void pair_exact_body(const PairArgs &a, double4 *lds_src,
+                                                double *lds_area)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < a.n_targets;
+    const double X0 = live ? a.tx[i] : 0.0, X1 = live ? a.ty[i] : 0.0, X2 = live ? a.tz[i] : 0.0;
+    const bool self = i < a.cls.self_end;
+    const bool smoothed = i >= a.cls.smooth_begin && i < a.cls.smooth_end;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int base = 0; base < a.n_sources; base += kBlock) {
+        __syncthreads();
+        const int k = base + threadIdx.x;
+        if (k < a.n_sources) {
+            const Source q = a.src[k];
+            lds_src[threadIdx.x] = make_double4(q.x, q.y, q.z, q.w);
+            lds_area[threadIdx.x] = a.area[k];
+        }
+        __syncthreads();
+        const int m = min(kBlock, a.n_sources - base);
+        if (!live) continue;
+        for (int kk = 0; kk < m; ++kk) {
+            if (self && base + kk == i) continue;
+            const double4 q = lds_src[kk];
+            const double ar = lds_area[kk];
+            double denom;
+            if (a.cls.voronoi) {
+                denom = 1.0 - X0 * q.x - X1 * q.y - X2 * q.z;
+            } else {
+                denom = 1.0 - ((q.x * X0 + q.y * X1) + q.z * X2);
+                if (smoothed) denom = denom + a.cls.eps2;
+            }
+            a0 = a0 + ((X1 * q.z - X2 * q.y) * q.w) * ar / denom;
+            a1 = a1 + ((X2 * q.x - X0 * q.z) * q.w) * ar / denom;
+            a2 = a2 + ((X0 * q.y - X1 * q.x) * q.w) * ar / denom;
+        }
+    }
+    if (live) {
+        a.partial[i] = a0;
+        a.partial[a.t_stride + i] = a1;
+        a.partial[2 * (size_t)a.t_stride + i] = a2;
+    }
+}
+
+// Velocity of target (x, y, z) from its reduced sums: u = (x x S) / (-4 PI)
+// for the factored sums, u = S / (-4 PI) for the reference-order pass, whose
+// sums are of the cross-product terms (PanelVelocityParallel.f90:1332).
+__device__ __forceinline__ void velocity_from_sums(int exact, double x, double y, double z,
+                                                   double Sx, double Sy, double Sz, double &ux,
+                                                   double &uy, double &uz)
+{
+#pragma clang fp contract(off)
+    const double c4 = -4.0 * kPi;
+    if (exact) {
+        ux = Sx / c4;
+        uy = Sy / c4;
+        uz = Sz / c4;
+    } else {
+        ux = (y * Sz - z * Sy) / c4;
+        uy = (z * Sx - x * Sz) / c4;
+        uz = (x * Sy - y * Sx) / c4;
+    }
+}
+
+// Weight of an active panel's source record: zeta A, or zeta itself when the
+// reference-order pass multiplies by the area term by term.
+__device__ __forceinline__ double source_weight(const StepState &st, double zeta, int i)
+{
+    return st.unweighted ? zeta : zeta * st.area[i];
+}
+
+// ---------------------------------------------------------------------------
 // float64 pair body
 // ---------------------------------------------------------------------------
 
@@ -639,7 +725,7 @@ __device__ __forceinline__ void advection_step_body(const StepState &st, int i, 
         st.x0[c * stride + i] = q[c];
         st.xs[c * stride + i] = q[c];
     }
-    if (i < st.n_active) st.src[i] = Source{q[0], q[1], q[2], st.zeta0[i] * st.area[i]};
+    if (i < st.n_active) st.src[i] = Source{q[0], q[1], q[2], source_weight(st, st.zeta0[i], i)};
 }
 
 // Sum of the partials of target i of a pass with plan p, in a fixed order
@@ -676,6 +762,8 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ parti
 __device__ __forceinline__ void stage_update_body(const StepState &st, const StageParams &sp,
                                                   int blk, double *sh)
 {
+    // every operation rounded on its own, as the reference's BVERK4 is
+#pragma clang fp contract(off)
     const int i = blk * 256 + threadIdx.x;
     const int stride = st.t_stride;
     double ke_local = 0.0;
@@ -683,11 +771,9 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
         double Sx, Sy, Sz;
         reduce_partials(st.partial, sp.plan, stride, i, Sx, Sy, Sz);
         const double x = st.xs[i], y = st.xs[stride + i], z = st.xs[2 * stride + i];
-        const double c4 = -4.0 * kPi;
         // u = sum_j w_j (x_i x x_j)/d_ij / (-4 PI)    (PanelVelocityParallel.f90:1332)
-        const double ux = (y * Sz - z * Sy) / c4;
-        const double uy = (z * Sx - x * Sz) / c4;
-        const double uz = (x * Sy - y * Sx) / c4;
+        double ux, uy, uz;
+        velocity_from_sums(sp.plan.exact, x, y, z, Sx, Sy, Sz, ux, uy, uz);
         const double kx = sp.dt * ux, ky = sp.dt * uy, kz = sp.dt * uz;
         const double *base = (sp.stage == 0) ? st.x0 : st.acc;
         // x + k1/6 + k2/3 + k3/3 + k4/6, left to right (:637-644)
@@ -725,7 +811,7 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
             if (active) {
                 if (forced)   // stage inputs' vorticity (:744-747, :792-795, :837-840)
                     zeta_next = forced_relvort(st.panel_absvort[i], sp.omega, nx, ny, nz, sp.t_next);
-                st.src[i] = Source{nx, ny, nz, zeta_next * st.area[i]};
+                st.src[i] = Source{nx, ny, nz, source_weight(st, zeta_next, i)};
             }
         } else {
             double nx = ax, ny = ay, nz = az;
@@ -745,7 +831,7 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
                 if (forced)   // :906-911
                     zacc = forced_relvort(st.panel_absvort[i], sp.omega, nx, ny, nz, sp.t_next);
                 st.zeta0[i] = zacc;
-                st.src[i] = Source{nx, ny, nz, zacc * st.area[i]};
+                st.src[i] = Source{nx, ny, nz, source_weight(st, zacc, i)};
             }
             // particles carry absolute vorticity: relVort = absVort - 2 Omega z (:651),
             // minus the forcing in a forced step (:912-915)
@@ -763,6 +849,18 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
     }
 }
 
+// totalKE of the reference-order mode: 1/2 sum_j ke_j A_j added one panel
+// after the other, as the reference's sum intrinsic does (:498); one lane.
+__device__ __forceinline__ void ke_total_sequential_body(const double *__restrict__ ke,
+                                                         const double *__restrict__ area, int n,
+                                                         double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    double t = 0.0;
+    for (int j = 0; j < n; ++j) t = t + ke[j] * area[j];
+    *out = 0.5 * t;
+}
+
 // totalKE = 1/2 sum_j ke_j A_j (:498), the n block partials added in order.
 __device__ __forceinline__ void ke_total_body(const double *__restrict__ part, int n,
                                               double *__restrict__ out, double *sh)
@@ -777,6 +875,7 @@ __device__ __forceinline__ void ke_total_body(const double *__restrict__ part, i
 // source records {x, zeta A} of the active panels.
 __device__ __forceinline__ void prepare_step_body(const StepState &st, int blk)
 {
+#pragma clang fp contract(off)
     const int i = blk * 256 + threadIdx.x;
     if (i >= st.n_targets) return;
     const int stride = st.t_stride;
@@ -789,7 +888,7 @@ __device__ __forceinline__ void prepare_step_body(const StepState &st, int blk)
         const double w = st.panel_absvort
                              ? forced_relvort(st.panel_absvort[i], st.omega, x, y, z, st.t_step)
                              : st.zeta0[i];
-        st.src[i] = Source{x, y, z, w * st.area[i]};
+        st.src[i] = Source{x, y, z, source_weight(st, w, i)};
     }
 }
 

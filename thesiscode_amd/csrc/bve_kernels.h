@@ -19,6 +19,10 @@ struct PairArgs {
     PairPlan plan;
     double *partial;
     unsigned *queue;
+    // reference-order pass: the sources' areas (their records then carry the
+    // vorticity itself in w) and the counts of real targets and sources
+    const double *area;
+    int n_targets, n_sources;
 };
 
 // One mixed-precision pair-sum pass: the float64 pass's arguments plus the
@@ -86,6 +90,9 @@ struct StepState {
     const double *panel_absvort;
     double t_step;    // time at the start of the step
     double omega;
+    // 1: source records carry the vorticity itself (w = zeta; the reference-
+    // order pass multiplies by the area term by term), 0: w = zeta * A
+    int unweighted;
 };
 
 struct StageParams {
@@ -105,6 +112,8 @@ int pair_f64_blocks_per_cu(int targets_per_lane, int tile, int recip);
 hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s);
 hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s);
 hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s);
+// Reference-order pass (a.plan.exact): bitwise the reference's sums.
+hipError_t launch_pair_sum_exact(const PairArgs &a, hipStream_t s);
 hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s);
 // One AdvectionRK4 (divergent = 0) / DivergentAdvectionRK4 (1) step of every target.
 hipError_t launch_advection_step(const StepState &st, double t, double dt, int divergent,
@@ -117,7 +126,7 @@ hipError_t launch_aos_to_soa(const double *aos, int n, double *dst, int stride, 
 hipError_t launch_soa_to_aos(const double *src, int stride, int off, int n, double *aos,
                              hipStream_t s);
 hipError_t launch_make_sources(const double *xyz, const double *vort, const double *area, int n,
-                               Source *src, hipStream_t s);
+                               int unweighted, Source *src, hipStream_t s);
 hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, hipStream_t s);

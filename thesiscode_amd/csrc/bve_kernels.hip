@@ -100,6 +100,28 @@ hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s)
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// Reference-order pair pass: one target per lane, one workgroup per 256
+// targets, every source in index order
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void pair_sum_exact_kernel(PairArgs a)
+{
+    __shared__ double4 lds_src[kBlock];
+    __shared__ double lds_area[kBlock];
+    pair_exact_body(a, lds_src, lds_area);
+}
+
+hipError_t launch_pair_sum_exact(const PairArgs &a, hipStream_t s)
+{
+    if (!a.plan.exact || !a.area || a.plan.n_chunks != 1 || a.n_targets > a.t_stride)
+        return hipErrorInvalidValue;
+    if (a.n_targets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_sum_exact_kernel, dim3(ceil_div(a.n_targets, kBlock)), dim3(kBlock),
+                       0, s, a);
+    return hipGetLastError();
+}
+
 bool plan_supported(int targets_per_lane, int tile)
 {
     return (targets_per_lane == 1 || targets_per_lane == 2 || targets_per_lane == 4) &&
@@ -139,13 +161,25 @@ __global__ __launch_bounds__(256) void ke_total_kernel(const double *__restrict_
     ke_total_body(part, n, out, sh);
 }
 
+__global__ void ke_total_sequential_kernel(const double *__restrict__ ke,
+                                           const double *__restrict__ area, int n,
+                                           double *__restrict__ out)
+{
+    if (threadIdx.x == 0) ke_total_sequential_body(ke, area, n, out);
+}
+
 hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s)
 {
     const int blocks = ceil_div(st.n_targets, 256);
     hipLaunchKernelGGL(stage_update_kernel, dim3(blocks), dim3(256), 0, s, st, sp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || sp.stage != 0) return e;
-    hipLaunchKernelGGL(ke_total_kernel, dim3(1), dim3(256), 0, s, st.ke_part, blocks, st.ke_total);
+    if (sp.plan.exact)
+        hipLaunchKernelGGL(ke_total_sequential_kernel, dim3(1), dim3(64), 0, s, st.ke, st.area,
+                           st.n_active, st.ke_total);
+    else
+        hipLaunchKernelGGL(ke_total_kernel, dim3(1), dim3(256), 0, s, st.ke_part, blocks,
+                           st.ke_total);
     return hipGetLastError();
 }
 
@@ -205,22 +239,24 @@ hipError_t launch_soa_to_aos(const double *src, int stride, int off, int n, doub
     return hipGetLastError();
 }
 
-// Source records from AoS positions, vorticity and area: w = vort * area.
+// Source records from AoS positions, vorticity and area: w = vort * area
+// (unweighted: w = vort, for the reference-order pass).
 __global__ void make_sources_kernel(const double *__restrict__ xyz, const double *__restrict__ vort,
-                                    const double *__restrict__ area, int n,
+                                    const double *__restrict__ area, int n, int unweighted,
                                     Source *__restrict__ src)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    src[i] = Source{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], vort[i] * area[i]};
+    src[i] = Source{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
+                    unweighted ? vort[i] : vort[i] * area[i]};
 }
 
 hipError_t launch_make_sources(const double *xyz, const double *vort, const double *area, int n,
-                               Source *src, hipStream_t s)
+                               int unweighted, Source *src, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(make_sources_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, xyz, vort,
-                       area, n, src);
+                       area, n, unweighted, src);
     return hipGetLastError();
 }
 
@@ -231,15 +267,16 @@ __global__ void finalize_velocity_kernel(const double *__restrict__ partial, Pai
                                          double omega, double *__restrict__ u,
                                          double *__restrict__ dvort, int n_dvort)
 {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double Sx, Sy, Sz;
     reduce_partials(partial, plan, stride, i, Sx, Sy, Sz);
     const double x = t[i], y = t[stride + i], z = t[2 * stride + i];
-    const double c4 = -4.0 * kPi;
-    const double uz = (x * Sy - y * Sx) / c4;
-    u[3 * i] = (y * Sz - z * Sy) / c4;
-    u[3 * i + 1] = (z * Sx - x * Sz) / c4;
+    double ux, uy, uz;
+    velocity_from_sums(plan.exact, x, y, z, Sx, Sy, Sz, ux, uy, uz);
+    u[3 * i] = ux;
+    u[3 * i + 1] = uy;
     u[3 * i + 2] = uz;
     if (dvort && i < n_dvort) dvort[i] = -(2.0 * omega * uz);
 }

@@ -18,6 +18,10 @@
 ! DivergentAdvectionRK4 (:922-1279, analytic winds) run on the same cached
 ! panels.
 ! procRank and numProcs are accepted; one GPU computes every target.
+! The arithmetic is float64 (within 1e-12 of the reference); the environment
+! variable BVE_PRECISION selects another mode, and BVE_PRECISION=reference
+! reproduces the reference's BVERK4 bit for bit (the sums in the reference's
+! order and rounding), so runs through host remeshing follow it exactly.
 module PanelVelocityParallelModule
 
 use iso_c_binding
@@ -99,6 +103,10 @@ subroutine InitializeMPIRK4(aParticles, aPanels, procRank, numProcs)
 				trim(bve_error_message(c_null_ptr)))
 			error stop 1
 		endif
+		! float64 unless BVE_PRECISION asks otherwise (reference: the
+		! reference's rounding bit for bit)
+		call check(bve_set_precision(ctx, bve_precision_from_env(BVE_PRECISION_FP64), &
+			0.02_c_double), 'InitializeMPIRK4 (BVE_PRECISION)')
 	endif
 	mpiIsReady = .True.
 	call LoadBalance(aParticles, aPanels, numProcs)

@@ -7,6 +7,7 @@ module BVEInterfaceModule
 	implicit none
 	private
 	public :: BVE_OK, BVE_PRECISION_FP64, BVE_PRECISION_MIXED, BVE_PRECISION_FP64_FAST
+	public :: BVE_PRECISION_MIXED_KAHAN, BVE_PRECISION_REFERENCE, bve_precision_from_env
 	public :: bve_create, bve_destroy, bve_set_omega, bve_set_smooth, bve_set_precision
 	public :: bve_set_renormalize, bve_load_panels, bve_rk4_step, bve_store_panels, bve_reset
 	public :: bve_load_panel_absvort, bve_stratosphere_rk4_step, bve_advection_rk4_step
@@ -15,7 +16,8 @@ module BVEInterfaceModule
 
 	integer(c_int), parameter :: BVE_OK = 0
 	integer(c_int), parameter :: BVE_PRECISION_FP64 = 0, BVE_PRECISION_MIXED = 1, &
-	                             BVE_PRECISION_FP64_FAST = 2
+	                             BVE_PRECISION_FP64_FAST = 2, BVE_PRECISION_MIXED_KAHAN = 3, &
+	                             BVE_PRECISION_REFERENCE = 4
 
 	interface
 		integer(c_int) function bve_create(ctx, device) bind(C, name="bve_create")
@@ -187,6 +189,35 @@ contains
 			if (chars(k) == c_null_char) exit
 			msg(k:k) = chars(k)
 		enddo
+	end function
+
+	! Arithmetic the drop-in modules run: the environment variable
+	! BVE_PRECISION (fp64, fp64_fast, mixed, mixed_kahan or reference) when it
+	! is set, else `default`.  An unknown value is an error (-1).
+	function bve_precision_from_env(default) result(precision)
+		integer(c_int), intent(in) :: default
+		integer(c_int) :: precision
+		character(len=32) :: v
+		integer :: n, stat
+		call get_environment_variable('BVE_PRECISION', v, n, stat)
+		if (stat /= 0 .or. n == 0) then
+			precision = default
+			return
+		endif
+		select case (trim(v))
+		case ('fp64')
+			precision = BVE_PRECISION_FP64
+		case ('fp64_fast')
+			precision = BVE_PRECISION_FP64_FAST
+		case ('mixed')
+			precision = BVE_PRECISION_MIXED
+		case ('mixed_kahan')
+			precision = BVE_PRECISION_MIXED_KAHAN
+		case ('reference')
+			precision = BVE_PRECISION_REFERENCE
+		case default
+			precision = -1
+		end select
 	end function
 
 end module BVEInterfaceModule

@@ -233,10 +233,13 @@ class VoronoiVelocity:
             area = self._saved_area
         else:
             area = panels.area
-        x, y, z, rv, tot, ke = self._ctx.voronoi_rk4_step(dt, panels.x, panels.y, panels.z,
-                                                          panels.relVort, area)
+        x, y, z, rv, _, ke = self._ctx.voronoi_rk4_step(dt, panels.x, panels.y, panels.z,
+                                                        panels.relVort, area)
+        # the integral weighs |u|^2 with the current areas, also when the sum
+        # uses the saved ones (VoronoiVelocityParallel.f90:358-361)
+        self.totalKE = 0.5 * float(np.sum(ke * panels.area))
         panels.x[:], panels.y[:], panels.z[:], panels.relVort[:] = x, y, z, rv
-        self.totalKE, self.kineticEnergy = tot, ke
+        self.kineticEnergy = ke
         if retriangulate is not None:
             retriangulate(panels)
 

@@ -39,13 +39,15 @@ FLOPS_PER_PAIR = 20.0
 PEAK_FP64_TFLOPS = 78.6      # MI355X float64 vector
 ERR_BAR = 1e-10
 METRIC = "Biot-Savart pair-interactions/s per RK4 step at <=1e-10 rel err; adjoint/fwd ratio"
-MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3}
+MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4}
 
 
 def kernel_name(mode, T, tile, recip, segments):
     """Name of the pair kernel instantiation a mode's launch runs."""
     if mode.startswith("mixed"):
         return f"pair_sum_mixed_kernel<{T},{str(mode == 'mixed_kahan').lower()}>"
+    if mode == "reference":
+        return "pair_sum_exact_kernel"
     kind = "pair_sum_f64_seg_kernel" if segments > 0 else "pair_sum_f64_kernel"
     return f"{kind}<{T},{tile},{recip}>"
 
@@ -57,7 +59,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--level", type=int, default=7)
     p.add_argument("--dt", type=float, default=0.005)
-    p.add_argument("--modes", type=str, default="fp64,fp64_fast,mixed")
+    p.add_argument("--modes", type=str, default="fp64,fp64_fast,mixed,reference")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target duration of the CPU-baseline sample (0 = skip)")
     p.add_argument("--plan", type=str, default="",
@@ -108,7 +110,7 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
     from thesiscode_amd import _lib
     from thesiscode_amd import grid as gridmod
     from thesiscode_amd import initial_conditions as ic
-    from thesiscode_amd.adjoint import GaussianWeightFunctional, gradient
+    from thesiscode_amd.adjoint import GaussianWeightFunctional, central_difference, gradient
 
     g = gridmod.cubed_sphere(level)
     z = ic.rh4_wave(g.active_x)
@@ -146,15 +148,26 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
         xT, zT = ctx.adjoint_forward(dt, steps, steps)
         return math.fsum((g.active_area * zT * F.weight(xT)).tolist())
 
-    fd_report = {}
+    def report(fd):
+        rel = np.abs(gz[idx] - fd) / np.abs(fd)
+        return dict(max_rel_err=float(rel.max()), median_rel_err=float(np.median(rel)),
+                    normwise_rel_err=float(np.abs(gz[idx] - fd).max() / np.abs(fd).max()))
+
+    # central differences at eps = 1e-6 with each J difference propagated in
+    # difference form (bve_difference_forward): no cancellation
+    load(z, particles=False)
+    t0 = time.perf_counter()
+    fd_exact = np.array([central_difference(ctx, F, g.active_area, dt, steps, j, 1e-6)
+                         for j in idx])
+    t_exact = time.perf_counter() - t0
+    fd_report = {"eps_1e-06_difference_form": dict(report(fd_exact),
+                                                   seconds=t_exact)}
+    # plain quotients of two forward runs, for comparison: at eps = 1e-6 the
+    # rounding noise of J (~1e-14 relative) dominates them
     for eps in (1e-6, 1e-3):
         fd = np.array([(Jof(z + eps * (np.arange(g.n_active) == j)) -
                         Jof(z - eps * (np.arange(g.n_active) == j))) / (2 * eps) for j in idx])
-        rel = np.abs(gz[idx] - fd) / np.abs(fd)
-        fd_report[f"eps_{eps:g}"] = dict(max_rel_err=float(rel.max()),
-                                         median_rel_err=float(np.median(rel)),
-                                         normwise_rel_err=float(np.abs(gz[idx] - fd).max() /
-                                                                np.abs(fd).max()))
+        fd_report[f"eps_{eps:g}_plain"] = report(fd)
     ctx.close()
     return dict(workload=f"rh4_cubed_sphere_L{level}_{steps}steps_dt{dt:g}_checkpoint{every}",
                 mode=mode, n_active=g.n_active, J=J, forward_ms=t_fwd * 1e3,

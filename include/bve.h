@@ -216,6 +216,19 @@ int bve_adjoint_forward(bve_ctx *ctx, double dt, int n_steps, int checkpoint_eve
 int bve_adjoint_backward(bve_ctx *ctx, const double *lambda_x, const double *lambda_zeta,
                          double *grad_x0, double *grad_zeta0);
 
+/* No reference counterpart.  Exact finite differences of the same active-
+ * panel RK4 map: from the resident state (x0, zeta0) and a vorticity
+ * perturbation dzeta0 (n_active), advances both the trajectory of
+ * (x0, zeta0) and the difference delta = trajectory(x0, zeta0 + dzeta0) -
+ * trajectory(x0, zeta0) for n_steps steps, the difference in difference form
+ * (every term built from differences, so it carries rounding relative to
+ * itself, not to the state: no cancellation at small perturbations).
+ * Returns x_T (3*n_active, may be NULL), zeta_T (may be NULL), dx_T and
+ * dzeta_T.  Float64; the resident state is left as it is.  Used to check the
+ * adjoint against central differences at small epsilon. */
+int bve_difference_forward(bve_ctx *ctx, double dt, int n_steps, const double *dzeta0,
+                           double *x_T, double *zeta_T, double *dx_T, double *dzeta_T);
+
 /* ---- ensembles (many independent problems, shared launches) ----------- */
 
 /* No reference counterpart (the reference runs one problem per MPI job).

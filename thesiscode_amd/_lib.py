@@ -46,6 +46,7 @@ SIGNATURES = {
     "bve_synchronize": (_i, [_p]),
     "bve_velocity": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p]),
     "bve_voronoi_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p, _p, _p]),
+    "bve_difference_forward": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
     "bve_adjoint_forward": (_i, [_p, _d, _i, _i, _p, _p]),
     "bve_adjoint_backward": (_i, [_p, _p, _p, _p, _p]),
     "bve_ensemble_create": (_i, [ctypes.POINTER(_p), _i, _i]),
@@ -297,6 +298,19 @@ class Context:
         self._check(self._L.bve_voronoi_rk4_step(self._h, dt, n, *(_ptr(a) for a in xs), _ptr(ar),
                                                  _ptr(tot), _ptr(ke)), "bve_voronoi_rk4_step")
         return xs[0], xs[1], xs[2], xs[3], float(tot[0]), ke
+
+    # exact differences
+    def difference_forward(self, dt, n_steps, dzeta0):
+        """Base trajectory and its exact difference to the trajectory with
+        vorticity zeta0 + dzeta0 (active panels); returns (x_T, zeta_T, dx_T,
+        dzeta_T)."""
+        n = self.n_active
+        dz0 = _f64(dzeta0, n)
+        xT, zT, dxT, dzT = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 3)), np.zeros(n)
+        self._check(self._L.bve_difference_forward(self._h, dt, n_steps, _ptr(dz0), _ptr(xT),
+                                                   _ptr(zT), _ptr(dxT), _ptr(dzT)),
+                    "bve_difference_forward")
+        return xT, zT, dxT, dzT
 
     # adjoint
     def adjoint_forward(self, dt, n_steps, checkpoint_every):

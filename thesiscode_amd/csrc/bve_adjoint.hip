@@ -342,4 +342,150 @@ hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Exact differences of the forward map
+// ---------------------------------------------------------------------------
+//
+// For the trajectory of (x, zeta) and that of (x', zeta') = (x + dx, zeta +
+// dzeta) the velocity sums differ by (exact algebra, no linearisation)
+//
+//   d'_ik - d_ik = Dd = -(dx_i . x'_k + x_i . dx_k),   d = 1 - x_i . x_k
+//   w'/d' - w/d  = Dq = (Dw - q Dd) / d',               q = w / d
+//   S'_i - S_i   = DS = sum_k (Dq x'_k + q dx_k)
+//   U'_i - U_i   = (dx_i x S'_i + x_i x DS_i) / (-4 PI)
+//
+// Every quantity is formed from differences, never as the difference of two
+// nearly equal numbers, so the propagated difference carries rounding
+// relative to itself: (J(zeta + e) - J(zeta - e)) / 2e from it is free of the
+// cancellation a plain finite difference suffers at small e.
+
+// One (target tile, source chunk) item: base sums S and their differences.
+__global__ __launch_bounds__(kBlock) void diff_pair_kernel(DiffState st, PairPlan plan)
+{
+    const int work = xcd_item(blockIdx.x, plan.grid);
+    if (work >= plan.n_work) return;
+    const int chunk = work / plan.n_tiles;
+    const int i = (work - chunk * plan.n_tiles) * kBlock + threadIdx.x;
+    __shared__ DiffRec lds[kBlock];
+    const DiffRec t = st.rec[i];   // records are zero beyond n, up to the row stride
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0, D0 = 0.0, D1 = 0.0, D2 = 0.0;
+    const int k0 = chunk * plan.chunk, k1 = min(k0 + plan.chunk, st.n);
+    for (int jt = k0; jt < k1; jt += kBlock) {
+        __syncthreads();
+        if (jt + (int)threadIdx.x < k1) lds[threadIdx.x] = st.rec[jt + threadIdx.x];
+        __syncthreads();
+        const int m = min(kBlock, k1 - jt);
+        for (int kk = 0; kk < m; ++kk) {
+            if (jt + kk == i) continue;   // self-exclusion (BVEActiveRHS)
+            const DiffRec q = lds[kk];
+            const double px = q.x + q.dx, py = q.y + q.dy, pz = q.z + q.dz;
+            const double d = 1.0 - (t.x * q.x + t.y * q.y + t.z * q.z);
+            const double dd = -((t.dx * px + t.dy * py + t.dz * pz) +
+                                (t.x * q.dx + t.y * q.dy + t.z * q.dz));
+            const double r = q.w / d;
+            const double dr = (q.dw - r * dd) / (d + dd);
+            S0 += r * q.x;
+            S1 += r * q.y;
+            S2 += r * q.z;
+            D0 += dr * px + r * q.dx;
+            D1 += dr * py + r * q.dy;
+            D2 += dr * pz + r * q.dz;
+        }
+    }
+    if (i < st.n) {
+        double *p = st.partial + (size_t)6 * chunk * st.stride + i;
+        p[0] = S0;
+        p[st.stride] = S1;
+        p[2 * (size_t)st.stride] = S2;
+        p[3 * (size_t)st.stride] = D0;
+        p[4 * (size_t)st.stride] = D1;
+        p[5 * (size_t)st.stride] = D2;
+    }
+}
+
+// Stage-1 records from the step-start state.
+__global__ void diff_prepare_kernel(DiffState st)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= st.n) return;
+    const size_t R = st.stride;
+    st.rec[i] = DiffRec{st.x0[i], st.x0[R + i], st.x0[2 * R + i], st.dx0[i], st.dx0[R + i],
+                        st.dx0[2 * R + i], st.z0[i] * st.area[i], st.dz0[i] * st.area[i]};
+}
+
+// RK4 stage of both trajectories (the same steps as the forward stage body:
+// k = dt U, dzeta/dt = -2 Omega U_z, inputs x + a k, update x + sum k_s / c_s).
+__global__ void diff_stage_kernel(DiffState st, PairPlan plan, int stage, double dt,
+                                  double omega, double a_next, double divisor)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= st.n) return;
+    const size_t R = st.stride;
+    double S[3] = {0.0, 0.0, 0.0}, D[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < plan.n_chunks; ++c) {
+        const double *p = st.partial + (size_t)6 * c * R + i;
+        for (int r = 0; r < 3; ++r) {
+            S[r] += p[r * R];
+            D[r] += p[(3 + r) * R];
+        }
+    }
+    const DiffRec t = st.rec[i];
+    const double c4 = -4.0 * kPi;
+    const double Sp[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
+    const double u[3] = {(t.y * S[2] - t.z * S[1]) / c4, (t.z * S[0] - t.x * S[2]) / c4,
+                         (t.x * S[1] - t.y * S[0]) / c4};
+    const double du[3] = {((t.dy * Sp[2] - t.dz * Sp[1]) + (t.y * D[2] - t.z * D[1])) / c4,
+                          ((t.dz * Sp[0] - t.dx * Sp[2]) + (t.z * D[0] - t.x * D[2])) / c4,
+                          ((t.dx * Sp[1] - t.dy * Sp[0]) + (t.x * D[1] - t.y * D[0])) / c4};
+    double nb[4], nd[4];
+    for (int r = 0; r < 4; ++r) {
+        const double k = r < 3 ? dt * u[r] : dt * (-(2.0 * omega * u[2]));
+        const double dk = r < 3 ? dt * du[r] : dt * (-(2.0 * omega * du[2]));
+        double *acc = r < 3 ? st.acc + r * R : st.zacc;
+        double *dacc = r < 3 ? st.dacc + r * R : st.dzacc;
+        const double *b0 = r < 3 ? st.x0 + r * R : st.z0;
+        const double *d0 = r < 3 ? st.dx0 + r * R : st.dz0;
+        const double a = (stage == 0 ? b0[i] : acc[i]) + k / divisor;
+        const double da = (stage == 0 ? d0[i] : dacc[i]) + dk / divisor;
+        acc[i] = a;
+        dacc[i] = da;
+        nb[r] = stage < 3 ? b0[i] + a_next * k : a;
+        nd[r] = stage < 3 ? d0[i] + a_next * dk : da;
+    }
+    if (stage == 3) {
+        for (int r = 0; r < 3; ++r) {
+            st.x0[r * R + i] = nb[r];
+            st.dx0[r * R + i] = nd[r];
+        }
+        st.z0[i] = nb[3];
+        st.dz0[i] = nd[3];
+    }
+    st.rec[i] = DiffRec{nb[0], nb[1], nb[2], nd[0], nd[1], nd[2], nb[3] * st.area[i],
+                        nd[3] * st.area[i]};
+}
+
+hipError_t launch_diff_prepare(const DiffState &st, hipStream_t s)
+{
+    hipLaunchKernelGGL(diff_prepare_kernel, dim3(ceil_div(st.n, 256)), dim3(256), 0, s, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff_pair(const DiffState &st, const PairPlan &plan, hipStream_t s)
+{
+    if (plan.targets_per_lane != 1 || plan.n_tiles * kBlock > st.stride || plan.grid % 8 != 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(diff_pair_kernel, dim3(plan.grid), dim3(kBlock), 0, s, st, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff_stage(const DiffState &st, const PairPlan &plan, int stage, double dt,
+                             double omega, hipStream_t s)
+{
+    static const double a_next[4] = {0.5, 0.5, 1.0, 0.0};
+    static const double divisor[4] = {6.0, 3.0, 3.0, 6.0};
+    hipLaunchKernelGGL(diff_stage_kernel, dim3(ceil_div(st.n, 256)), dim3(256), 0, s, st, plan,
+                       stage, dt, omega, a_next[stage], divisor[stage]);
+    return hipGetLastError();
+}
+
 }  // namespace bve

@@ -1313,6 +1313,86 @@ int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every
     return BVE_OK;
 }
 
+int bve_difference_forward(bve_ctx *c, double dt, int n_steps, const double *dzeta0,
+                           double *x_T, double *zeta_T, double *dx_T, double *dzeta_T)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded)
+        return c->fail(BVE_ERR_STATE, "bve_difference_forward: no panels loaded");
+    if (n_steps < 0 || !dzeta0 || !dx_T || !dzeta_T)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_difference_forward: bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int n = c->n_active;
+    const size_t R = (size_t)round_up(n, kTargetAlign);
+    PairPlan plan = make_plan(n, n, 1, kBlock, 4096, 1);
+    DiffState st{};
+    st.n = n;
+    st.stride = (int)R;
+    st.area = c->area;
+    double *rows = nullptr;
+    DiffRec *rec = nullptr;
+    double *partial = nullptr;
+    int rc = BVE_OK;
+    hipError_t e = hipSuccess;
+    auto cleanup = [&]() {
+        dev_free(rows);
+        dev_free(rec);
+        dev_free(partial);
+    };
+#define DTRY(call)                                       \
+    do {                                                 \
+        e = (call);                                      \
+        if (e != hipSuccess) {                           \
+            rc = c->hip_fail(e, #call);                  \
+            (void)hipStreamSynchronize(c->stream);       \
+            cleanup();                                   \
+            return rc;                                   \
+        }                                                \
+    } while (0)
+    DTRY(dev_alloc(&rows, 16 * R));
+    DTRY(dev_alloc(&rec, R));
+    DTRY(dev_alloc(&partial, (size_t)plan.n_chunks * 6 * R));
+    DTRY(hipMemsetAsync(rows, 0, sizeof(double) * 16 * R, c->stream));
+    DTRY(hipMemsetAsync(rec, 0, sizeof(DiffRec) * R, c->stream));
+    st.x0 = rows;
+    st.z0 = rows + 3 * R;
+    st.acc = rows + 4 * R;
+    st.zacc = rows + 7 * R;
+    st.dx0 = rows + 8 * R;
+    st.dz0 = rows + 11 * R;
+    st.dacc = rows + 12 * R;
+    st.dzacc = rows + 15 * R;
+    st.rec = rec;
+    st.partial = partial;
+    // the resident active state is the base; it is left as it is
+    for (int r = 0; r < 3; ++r)
+        DTRY(hipMemcpyAsync(st.x0 + r * R, c->x0 + (size_t)r * c->t_stride, sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, c->stream));
+    DTRY(hipMemcpyAsync(st.z0, c->zeta0, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    DTRY(hipMemcpyAsync(st.dz0, dzeta0, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    DTRY(launch_diff_prepare(st, c->stream));
+    for (int k = 0; k < n_steps; ++k)
+        for (int s = 0; s < 4; ++s) {
+            DTRY(launch_diff_pair(st, plan, c->stream));
+            DTRY(launch_diff_stage(st, plan, s, dt, c->omega, c->stream));
+        }
+    std::vector<double> h(16 * R);
+    DTRY(hipMemcpyAsync(h.data(), rows, sizeof(double) * 16 * R, hipMemcpyDeviceToHost, c->stream));
+    DTRY(hipStreamSynchronize(c->stream));
+#undef DTRY
+    cleanup();
+    for (int i = 0; i < n; ++i)
+        for (int r = 0; r < 3; ++r) {
+            if (x_T) x_T[3 * (size_t)i + r] = h[r * R + i];
+            dx_T[3 * (size_t)i + r] = h[(8 + r) * R + i];
+        }
+    for (int i = 0; i < n; ++i) {
+        if (zeta_T) zeta_T[i] = h[3 * R + i];
+        dzeta_T[i] = h[11 * R + i];
+    }
+    return BVE_OK;
+}
+
 int bve_adjoint_backward(bve_ctx *c, const double *lambda_x, const double *lambda_zeta,
                          double *grad_x0, double *grad_zeta0)
 {

@@ -132,6 +132,27 @@ hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan,
                                     int n_dvort, hipStream_t s);
 hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_t s);
 
+// Exact-difference run of the active panels (bve_difference_forward): the
+// trajectory of (x, zeta) and the difference to the trajectory of
+// (x + dx, zeta + dzeta), both advanced by the same RK4 map.
+struct __align__(16) DiffRec {
+    double x, y, z;      // stage input of the base trajectory
+    double dx, dy, dz;   // its difference to the perturbed trajectory's
+    double w, dw;        // zeta A and delta zeta A of the stage input
+};
+struct DiffState {
+    int n, stride;                     // panels; row stride (multiple of 1024)
+    double *x0, *z0, *acc, *zacc;      // base: step start (3 + 1 rows), RK4 accumulators
+    double *dx0, *dz0, *dacc, *dzacc;  // the same for the difference
+    const double *area;
+    DiffRec *rec;                      // stage inputs, zero beyond n
+    double *partial;                   // per source chunk: S (3 rows), delta S (3 rows)
+};
+hipError_t launch_diff_prepare(const DiffState &st, hipStream_t s);
+hipError_t launch_diff_pair(const DiffState &st, const PairPlan &plan, hipStream_t s);
+hipError_t launch_diff_stage(const DiffState &st, const PairPlan &plan, int stage, double dt,
+                             double omega, hipStream_t s);
+
 // Ensemble: one work item of the persistent pair launch.
 struct EnsItem {
     int member, chunk, ttile;

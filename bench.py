@@ -21,8 +21,11 @@ a level-8 two-vortex single step in every arithmetic mode.
 Prints one JSON line.  The roofline block uses HIP events recorded by the
 library around every pair-kernel launch (one launch per RK4 stage) at 20
 flops per pair against the MI355X float64 vector peak.  The CPU baseline
-times the oracle (the reference's loop restated in C, OpenMP over targets)
-on a bounded sample of the same sum.
+times the reference's own BVERK4 (compiled unmodified into
+oracle/_ref/ref_bench, MPI over 16 host cores) on three level-6 steps;
+the oracle (the reference's loop restated in C, OpenMP over targets) is
+timed beside it (config.cpu_port) and stands in when the reference build is
+absent.
 """
 import argparse
 import json
@@ -97,6 +100,41 @@ def cpu_baseline(grid, zeta, seconds):
                        f"BVESmoothVelocity (reference rounding order, IEEE divisions) at level "
                        f"{grid.level}: {stages * pairs_stage:.4g} pairs in {el:.1f} s, "
                        f"{threads} OpenMP threads")
+
+
+def cpu_baseline_reference(level=6, steps=3, ranks=16):
+    """The reference's own BVERK4 (PanelVelocityParallel.f90, compiled
+    unmodified with flang -O2 into oracle/_ref/ref_bench) timed with
+    MPI_Wtime on `ranks` MPI ranks of the host (its native parallelisation:
+    contiguous target ranges per rank, MPI_BCAST of every stage).  Returns
+    None when the binary or mpiexec is missing."""
+    import shutil
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    ranks = max(1, min(ranks, os.cpu_count() or 1))
+    try:
+        r = subprocess.run([mpiexec, "-n", str(ranks), exe, str(level), str(steps)],
+                           capture_output=True, text=True, timeout=180)
+    except subprocess.TimeoutExpired:
+        return None
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("ref_bench")]
+    if r.returncode != 0 or not line:
+        return None
+    f = line[-1].split()
+    n_a, n_p, n_pp, nprocs, nsteps = (int(v) for v in f[2:7])
+    seconds, pairs = float(f[7]), float(f[8])
+    # the metric's pairs (active and vertex targets) per second of the steps
+    primary = 4.0 * nsteps * (n_a * (n_a - 1.0) + n_p * n_a)
+    return dict(value=primary / seconds, unit="pair-interactions/s", cores=nprocs,
+                kind="reference",
+                sample=f"{nsteps} BVERK4 step(s) of the reference's PanelVelocityParallelModule "
+                       f"(flang -O2, MPICH, {nprocs} ranks) on the level-{level} cubed sphere "
+                       f"({n_a} active panels, {n_p} vertices, {n_pp} parents): {pairs:.4g} "
+                       f"pairs of all classes in {seconds:.1f} s; all-class rate "
+                       f"{pairs / seconds:.4g}/s")
 
 
 def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64"):
@@ -422,7 +460,10 @@ def main():
         big = large_solve_measurement(args.large_level)
         out["config"]["large_solve"] = big
     if args.cpu_seconds > 0:
-        out["cpu_baseline"] = cpu_baseline(g, za, args.cpu_seconds)
+        port = cpu_baseline(g, za, args.cpu_seconds)
+        ref = cpu_baseline_reference()
+        out["cpu_baseline"] = ref if ref is not None else port
+        out["config"]["cpu_port"] = port
     print(json.dumps(out), flush=True)
 
 

@@ -850,15 +850,26 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
 }
 
 // totalKE of the reference-order mode: 1/2 sum_j ke_j A_j added one panel
-// after the other, as the reference's sum intrinsic does (:498); one lane.
+// after the other, as the reference's sum intrinsic does (:498).  The
+// workgroup forms 256 products ke_j A_j at a time (each rounded on its own,
+// as in the reference) into sh; lane 0 adds them in index order.
 __device__ __forceinline__ void ke_total_sequential_body(const double *__restrict__ ke,
                                                          const double *__restrict__ area, int n,
-                                                         double *__restrict__ out)
+                                                         double *__restrict__ out, double *sh)
 {
 #pragma clang fp contract(off)
     double t = 0.0;
-    for (int j = 0; j < n; ++j) t = t + ke[j] * area[j];
-    *out = 0.5 * t;
+    for (int base = 0; base < n; base += 256) {
+        const int j = base + threadIdx.x;
+        if (j < n) sh[threadIdx.x] = ke[j] * area[j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(256, n - base);
+            for (int k = 0; k < m; ++k) t = t + sh[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = 0.5 * t;
 }
 
 // totalKE = 1/2 sum_j ke_j A_j (:498), the n block partials added in order.

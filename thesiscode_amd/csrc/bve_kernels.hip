@@ -161,11 +161,12 @@ __global__ __launch_bounds__(256) void ke_total_kernel(const double *__restrict_
     ke_total_body(part, n, out, sh);
 }
 
-__global__ void ke_total_sequential_kernel(const double *__restrict__ ke,
-                                           const double *__restrict__ area, int n,
-                                           double *__restrict__ out)
+__global__ __launch_bounds__(256) void ke_total_sequential_kernel(const double *__restrict__ ke,
+                                                                  const double *__restrict__ area,
+                                                                  int n, double *__restrict__ out)
 {
-    if (threadIdx.x == 0) ke_total_sequential_body(ke, area, n, out);
+    __shared__ double sh[256];
+    ke_total_sequential_body(ke, area, n, out, sh);
 }
 
 hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s)
@@ -175,7 +176,7 @@ hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipSt
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || sp.stage != 0) return e;
     if (sp.plan.exact)
-        hipLaunchKernelGGL(ke_total_sequential_kernel, dim3(1), dim3(64), 0, s, st.ke, st.area,
+        hipLaunchKernelGGL(ke_total_sequential_kernel, dim3(1), dim3(256), 0, s, st.ke, st.area,
                            st.n_active, st.ke_total);
     else
         hipLaunchKernelGGL(ke_total_kernel, dim3(1), dim3(256), 0, s, st.ke_part, blocks,

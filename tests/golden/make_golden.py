@@ -7,7 +7,8 @@ compressed ``.npz`` (float64 / int32 arrays only, loadable with
 ``allow_pickle=False``) plus ``manifest.json``.
 
 It also runs the reference's own drivers (RHWavePanelsMPI.f90,
-StratospherePanels.f90, JetPanels.f90, NonDivAdvection.f90; unmodified,
+StratospherePanels.f90, JetPanels.f90, NonDivAdvection.f90, GaussVort.f90;
+unmodified,
 built by the same makefile's `drivers` target) on the committed namelists
 and keeps their .dat tables.  ``make_golden.py voronoi`` redoes only the
 Voronoi cases (make_voronoi_golden.f90 and the two Voronoi drivers).
@@ -82,6 +83,8 @@ DRIVER_CASES = {
     "jet_driver_L4_amr2": ("jet_ref", "JetPanels.namelist"),
     "jet_driver_L4_amr2_remesh5": ("jet_ref", "JetPanels.namelist"),
     "adv_driver_L3": ("adv_ref", "AdvectGaussHills.namelist"),
+    "gauss_driver_tri_L4": ("gauss_ref", "GaussVortPanels.namelist"),
+    "gauss_driver_tri_L3_remesh5": ("gauss_ref", "GaussVortPanels.namelist"),
 }
 # the reference's Voronoi Gaussian-vortex driver (GaussVortVoronoi.f90, MPI
 # module) and its serial RH4 Voronoi driver (RHWaveVoronoi.f90, serial module)

@@ -339,3 +339,32 @@ def test_reference_amr_jet_driver_bitwise_in_reference_mode(tmp_path, name):
     got = run_dropin(tmp_path, name + ".namelist", JET_DROPIN, "JetPanels.namelist",
                      precision="reference", raw=True)
     assert got == open(os.path.join(GOLDEN, name + ".dat"), "rb").read()
+
+
+GAUSS_DROPIN = os.path.join(ROOT, "oracle", "_ref", "gauss_gpu")
+
+
+@pytest.mark.skipif(not os.path.exists(GAUSS_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+def test_reference_gauss_vortex_driver_runs_on_the_dropin(tmp_path):
+    """GaussVort (the Gaussian-vortex panel driver on icosahedral triangles,
+    level 4: 5,120 active panels, 10 steps at dt = 0.01, no remesh) on the
+    GPU module reproduces the reference's .dat table -- totalKE and
+    enstrophy of every step -- to the printed 15 decimals."""
+    got = run_dropin(tmp_path, "gauss_driver_tri_L4.namelist", GAUSS_DROPIN,
+                     "GaussVortPanels.namelist", precision="fp64")
+    ref = read_dat(os.path.join(GOLDEN, "gauss_driver_tri_L4.dat"))
+    assert got.shape == ref.shape
+    assert np.allclose(got, ref, rtol=1e-12, atol=2e-15), np.abs(got - ref).max(axis=0)
+
+
+@pytest.mark.skipif(not os.path.exists(GAUSS_DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
+@pytest.mark.parametrize("name", ["gauss_driver_tri_L4", "gauss_driver_tri_L3_remesh5"])
+def test_reference_gauss_vortex_driver_bitwise_in_reference_mode(tmp_path, name):
+    """GaussVort on triangles, with and without a remesh every 5 steps, on
+    the drop-in with BVE_PRECISION=reference: the reference run's .dat file
+    byte for byte."""
+    got = run_dropin(tmp_path, name + ".namelist", GAUSS_DROPIN, "GaussVortPanels.namelist",
+                     precision="reference", raw=True)
+    assert got == open(os.path.join(GOLDEN, name + ".dat"), "rb").read()

@@ -519,3 +519,68 @@ def test_step_into_caller_arrays_matches_step(ctx):
     assert np.array_equal(qx1, qx2) and np.array_equal(rel1, rel2) and tot1 == tot2
     assert np.array_equal(ke1, ke2)
     assert all(np.array_equal(a, b) for a, b in zip(a1, (ax, av, px)))
+
+
+def random_sphere(n, seed):
+    p = np.random.default_rng(seed).normal(size=(n, 3))
+    return p / np.linalg.norm(p, axis=1)[:, None]
+
+
+@pytest.mark.parametrize("na,npp,npart", [(1, 0, 0), (7, 3, 5), (300, 0, 257), (1029, 511, 1537)])
+def test_reference_mode_equals_oracle_on_ragged_inputs(ctx, na, npp, npart):
+    """BVE_PRECISION_REFERENCE on random points with sizes that fill no tile
+    evenly (and a single panel): one RK4 step of all three target classes,
+    KE and totalKE, bit for bit the oracle's (which is bit for bit the
+    reference's on its golden cases)."""
+    rng = np.random.default_rng(na + npp + npart)
+    ax, px, qx = random_sphere(na, 1), random_sphere(npp, 2), random_sphere(npart, 3)
+    zv, area = rng.normal(size=na), rng.uniform(0.5, 1.5, na) * 4 * np.pi / max(na, 1)
+    qa = rng.normal(size=npart)
+    ctx.reset()
+    ctx.set_precision(_lib.BVE_PRECISION_REFERENCE)
+    ctx.set_omega(2 * np.pi)
+    ctx.set_smooth(0.05)
+    try:
+        ctx.load_panels(ax, zv, area, px)
+        qx1, qrel, tot, ke = ctx.rk4_step(0.01, qx, qa)
+        ax1, av1, px1 = ctx.store_panels()
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    r = oracle.rk4_step(ax, zv, area, px, qx, qa, 0.01, 2 * np.pi, 0.05)
+    for got, key in ((ax1, "active_x"), (av1, "active_relvort"), (px1, "passive_x"),
+                     (qx1, "particles_x"), (qrel, "particles_relvort"), (ke, "ke")):
+        assert np.array_equal(got, r[key]), key
+    assert tot == r["total_ke"]
+
+
+@pytest.mark.parametrize("n", [2, 97, 642])
+def test_reference_mode_voronoi_equals_oracle_on_ragged_inputs(ctx, n):
+    """The Voronoi step in the reference order on random generators: bit for
+    bit the oracle's Voronoi BVERK4 (its own denominator form, NormalizeGrid)."""
+    rng = np.random.default_rng(n)
+    x = random_sphere(n, 4 + n)
+    zv, area = rng.normal(size=n), rng.uniform(0.5, 1.5, n) * 4 * np.pi / n
+    ctx.reset()
+    ctx.set_precision(_lib.BVE_PRECISION_REFERENCE)
+    ctx.set_omega(2 * np.pi)
+    try:
+        gx, gy, gz, grv, _, ke = ctx.voronoi_rk4_step(0.01, x[:, 0], x[:, 1], x[:, 2], zv, area)
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    rx, ry, rz, rrv, rke, _ = oracle.voronoi_rk4_step(x[:, 0], x[:, 1], x[:, 2], zv, area, area,
+                                                      0.01, 2 * np.pi)
+    for got, ref in ((gx, rx), (gy, ry), (gz, rz), (grv, rrv), (ke, rke)):
+        assert np.array_equal(got, ref)
+
+
+def test_difference_forward_of_zero_is_zero(ctx):
+    """A zero perturbation propagates as exactly zero, and the base
+    trajectory equals the plain forward run's."""
+    g = gridmod.cubed_sphere(3)
+    z = ic.rh4_wave(g.active_x)
+    ctx.reset()
+    ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
+    xb, zb = ctx.adjoint_forward(0.01, 3, 3)
+    x0, z0, dx, dz = ctx.difference_forward(0.01, 3, np.zeros(g.n_active))
+    assert not dx.any() and not dz.any()
+    assert np.abs(x0 - xb).max() < 1e-13 and np.abs(z0 - zb).max() < 1e-12

@@ -194,7 +194,9 @@ int bve_velocity(bve_ctx *ctx, int n_targets, const double *x_targets, int self_
 /* BVERK4(aPanels, dt[, procRank, numProcs]) of VoronoiVelocityParallel.f90:304-483
  * on n generators in SoA form (x, y, z, relvort in/out, area in), including
  * NormalizeGrid (:480); the host re-triangulates afterwards.  total_ke, ke
- * as for the panel step (may be NULL). */
+ * as for the panel step (may be NULL).  The generators replace the
+ * context's loaded problem; its device arrays are kept for the next call
+ * with the same n. */
 int bve_voronoi_rk4_step(bve_ctx *ctx, double dt, int n, double *x, double *y, double *z,
                          double *relvort, const double *area, double *total_ke, double *ke);
 

@@ -902,17 +902,27 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
     c->downloads.clear();
     int rc = host_sync(c);   // uploads in flight into the old arrays
     if (rc) return rc;
-    free_state(c);
     free_adjoint(c);
-    c->n_active = n_active;
-    c->n_passive = n_passive;
-    c->n_src_pad = round_up(n_active, kSourceAlign);
-    HIP_TRY(c, dev_alloc(&c->zeta0, n_active));
-    HIP_TRY(c, dev_alloc(&c->zacc, n_active));
-    HIP_TRY(c, dev_alloc(&c->area, n_active));
-    HIP_TRY(c, dev_alloc(&c->ke, n_active));
-    HIP_TRY(c, dev_alloc(&c->ke_total, 1));
-    HIP_TRY(c, dev_alloc(&c->src, c->n_src_pad));
+    // Same sizes as the loaded problem (a Voronoi step, a reload after a
+    // remesh that kept the counts): the device arrays are reused, only the
+    // per-problem state is dropped; otherwise they are allocated afresh.
+    const bool reuse = c->panels_loaded && n_active == c->n_active && n_passive == c->n_passive;
+    if (reuse) {
+        dev_free(c->panel_absvort);
+        c->particles_have_absvort = false;
+        c->panels_loaded = false;
+    } else {
+        free_state(c);
+        c->n_active = n_active;
+        c->n_passive = n_passive;
+        c->n_src_pad = round_up(n_active, kSourceAlign);
+        HIP_TRY(c, dev_alloc(&c->zeta0, n_active));
+        HIP_TRY(c, dev_alloc(&c->zacc, n_active));
+        HIP_TRY(c, dev_alloc(&c->area, n_active));
+        HIP_TRY(c, dev_alloc(&c->ke, n_active));
+        HIP_TRY(c, dev_alloc(&c->ke_total, 1));
+        HIP_TRY(c, dev_alloc(&c->src, c->n_src_pad));
+    }
     HIP_TRY(c, hipMemsetAsync(c->src, 0, sizeof(Source) * c->n_src_pad, c->stream));
     rc = host_up(c, c->zeta0, relvort, sizeof(double) * n_active);
     if (!rc) rc = host_up(c, c->area, area, sizeof(double) * n_active);
@@ -1252,7 +1262,8 @@ int bve_voronoi_rk4_step(bve_ctx *c, double dt, int n, double *x, double *y, dou
         y[j] = aos[3 * (size_t)j + 1];
         z[j] = aos[3 * (size_t)j + 2];
     }
-    return bve_reset(c);
+    // the device arrays stay allocated for the next step of the same size
+    return BVE_OK;
 }
 
 int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every, double *x_T,

@@ -584,3 +584,17 @@ def test_difference_forward_of_zero_is_zero(ctx):
     x0, z0, dx, dz = ctx.difference_forward(0.01, 3, np.zeros(g.n_active))
     assert not dx.any() and not dz.any()
     assert np.abs(x0 - xb).max() < 1e-13 and np.abs(z0 - zb).max() < 1e-12
+
+
+@pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_FP64, _lib.BVE_PRECISION_FP64_FAST,
+                                       _lib.BVE_PRECISION_MIXED, _lib.BVE_PRECISION_REFERENCE])
+def test_velocity_without_sources_is_zero(ctx, precision):
+    """No sources: every mode returns zero velocity (and dVort) at any
+    targets, as the reference's empty k loops do."""
+    ctx.set_precision(precision)
+    try:
+        u, dv = ctx.velocity(random_sphere(5, 9), np.zeros((0, 3)), np.zeros(0), np.zeros(0),
+                             want_dvort=True)
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    assert u.shape == (5, 3) and not u.any() and len(dv) == 0

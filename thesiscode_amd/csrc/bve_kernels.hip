@@ -114,7 +114,8 @@ __global__ __launch_bounds__(kBlock) void pair_sum_exact_kernel(PairArgs a)
 
 hipError_t launch_pair_sum_exact(const PairArgs &a, hipStream_t s)
 {
-    if (!a.plan.exact || !a.area || a.plan.n_chunks != 1 || a.n_targets > a.t_stride)
+    if (!a.plan.exact || (!a.area && a.n_sources > 0) || a.plan.n_chunks != 1 ||
+        a.n_targets > a.t_stride)
         return hipErrorInvalidValue;
     if (a.n_targets <= 0) return hipSuccess;
     hipLaunchKernelGGL(pair_sum_exact_kernel, dim3(ceil_div(a.n_targets, kBlock)), dim3(kBlock),

@@ -192,6 +192,35 @@ def test_fast_fp64_step_matches_reference(ctx):
 
 
 @pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_MIXED, _lib.BVE_PRECISION_MIXED_KAHAN])
+def test_mixed_step_matches_reference(ctx, precision):
+    """Mixed arithmetic through the golden 12-step level-4 case.  The mixed
+    pass visits the passive (parent) panels in a spatial order of its own
+    (Hilbert order per cube face), so this also pins that every velocity
+    lands on its own panel: a mix-up would move panels by the order of the
+    displacement itself, against a float32 far-field error of ~1e-6 of it."""
+    name = "rh4_L4_12steps"
+    c = golden_manifest()[name]
+    g, s = golden_state(name)
+    ctx.reset()
+    ctx.set_precision(precision)
+    try:
+        ctx.set_omega(c["omega"])
+        ctx.set_smooth(c["smooth"])
+        ctx.load_panels(s["ax"], s["av"], s["ar"], s["px"])
+        qx = s["qx"]
+        for _ in range(c["nsteps"]):
+            qx, _, _, _ = ctx.rk4_step(c["dt"], qx, s["qa"])
+        ax, av, px = ctx.store_panels()
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    disp = np.linalg.norm(g["sn_active_x"] - s["ax"], axis=1).max()
+    for got, want in ((ax, g["sn_active_x"]), (qx, g["sn_particles_x"]), (px, g["sn_passive_x"])):
+        err = np.abs(got - want).max() / disp
+        print(f"mixed precision={precision}: max position error / displacement {err:.2e}")
+        assert err < 1e-5
+
+
+@pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_MIXED, _lib.BVE_PRECISION_MIXED_KAHAN])
 def test_mixed_with_everything_near_is_float64(ctx, precision):
     """With a near-field radius above the sphere's diameter every pair takes
     the mixed kernel's float64 path: the result must meet the float64 bar

@@ -1,7 +1,9 @@
 // C ABI of libbve: contexts, device state of the panel and Voronoi RK4
 // paths, single velocity evaluations and launch-level profiling.
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -75,12 +77,14 @@ struct bve_ctx {
     double *absvort = nullptr, *relvort = nullptr;
     double *panel_absvort = nullptr;   // active panels' absolute vorticity (forced steps)
     Source *src = nullptr;
+    int *passive_perm = nullptr;   // mixed pass: spatial order of the passive panels
     double *partial = nullptr;
     size_t partial_cap = 0;   // doubles
     double *ke = nullptr, *ke_part = nullptr, *ke_total = nullptr;
     int ke_part_cap = 0;
     double *staging = nullptr;   // AoS staging for uploads/downloads
     float4 *src32 = nullptr;     // mixed precision: float32 source records
+    float4 *srcw = nullptr;
     float *srcc = nullptr;
     double4 *tile_box = nullptr, *group_box = nullptr;
     int mixed_src_cap = 0, mixed_tgt_cap = 0;
@@ -209,6 +213,7 @@ void free_state(bve_ctx *c)
     dev_free(c->relvort);
     dev_free(c->panel_absvort);
     dev_free(c->src);
+    dev_free(c->passive_perm);
     dev_free(c->ke);
     dev_free(c->ke_part);
     dev_free(c->ke_total);
@@ -257,7 +262,7 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
         // items (self-exclusion tiles; near-field tiles of the mixed kernel,
         // hence its finer split) cost little.  Chunks are never shorter than
         // one tile.
-        const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
+        const int wg = c->plan_wg ? c->plan_wg : (mixed ? 131072 : 16384);
         return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
     }
     // float64: items cost the same, so a persistent grid of one workgroup
@@ -395,9 +400,11 @@ int ensure_mixed(bve_ctx *c, int n_src_pad, int t_stride)
 {
     if (n_src_pad > c->mixed_src_cap) {
         dev_free(c->src32);
+        dev_free(c->srcw);
         dev_free(c->srcc);
         dev_free(c->tile_box);
         HIP_TRY(c, dev_alloc(&c->src32, n_src_pad));
+        HIP_TRY(c, dev_alloc(&c->srcw, n_src_pad));
         HIP_TRY(c, dev_alloc(&c->srcc, n_src_pad));
         HIP_TRY(c, dev_alloc(&c->tile_box, n_src_pad / 256));
         c->mixed_src_cap = n_src_pad;
@@ -418,6 +425,7 @@ MixedArgs mixed_args_of(const bve_ctx *c, const PairArgs &a, int n_targets, int 
     ma.n_targets = n_targets;
     ma.n_sources = n_sources;
     ma.src32 = c->src32;
+    ma.srcw = c->srcw;
     ma.srcc = c->srcc;
     ma.tile_box = c->tile_box;
     ma.group_box = c->group_box;
@@ -445,6 +453,9 @@ PairArgs pair_args_of(const bve_ctx *c, const PairPlan &plan)
     pa.partial = c->partial;
     pa.queue = c->queue;
     pa.area = c->area;
+    pa.perm = c->passive_perm;
+    pa.perm_begin = c->n_active;
+    pa.perm_end = c->n_active + c->n_passive;
     return pa;
 }
 
@@ -455,6 +466,7 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
     PairArgs a = a_in;
     a.n_targets = n_targets;
     a.n_sources = n_sources;
+    if (a.perm_end > n_targets) a.perm = nullptr;   // a pass over the active prefix only
     MixedArgs ma{};
     if (is_mixed(c)) {
         int rc = ensure_mixed(c, a.n_src_pad, a.t_stride);
@@ -705,6 +717,48 @@ int host_down(bve_ctx *c, void *host, const void *dev, size_t bytes)
     return BVE_OK;
 }
 
+// Target order of the mixed pass for n points (AoS xyz): indices sorted by
+// cube face (axis of the largest |component| and its sign), then by the
+// Hilbert index of the point's gnomonic face coordinates on a 1024 x 1024
+// grid (the curve has no jumps inside a face), so that runs of 64 points
+// are spatially compact.
+std::vector<int> hilbert_order(const double *x, int n)
+{
+    constexpr uint32_t N = 1024;
+    auto cell = [](double w) {
+        const double c = (w + 1.0) * 0.5 * N;
+        return (uint32_t)std::min((double)(N - 1), std::max(0.0, c));   // NaN -> 0
+    };
+    std::vector<std::pair<uint64_t, int>> key((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const double *p = x + 3 * (size_t)i;
+        int ax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (std::fabs(p[d]) > std::fabs(p[ax])) ax = d;
+        const double m = std::fabs(p[ax]);
+        const uint64_t face = 2 * (uint64_t)ax + (p[ax] < 0.0 ? 1 : 0);
+        uint32_t hx = cell(m > 0.0 ? p[ax == 0 ? 1 : 0] / m : 0.0);
+        uint32_t hy = cell(m > 0.0 ? p[ax == 2 ? 1 : 2] / m : 0.0);
+        uint64_t d = 0;
+        for (uint32_t s = N / 2; s > 0; s /= 2) {
+            const uint32_t rx = (hx & s) ? 1u : 0u, ry = (hy & s) ? 1u : 0u;
+            d += (uint64_t)s * s * ((3u * rx) ^ ry);
+            if (ry == 0) {   // rotate the quadrant
+                if (rx == 1) {
+                    hx = N - 1 - hx;
+                    hy = N - 1 - hy;
+                }
+                std::swap(hx, hy);
+            }
+        }
+        key[i] = {face * N * N + d, i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm((size_t)n);
+    for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+    return perm;
+}
+
 int upload_aos(bve_ctx *c, const double *host, int n, double *dst_rows, int stride, int off)
 {
     if (n <= 0) return BVE_OK;
@@ -792,6 +846,7 @@ int bve_destroy(bve_ctx *c)
     dev_free(c->partial);
     dev_free(c->staging);
     dev_free(c->src32);
+    dev_free(c->srcw);
     dev_free(c->srcc);
     dev_free(c->tile_box);
     dev_free(c->group_box);
@@ -922,6 +977,7 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
         HIP_TRY(c, dev_alloc(&c->ke, n_active));
         HIP_TRY(c, dev_alloc(&c->ke_total, 1));
         HIP_TRY(c, dev_alloc(&c->src, c->n_src_pad));
+        HIP_TRY(c, dev_alloc(&c->passive_perm, n_passive));
     }
     HIP_TRY(c, hipMemsetAsync(c->src, 0, sizeof(Source) * c->n_src_pad, c->stream));
     rc = host_up(c, c->zeta0, relvort, sizeof(double) * n_active);
@@ -934,6 +990,15 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
     if (rc) return rc;
     rc = upload_aos(c, x_passive, n_passive, c->x0, c->t_stride, n_active);
     if (rc) return rc;
+    if (n_passive > 0) {
+        // passive panels come in tree order, parents of all levels mixed;
+        // the mixed pass groups them by position instead (target slot
+        // n_active + k holds passive panel n_active + perm[k])
+        std::vector<int> perm = hilbert_order(x_passive, n_passive);
+        for (int &k : perm) k += n_active;
+        rc = host_up(c, c->passive_perm, perm.data(), sizeof(int) * (size_t)n_passive);
+        if (rc) return rc;
+    }
     rc = host_sync(c);
     if (rc) return rc;
     c->panels_loaded = true;

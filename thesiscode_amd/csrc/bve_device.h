@@ -339,20 +339,44 @@ __device__ __forceinline__ void pair_f64_segments(const PairArgs &a, StagedSourc
 // pairs, and x_i x S_i is the same cross product as in the float64 body
 // (x_i x x_i = 0).  Float32 differences are formed from per-tile float64
 // origins: every source tile (256 sources) and every group of 64 targets
-// has a float64 centre c; records hold o = fl32(x - c), and a pair's
-// difference is (fl32(c_g - c_s) + o_i) - o_j, so its rounding error scales
-// with |x_i - x_j| plus the tile sizes, not with |x| = 1.
+// has a float64 centre c; records hold o = fl32(x - c), and a target's
+// offset from a source tile's centre is a = fl32(c_g - c_s) + o_i, so its
+// rounding error scales with the tile sizes and the distance of the tiles,
+// not with |x| = 1.
+//
+// Far-field tiles run in moment form.  With A_i = |a|^2 / 2 + c_i and
+// B_j = |o_j|^2 / 2 + c_j (float64, rounded once, from the rounded a and
+// o_j),
+//
+//     D_ij = (A_i + B_j) - a . o_j,
+//     sum_j q_ij (o_j - a) = M_i - a Q_i,
+//     M_i = sum_j (w_j o_j) / D_ij,  Q_i = sum_j w_j / D_ij,
+//
+// so a pair costs an add and three FMAs for D, one v_rcp_f32 and four FMAs,
+// against three subtractions more and a weight product in the difference
+// form.  M - a Q is formed per tile in float64 with a = x_i - c_s in
+// float64.  A non-careful tile lies at least r_nf + r_s from the target
+// (r_s its radius), so |a| / |a - o_j| stays below (r_nf + r_s) / r_nf and
+// the cancellation in D stays within that factor squared of one rounding.
 //
 // Pairs closer than the near-field radius are evaluated in float64 from
 // the float64 source records.  A tile pair is sent down the careful path
-// (per-pair distance test) only when the bounding spheres of the target
-// group and the source tile come within the radius, or when the tile holds
-// a target's own index; all other tiles run the float32 loop.  Float32
-// partial sums are carried into float64 accumulators after every tile, or
-// after every pair with Kahan compensation when requested.
+// (per-pair distance test, difference form) only when the bounding spheres
+// of the target group and the source tile come within the radius, or when
+// the tile holds a target's own index; all other tiles run the float32
+// moment loop.  Float32 partial sums are carried into float64 accumulators
+// after every tile, or after every pair with Kahan compensation when
+// requested.
 
 constexpr int kMixedTile = 256;   // sources per LDS tile and per source bounding sphere
 constexpr int kGroup = 64;        // targets per bounding sphere (one wave's lanes)
+
+// One source tile of the mixed pass in LDS.
+struct MixedLds {
+    float4 pos[kMixedTile];   // o_j, B_j
+    float4 wgt[kMixedTile];   // w_j o_j, w_j
+    float c[kMixedTile];      // c_j
+};
 
 template <bool KAHAN>
 __device__ __forceinline__ void add_f32(float &acc, float &comp, float term)
@@ -367,9 +391,102 @@ __device__ __forceinline__ void add_f32(float &acc, float &comp, float term)
     }
 }
 
+// Target index held by target slot `slot` of the mixed pass.
+__device__ __forceinline__ int mixed_target(const PairArgs &b, int slot)
+{
+    return (b.perm && slot >= b.perm_begin && slot < b.perm_end) ? b.perm[slot - b.perm_begin]
+                                                                 : slot;
+}
+
+// Moment-form sum of the source tile in LDS (centre sb) for N targets with
+// float32 offsets a from the tile centre; adds M - a Q (a = x_i - c_s in
+// float64) to the float64 accumulators.
+template <int N, bool KAHAN>
+__device__ __forceinline__ void moment_tile(const MixedLds &lds, const double4 &sb,
+                                            const float *ax, const float *ay, const float *az,
+                                            const double *ci, const double *xi,
+                                            const double *yi, const double *zi, double *S64x,
+                                            double *S64y, double *S64z)
+{
+    float nax[N], nay[N], naz[N], A[N];
+    float Mx[N], My[N], Mz[N], Q[N], Cx[N], Cy[N], Cz[N], CQ[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        nax[t] = -ax[t];
+        nay[t] = -ay[t];
+        naz[t] = -az[t];
+        A[t] = (float)(0.5 * ((double)ax[t] * ax[t] + (double)ay[t] * ay[t] +
+                              (double)az[t] * az[t]) + ci[t]);
+        Mx[t] = My[t] = Mz[t] = Q[t] = Cx[t] = Cy[t] = Cz[t] = CQ[t] = 0.f;
+    }
+    // One v_rcp_f32 per pair: sharing it between two or four pairs (products
+    // of denominators) measured no faster on gfx950 and adds rounding.
+#pragma unroll 4
+    for (int k = 0; k < kMixedTile; ++k) {
+        const float4 s = lds.pos[k];
+        const float4 v = lds.wgt[k];
+#pragma unroll
+        for (int t = 0; t < N; ++t) {
+            const float D = __builtin_fmaf(
+                nax[t], s.x, __builtin_fmaf(nay[t], s.y, __builtin_fmaf(naz[t], s.z, A[t] + s.w)));
+            const float r = __builtin_amdgcn_rcpf(D);
+            add_f32<KAHAN>(Mx[t], Cx[t], r * v.x);
+            add_f32<KAHAN>(My[t], Cy[t], r * v.y);
+            add_f32<KAHAN>(Mz[t], Cz[t], r * v.z);
+            add_f32<KAHAN>(Q[t], CQ[t], r * v.w);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        const double q = (double)Q[t] - (double)CQ[t];
+        S64x[t] += ((double)Mx[t] - (double)Cx[t]) - (xi[t] - sb.x) * q;
+        S64y[t] += ((double)My[t] - (double)Cy[t]) - (yi[t] - sb.y) * q;
+        S64z[t] += ((double)Mz[t] - (double)Cz[t]) - (zi[t] - sb.z) * q;
+    }
+}
+
+// Careful sum of the source tile in LDS (first source jt) for one target:
+// per-pair near-field test, float64 from the float64 records below the
+// radius, float32 difference form above it, the target's own index skipped.
+template <bool KAHAN>
+__device__ __forceinline__ void careful_tile(const MixedArgs &a, const MixedLds &lds, int jt,
+                                             float ax, float ay, float az, float cif, int self,
+                                             double xi, double yi, double zi, double ci,
+                                             float rnf2, double &S64x, double &S64y,
+                                             double &S64z)
+{
+    float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+    for (int k = 0; k < kMixedTile; ++k) {
+        const float4 s = lds.pos[k];
+        const float dx = ax - s.x, dy = ay - s.y, dz = az - s.z;
+        const float h = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const int j = jt + k;
+        if (j == self) continue;
+        if (h < rnf2) {
+            const Source sj = a.base.src[j];
+            const double ex = sj.x - xi, ey = sj.y - yi, ez = sj.z - zi;
+            const double cj64 = 0.5 * (1.0 - (sj.x * sj.x + sj.y * sj.y + sj.z * sj.z));
+            const double D = 0.5 * (ex * ex + ey * ey + ez * ez) + (ci + cj64);
+            const double q = sj.w / D;
+            S64x = __builtin_fma(q, ex, S64x);
+            S64y = __builtin_fma(q, ey, S64y);
+            S64z = __builtin_fma(q, ez, S64z);
+        } else {
+            const float D = __builtin_fmaf(0.5f, h, cif + lds.c[k]);
+            const float q = lds.wgt[k].w * __builtin_amdgcn_rcpf(D);
+            add_f32<KAHAN>(Sx, Cx, -q * dx);
+            add_f32<KAHAN>(Sy, Cy, -q * dy);
+            add_f32<KAHAN>(Sz, Cz, -q * dz);
+        }
+    }
+    S64x += (double)Sx - (double)Cx;
+    S64y += (double)Sy - (double)Cy;
+    S64z += (double)Sz - (double)Cz;
+}
+
 template <int T, bool KAHAN>
 __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, int ttile,
-                                                float4 *lds, float *ldsc)
+                                                MixedLds &lds)
 {
     constexpr int TILE = kMixedTile;
     const PairPlan &plan = a.base.plan;
@@ -381,10 +498,11 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
     double xi[T], yi[T], zi[T], ci[T], gx[T], gy[T], gz[T], gr[T];
     double S64x[T], S64y[T], S64z[T];
     float oix[T], oiy[T], oiz[T], cif[T];
-    int self[T];
+    int self[T], idx[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int i = tile_base + t * kBlock + threadIdx.x;
+        const int i = mixed_target(a.base, tile_base + t * kBlock + threadIdx.x);
+        idx[t] = i;
         xi[t] = a.base.tx[i];
         yi[t] = a.base.ty[i];
         zi[t] = a.base.tz[i];
@@ -409,12 +527,14 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
     const int j_end = min(j_begin + plan.chunk, a.base.n_src_pad);
     for (int jt = j_begin; jt < j_end; jt += TILE) {
         __syncthreads();
-        lds[threadIdx.x] = a.src32[jt + threadIdx.x];
-        ldsc[threadIdx.x] = a.srcc[jt + threadIdx.x];
+        lds.pos[threadIdx.x] = a.src32[jt + threadIdx.x];
+        lds.wgt[threadIdx.x] = a.srcw[jt + threadIdx.x];
+        lds.c[threadIdx.x] = a.srcc[jt + threadIdx.x];
         __syncthreads();
         const double4 sb = a.tile_box[jt / TILE];
         float ax[T], ay[T], az[T];
-        bool careful = false;
+        bool care[T];
+        bool any = false;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const double dx = gx[t] - sb.x, dy = gy[t] - sb.y, dz = gz[t] - sb.z;
@@ -422,97 +542,32 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
             ay[t] = (float)dy + oiy[t];
             az[t] = (float)dz + oiz[t];
             const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-            // 1e-6: slack for the float32 rounding of the per-pair test below
-            careful |= sb.w >= 0.0 && dist < gr[t] + sb.w + a.rnf + 1e-6;
-            careful |= diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, TILE);
+            // The decision is per target group (one wave's lanes hold one
+            // group per t), hence uniform over the wave.  1e-6: slack for the
+            // float32 rounding of the per-pair test.
+            care[t] = (sb.w >= 0.0 && dist < gr[t] + sb.w + a.rnf + 1e-6) ||
+                      diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, TILE);
+            any |= care[t];
         }
-        float Sx[T], Sy[T], Sz[T], Cx[T], Cy[T], Cz[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) Sx[t] = Sy[t] = Sz[t] = Cx[t] = Cy[t] = Cz[t] = 0.f;
-        if (!careful) {
-            // Four sources share one v_rcp_f32 (the slow transcendental
-            // path) of the product of their denominators:
-            // 1/D_a = D_b D_c D_d / (D_a D_b D_c D_d), at the price of ~2 ulp
-            // more rounding per term.  Far-field denominators lie in
-            // [r_nf^2 / 2, 2 + eps2] (and above ~1e-6 for any r_nf down to
-            // level-10 spacing), so products of four stay inside the float32
-            // range.
-            for (int k = 0; k < TILE; k += 4) {
-                float4 s[4];
-                float cj[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    s[r] = lds[k + r];
-                    cj[r] = ldsc[k + r];
-                }
-#pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    float dx[4], dy[4], dz[4], D[4];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        dx[r] = ax[t] - s[r].x;
-                        dy[r] = ay[t] - s[r].y;
-                        dz[r] = az[t] - s[r].z;
-                        const float h =
-                            __builtin_fmaf(dz[r], dz[r], __builtin_fmaf(dy[r], dy[r], dx[r] * dx[r]));
-                        D[r] = __builtin_fmaf(0.5f, h, cif[t] + cj[r]);
-                    }
-                    const float p01 = D[0] * D[1], p23 = D[2] * D[3];
-                    const float rr = __builtin_amdgcn_rcpf(p01 * p23);
-                    const float r01 = rr * p23, r23 = rr * p01;
-                    const float q[4] = {s[0].w * (D[1] * r01), s[1].w * (D[0] * r01),
-                                        s[2].w * (D[3] * r23), s[3].w * (D[2] * r23)};
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        add_f32<KAHAN>(Sx[t], Cx[t], -q[r] * dx[r]);
-                        add_f32<KAHAN>(Sy[t], Cy[t], -q[r] * dy[r]);
-                        add_f32<KAHAN>(Sz[t], Cz[t], -q[r] * dz[r]);
-                    }
-                }
-            }
+        if (!any) {
+            moment_tile<T, KAHAN>(lds, sb, ax, ay, az, ci, xi, yi, zi, S64x, S64y, S64z);
         } else {
-            for (int k = 0; k < TILE; ++k) {
-                const float4 s = lds[k];
-                const float cj = ldsc[k];
-                const int j = jt + k;
 #pragma unroll
-                for (int t = 0; t < T; ++t) {
-                    const float dx = ax[t] - s.x, dy = ay[t] - s.y, dz = az[t] - s.z;
-                    const float h = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    if (j == self[t]) continue;
-                    if (h < rnf2) {
-                        // near field: float64 from the float64 record
-                        const Source sj = a.base.src[j];
-                        const double ex = sj.x - xi[t], ey = sj.y - yi[t], ez = sj.z - zi[t];
-                        const double cj64 = 0.5 * (1.0 - (sj.x * sj.x + sj.y * sj.y + sj.z * sj.z));
-                        const double D = 0.5 * (ex * ex + ey * ey + ez * ez) + (ci[t] + cj64);
-                        const double q = sj.w / D;
-                        S64x[t] = __builtin_fma(q, ex, S64x[t]);
-                        S64y[t] = __builtin_fma(q, ey, S64y[t]);
-                        S64z[t] = __builtin_fma(q, ez, S64z[t]);
-                    } else {
-                        const float D = __builtin_fmaf(0.5f, h, cif[t] + cj);
-                        const float q = s.w * __builtin_amdgcn_rcpf(D);
-                        add_f32<KAHAN>(Sx[t], Cx[t], -q * dx);
-                        add_f32<KAHAN>(Sy[t], Cy[t], -q * dy);
-                        add_f32<KAHAN>(Sz[t], Cz[t], -q * dz);
-                    }
-                }
+            for (int t = 0; t < T; ++t) {
+                if (care[t])
+                    careful_tile<KAHAN>(a, lds, jt, ax[t], ay[t], az[t], cif[t], self[t], xi[t],
+                                        yi[t], zi[t], ci[t], rnf2, S64x[t], S64y[t], S64z[t]);
+                else
+                    moment_tile<1, KAHAN>(lds, sb, &ax[t], &ay[t], &az[t], &ci[t], &xi[t],
+                                          &yi[t], &zi[t], &S64x[t], &S64y[t], &S64z[t]);
             }
-        }
-        // float64 carry of the tile's float32 sums
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            S64x[t] += (double)Sx[t] - (double)Cx[t];
-            S64y[t] += (double)Sy[t] - (double)Cy[t];
-            S64z[t] += (double)Sz[t] - (double)Cz[t];
         }
     }
 
     double *px = a.base.partial + (size_t)(chunk * 3) * stride;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int i = tile_base + t * kBlock + threadIdx.x;
+        const int i = idx[t];
         px[i] = S64x[t];
         px[stride + i] = S64y[t];
         px[2 * (size_t)stride + i] = S64z[t];
@@ -553,10 +608,13 @@ __device__ __forceinline__ double block_max(double v, double *sh)
 // Mixed-precision per-stage preparation bodies
 // ---------------------------------------------------------------------------
 
-// Source tile `tile` (kMixedTile threads): centre (mean of the valid
-// sources), radius, and the float32 records {x - c, w} and c_j.
+// Source tile `tile` (kMixedTile threads): centre c (mean of the valid
+// sources), radius, and the float32 records pos = {o, B}, wgt = {w o, w}
+// and c_j, with o = fl32(x - c), c_j = (1 - |x|^2) / 2 and
+// B = |o|^2 / 2 + c_j, w o formed in float64 from the rounded o.
 __device__ __forceinline__ void mixed_source_prep_body(const Source *__restrict__ src,
                                                        int n_sources, float4 *__restrict__ src32,
+                                                       float4 *__restrict__ srcw,
                                                        float *__restrict__ srcc,
                                                        double4 *__restrict__ tile_box, int tile,
                                                        double *sh)
@@ -573,20 +631,24 @@ __device__ __forceinline__ void mixed_source_prep_body(const Source *__restrict_
     const double dx = s.x - cx, dy = s.y - cy, dz = s.z - cz;
     const double r = block_max(valid ? sqrt(dx * dx + dy * dy + dz * dz) : -1.0, sh);
     if (threadIdx.x == 0) tile_box[tile] = make_double4(cx, cy, cz, r);
-    src32[j] = make_float4((float)dx, (float)dy, (float)dz, (float)s.w);
-    srcc[j] = (float)(0.5 * (1.0 - (s.x * s.x + s.y * s.y + s.z * s.z)));
+    const float ox = (float)dx, oy = (float)dy, oz = (float)dz;
+    const double cj = 0.5 * (1.0 - (s.x * s.x + s.y * s.y + s.z * s.z));
+    const double o2 = (double)ox * ox + (double)oy * oy + (double)oz * oz;
+    src32[j] = make_float4(ox, oy, oz, (float)(0.5 * o2 + cj));
+    srcw[j] = make_float4((float)(s.w * ox), (float)(s.w * oy), (float)(s.w * oz), (float)s.w);
+    srcc[j] = (float)cj;
 }
 
-// Targets [blk * 256, blk * 256 + 256), one wave per group of kGroup:
+// Target slots [blk * 256, blk * 256 + 256), one wave per group of kGroup:
 // centre and radius of each group.
-__device__ __forceinline__ void mixed_target_prep_body(const double *__restrict__ t, int stride,
-                                                       int n_targets,
+__device__ __forceinline__ void mixed_target_prep_body(const PairArgs &b, int n_targets,
                                                        double4 *__restrict__ group_box, int blk,
                                                        double *sh)
 {
-    const int i = blk * 256 + threadIdx.x;
-    const bool valid = i < n_targets;
-    const double x = t[i], y = t[stride + i], z = t[2 * (size_t)stride + i];
+    const int slot = blk * 256 + threadIdx.x;
+    const int i = mixed_target(b, slot);
+    const bool valid = slot < n_targets;
+    const double x = b.tx[i], y = b.ty[i], z = b.tz[i];
     const int lane = threadIdx.x & 63, base = threadIdx.x & ~63;
     double v[4] = {valid ? 1.0 : 0.0, valid ? x : 0.0, valid ? y : 0.0, valid ? z : 0.0};
     double m[4];
@@ -609,7 +671,7 @@ __device__ __forceinline__ void mixed_target_prep_body(const double *__restrict_
         if (lane < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
-    if (lane == 0) group_box[i / kGroup] = make_double4(cx, cy, cz, sh[base]);
+    if (lane == 0) group_box[slot / kGroup] = make_double4(cx, cy, cz, sh[base]);
     __syncthreads();
 }
 

@@ -35,15 +35,14 @@ __global__ __launch_bounds__(kBlock) void ens_pair_mixed_kernel(const EnsItem *_
                                                                 int n_items,
                                                                 const MixedArgs *__restrict__ members)
 {
-    __shared__ float4 lds[kMixedTile];
-    __shared__ float ldsc[kMixedTile];
+    __shared__ MixedLds lds;
     for (int k = blockIdx.x; k < n_items; k += gridDim.x) {
         const EnsItem it = items[k];
         const MixedArgs a = members[it.member];
         switch (a.base.plan.targets_per_lane) {
-        case 1: pair_mixed_item<1, KAHAN>(a, it.chunk, it.ttile, lds, ldsc); break;
-        case 2: pair_mixed_item<2, KAHAN>(a, it.chunk, it.ttile, lds, ldsc); break;
-        default: pair_mixed_item<4, KAHAN>(a, it.chunk, it.ttile, lds, ldsc); break;
+        case 1: pair_mixed_item<1, KAHAN>(a, it.chunk, it.ttile, lds); break;
+        case 2: pair_mixed_item<2, KAHAN>(a, it.chunk, it.ttile, lds); break;
+        default: pair_mixed_item<4, KAHAN>(a, it.chunk, it.ttile, lds); break;
         }
     }
 }
@@ -81,7 +80,8 @@ __global__ __launch_bounds__(kMixedTile) void ens_mixed_src_prep_kernel(
     __shared__ double sh[kMixedTile];
     const int2 mb = map[blockIdx.x];
     const MixedArgs &a = members[mb.x];
-    mixed_source_prep_body(a.base.src, a.n_sources, a.src32, a.srcc, a.tile_box, mb.y, sh);
+    mixed_source_prep_body(a.base.src, a.n_sources, a.src32, a.srcw, a.srcc, a.tile_box, mb.y,
+                           sh);
 }
 
 __global__ __launch_bounds__(256) void ens_mixed_tgt_prep_kernel(
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void ens_mixed_tgt_prep_kernel(
     __shared__ double sh[256];
     const int2 mb = map[blockIdx.x];
     const MixedArgs &a = members[mb.x];
-    mixed_target_prep_body(a.base.tx, a.base.t_stride, a.n_targets, a.group_box, mb.y, sh);
+    mixed_target_prep_body(a.base, a.n_targets, a.group_box, mb.y, sh);
 }
 
 hipError_t launch_ensemble_pairs(const EnsLaunch &e, hipStream_t s)

@@ -23,6 +23,11 @@ struct PairArgs {
     // vorticity itself in w) and the counts of real targets and sources
     const double *area;
     int n_targets, n_sources;
+    // mixed pass only: target slots [perm_begin, perm_end) hold the targets
+    // perm[slot - perm_begin] (a spatial order of the passive panels, so
+    // that its 64-target groups are compact); other slots are the identity
+    const int *perm;
+    int perm_begin, perm_end;
 };
 
 // One mixed-precision pair-sum pass: the float64 pass's arguments plus the
@@ -31,8 +36,9 @@ struct MixedArgs {
     PairArgs base;
     int n_targets;        // real targets (padding is left out of the spheres)
     int n_sources;        // real sources
-    float4 *src32;        // per source: x - tile centre (float32), weight
-    float *srcc;          // per source: (1 - |x|^2) / 2
+    float4 *src32;        // per source: o = x - tile centre (float32), |o|^2 / 2 + c
+    float4 *srcw;         // per source: w o, w (float32)
+    float *srcc;          // per source: c = (1 - |x|^2) / 2
     double4 *tile_box;    // per 256-source tile: centre, radius
     double4 *group_box;   // per 64 targets: centre, radius
     double rnf;           // near-field radius (float64 below it)

@@ -10,17 +10,17 @@ namespace bve {
 
 __global__ __launch_bounds__(kMixedTile) void mixed_source_prep_kernel(
     const Source *__restrict__ src, int n_sources, float4 *__restrict__ src32,
-    float *__restrict__ srcc, double4 *__restrict__ tile_box)
+    float4 *__restrict__ srcw, float *__restrict__ srcc, double4 *__restrict__ tile_box)
 {
     __shared__ double sh[kMixedTile];
-    mixed_source_prep_body(src, n_sources, src32, srcc, tile_box, blockIdx.x, sh);
+    mixed_source_prep_body(src, n_sources, src32, srcw, srcc, tile_box, blockIdx.x, sh);
 }
 
-__global__ __launch_bounds__(256) void mixed_target_prep_kernel(
-    const double *__restrict__ t, int stride, int n_targets, double4 *__restrict__ group_box)
+__global__ __launch_bounds__(256) void mixed_target_prep_kernel(const PairArgs b, int n_targets,
+                                                                double4 *__restrict__ group_box)
 {
     __shared__ double sh[256];
-    mixed_target_prep_body(t, stride, n_targets, group_box, blockIdx.x, sh);
+    mixed_target_prep_body(b, n_targets, group_box, blockIdx.x, sh);
 }
 
 template <int T, bool KAHAN>
@@ -28,10 +28,9 @@ __global__ __launch_bounds__(kBlock) void pair_sum_mixed_kernel(MixedArgs a)
 {
     const int work = xcd_work_item(blockIdx.x, a.base.plan.grid);
     if (work >= a.base.plan.n_work) return;
-    __shared__ float4 lds[kMixedTile];
-    __shared__ float ldsc[kMixedTile];
+    __shared__ MixedLds lds;
     const int chunk = work / a.base.plan.n_tiles;
-    pair_mixed_item<T, KAHAN>(a, chunk, work - chunk * a.base.plan.n_tiles, lds, ldsc);
+    pair_mixed_item<T, KAHAN>(a, chunk, work - chunk * a.base.plan.n_tiles, lds);
 }
 
 template <int T>
@@ -49,12 +48,12 @@ static hipError_t launch_mixed_t(const MixedArgs &a, hipStream_t s)
 hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(mixed_source_prep_kernel, dim3(a.base.n_src_pad / kMixedTile),
-                       dim3(kMixedTile), 0, s, a.base.src, a.n_sources, a.src32, a.srcc,
-                       a.tile_box);
+                       dim3(kMixedTile), 0, s, a.base.src, a.n_sources, a.src32, a.srcw,
+                       a.srcc, a.tile_box);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mixed_target_prep_kernel, dim3(a.base.t_stride / 256), dim3(256), 0, s,
-                       a.base.tx, a.base.t_stride, a.n_targets, a.group_box);
+                       a.base, a.n_targets, a.group_box);
     return hipGetLastError();
 }
 

@@ -193,11 +193,12 @@ def test_fast_fp64_step_matches_reference(ctx):
 
 @pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_MIXED, _lib.BVE_PRECISION_MIXED_KAHAN])
 def test_mixed_step_matches_reference(ctx, precision):
-    """Mixed arithmetic through the golden 12-step level-4 case.  The mixed
-    pass visits the passive (parent) panels in a spatial order of its own
-    (Hilbert order per cube face), so this also pins that every velocity
-    lands on its own panel: a mix-up would move panels by the order of the
-    displacement itself, against a float32 far-field error of ~1e-6 of it."""
+    """Mixed arithmetic through the golden 12-step level-4 case.  The device
+    holds the passive (parent) panels in a spatial order of its own (Hilbert
+    order per cube face) and puts them back in the caller's order on store,
+    so this also pins that every panel comes back as itself: a mix-up would
+    move panels by the order of the displacement, against a float32
+    far-field error of ~1e-6 of it."""
     name = "rh4_L4_12steps"
     c = golden_manifest()[name]
     g, s = golden_state(name)

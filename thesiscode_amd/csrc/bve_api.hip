@@ -63,6 +63,7 @@ struct bve_ctx {
         void *host;
         const char *pinned;
         size_t bytes;
+        const int *order;   // non-null: xyz record k goes to host record order[k]
     };
     std::vector<Download> downloads;
 
@@ -77,7 +78,7 @@ struct bve_ctx {
     double *absvort = nullptr, *relvort = nullptr;
     double *panel_absvort = nullptr;   // active panels' absolute vorticity (forced steps)
     Source *src = nullptr;
-    int *passive_perm = nullptr;   // mixed pass: spatial order of the passive panels
+    std::vector<int> passive_order;   // passive row k holds the caller's panel passive_order[k]
     double *partial = nullptr;
     size_t partial_cap = 0;   // doubles
     double *ke = nullptr, *ke_part = nullptr, *ke_total = nullptr;
@@ -213,13 +214,13 @@ void free_state(bve_ctx *c)
     dev_free(c->relvort);
     dev_free(c->panel_absvort);
     dev_free(c->src);
-    dev_free(c->passive_perm);
     dev_free(c->ke);
     dev_free(c->ke_part);
     dev_free(c->ke_total);
     c->ke_part_cap = 0;
     c->panels_loaded = false;
     c->n_active = c->n_passive = c->n_particles = 0;
+    c->passive_order.clear();
     c->n_targets = c->t_stride = c->n_src_pad = 0;
     c->particles_have_absvort = false;
 }
@@ -453,9 +454,6 @@ PairArgs pair_args_of(const bve_ctx *c, const PairPlan &plan)
     pa.partial = c->partial;
     pa.queue = c->queue;
     pa.area = c->area;
-    pa.perm = c->passive_perm;
-    pa.perm_begin = c->n_active;
-    pa.perm_end = c->n_active + c->n_passive;
     return pa;
 }
 
@@ -466,7 +464,6 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
     PairArgs a = a_in;
     a.n_targets = n_targets;
     a.n_sources = n_sources;
-    if (a.perm_end > n_targets) a.perm = nullptr;   // a pass over the active prefix only
     MixedArgs ma{};
     if (is_mixed(c)) {
         int rc = ensure_mixed(c, a.n_src_pad, a.t_stride);
@@ -645,7 +642,15 @@ int adjoint_reverse_step(bve_ctx *c, const Source *stages, const PairPlan &vjp)
 int host_sync(bve_ctx *c)
 {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (const auto &d : c->downloads) std::memcpy(d.host, d.pinned, d.bytes);
+    for (const auto &d : c->downloads) {
+        if (!d.order) {
+            std::memcpy(d.host, d.pinned, d.bytes);
+            continue;
+        }
+        const size_t rec = 3 * sizeof(double);
+        for (size_t k = 0; k < d.bytes / rec; ++k)
+            std::memcpy((char *)d.host + rec * (size_t)d.order[k], d.pinned + rec * k, rec);
+    }
     c->downloads.clear();
     c->pinned_used = 0;
     return BVE_OK;
@@ -706,18 +711,18 @@ int host_up(bve_ctx *c, void *dev, const void *host, size_t bytes)
 
 // Device -> host through the pinned arena; the host array is written by the
 // next host_sync.
-int host_down(bve_ctx *c, void *host, const void *dev, size_t bytes)
+int host_down(bve_ctx *c, void *host, const void *dev, size_t bytes, const int *order = nullptr)
 {
     if (bytes == 0 || !host) return BVE_OK;
     char *p = nullptr;
     int rc = pinned_take(c, bytes, &p);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(p, dev, bytes, hipMemcpyDeviceToHost, c->stream));
-    c->downloads.push_back({host, p, bytes});
+    c->downloads.push_back({host, p, bytes, order});
     return BVE_OK;
 }
 
-// Target order of the mixed pass for n points (AoS xyz): indices sorted by
+// Spatial order of n points (AoS xyz): indices sorted by
 // cube face (axis of the largest |component| and its sign), then by the
 // Hilbert index of the point's gnomonic face coordinates on a 1024 x 1024
 // grid (the curve has no jumps inside a face), so that runs of 64 points
@@ -771,13 +776,15 @@ int upload_aos(bve_ctx *c, const double *host, int n, double *dst_rows, int stri
 }
 
 // Enqueues the download; the host array is written by the next host_sync.
-int download_aos(bve_ctx *c, const double *src_rows, int stride, int off, int n, double *host)
+// order non-null: row off + k goes to host record order[k].
+int download_aos(bve_ctx *c, const double *src_rows, int stride, int off, int n, double *host,
+                 const int *order = nullptr)
 {
     if (n <= 0 || !host) return BVE_OK;
     int rc = ensure_staging(c, 3 * (size_t)n);
     if (rc) return rc;
     HIP_TRY(c, launch_soa_to_aos(src_rows, stride, off, n, c->staging, c->stream));
-    return host_down(c, host, c->staging, 3 * sizeof(double) * n);
+    return host_down(c, host, c->staging, 3 * sizeof(double) * n, order);
 }
 
 }  // namespace
@@ -977,7 +984,6 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
         HIP_TRY(c, dev_alloc(&c->ke, n_active));
         HIP_TRY(c, dev_alloc(&c->ke_total, 1));
         HIP_TRY(c, dev_alloc(&c->src, c->n_src_pad));
-        HIP_TRY(c, dev_alloc(&c->passive_perm, n_passive));
     }
     HIP_TRY(c, hipMemsetAsync(c->src, 0, sizeof(Source) * c->n_src_pad, c->stream));
     rc = host_up(c, c->zeta0, relvort, sizeof(double) * n_active);
@@ -988,17 +994,21 @@ int bve_load_panels(bve_ctx *c, int n_active, const double *x_active, const doub
     if (rc) return rc;
     rc = upload_aos(c, x_active, n_active, c->x0, c->t_stride, 0);
     if (rc) return rc;
-    rc = upload_aos(c, x_passive, n_passive, c->x0, c->t_stride, n_active);
-    if (rc) return rc;
-    if (n_passive > 0) {
-        // passive panels come in tree order, parents of all levels mixed;
-        // the mixed pass groups them by position instead (target slot
-        // n_active + k holds passive panel n_active + perm[k])
-        std::vector<int> perm = hilbert_order(x_passive, n_passive);
-        for (int &k : perm) k += n_active;
-        rc = host_up(c, c->passive_perm, perm.data(), sizeof(int) * (size_t)n_passive);
-        if (rc) return rc;
+    // Passive panels come in tree order, parents of all levels mixed; the
+    // device holds them in Hilbert order (passive row k = the caller's
+    // panel passive_order[k]), so that runs of targets are spatially
+    // compact -- the mixed pass's 64-target groups need that.  Every
+    // target's sum is independent of its row, and bve_store_panels puts
+    // them back in the caller's order.
+    c->passive_order = hilbert_order(x_passive, n_passive);
+    {
+        std::vector<double> sorted(3 * (size_t)n_passive);
+        for (int k = 0; k < n_passive; ++k)
+            std::memcpy(&sorted[3 * (size_t)k], x_passive + 3 * (size_t)c->passive_order[k],
+                        3 * sizeof(double));
+        rc = upload_aos(c, sorted.data(), n_passive, c->x0, c->t_stride, n_active);
     }
+    if (rc) return rc;
     rc = host_sync(c);
     if (rc) return rc;
     c->panels_loaded = true;
@@ -1190,7 +1200,9 @@ int bve_store_panels(bve_ctx *c, double *x_active, double *relvort, double *x_pa
     HIP_TRY(c, hipSetDevice(c->device));
     c->downloads.clear();
     int rc = download_aos(c, c->x0, c->t_stride, 0, c->n_active, x_active);
-    if (!rc) rc = download_aos(c, c->x0, c->t_stride, c->n_active, c->n_passive, x_passive);
+    if (!rc)
+        rc = download_aos(c, c->x0, c->t_stride, c->n_active, c->n_passive, x_passive,
+                          c->passive_order.data());
     if (!rc && relvort) rc = host_down(c, relvort, c->zeta0, sizeof(double) * c->n_active);
     if (!rc) rc = host_sync(c);
     return rc;

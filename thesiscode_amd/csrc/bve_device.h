@@ -391,13 +391,6 @@ __device__ __forceinline__ void add_f32(float &acc, float &comp, float term)
     }
 }
 
-// Target index held by target slot `slot` of the mixed pass.
-__device__ __forceinline__ int mixed_target(const PairArgs &b, int slot)
-{
-    return (b.perm && slot >= b.perm_begin && slot < b.perm_end) ? b.perm[slot - b.perm_begin]
-                                                                 : slot;
-}
-
 // Moment-form sum of the source tile in LDS (centre sb) for N targets with
 // float32 offsets a from the tile centre; adds M - a Q (a = x_i - c_s in
 // float64) to the float64 accumulators.
@@ -498,11 +491,10 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
     double xi[T], yi[T], zi[T], ci[T], gx[T], gy[T], gz[T], gr[T];
     double S64x[T], S64y[T], S64z[T];
     float oix[T], oiy[T], oiz[T], cif[T];
-    int self[T], idx[T];
+    int self[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int i = mixed_target(a.base, tile_base + t * kBlock + threadIdx.x);
-        idx[t] = i;
+        const int i = tile_base + t * kBlock + threadIdx.x;
         xi[t] = a.base.tx[i];
         yi[t] = a.base.ty[i];
         zi[t] = a.base.tz[i];
@@ -567,7 +559,7 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
     double *px = a.base.partial + (size_t)(chunk * 3) * stride;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int i = idx[t];
+        const int i = tile_base + t * kBlock + threadIdx.x;
         px[i] = S64x[t];
         px[stride + i] = S64y[t];
         px[2 * (size_t)stride + i] = S64z[t];
@@ -639,15 +631,14 @@ __device__ __forceinline__ void mixed_source_prep_body(const Source *__restrict_
     srcc[j] = (float)cj;
 }
 
-// Target slots [blk * 256, blk * 256 + 256), one wave per group of kGroup:
+// Targets [blk * 256, blk * 256 + 256), one wave per group of kGroup:
 // centre and radius of each group.
 __device__ __forceinline__ void mixed_target_prep_body(const PairArgs &b, int n_targets,
                                                        double4 *__restrict__ group_box, int blk,
                                                        double *sh)
 {
-    const int slot = blk * 256 + threadIdx.x;
-    const int i = mixed_target(b, slot);
-    const bool valid = slot < n_targets;
+    const int i = blk * 256 + threadIdx.x;
+    const bool valid = i < n_targets;
     const double x = b.tx[i], y = b.ty[i], z = b.tz[i];
     const int lane = threadIdx.x & 63, base = threadIdx.x & ~63;
     double v[4] = {valid ? 1.0 : 0.0, valid ? x : 0.0, valid ? y : 0.0, valid ? z : 0.0};
@@ -671,7 +662,7 @@ __device__ __forceinline__ void mixed_target_prep_body(const PairArgs &b, int n_
         if (lane < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
-    if (lane == 0) group_box[slot / kGroup] = make_double4(cx, cy, cz, sh[base]);
+    if (lane == 0) group_box[i / kGroup] = make_double4(cx, cy, cz, sh[base]);
     __syncthreads();
 }
 

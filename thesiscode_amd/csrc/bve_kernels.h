@@ -23,11 +23,6 @@ struct PairArgs {
     // vorticity itself in w) and the counts of real targets and sources
     const double *area;
     int n_targets, n_sources;
-    // mixed pass only: target slots [perm_begin, perm_end) hold the targets
-    // perm[slot - perm_begin] (a spatial order of the passive panels, so
-    // that its 64-target groups are compact); other slots are the identity
-    const int *perm;
-    int perm_begin, perm_end;
 };
 
 // One mixed-precision pair-sum pass: the float64 pass's arguments plus the

@@ -258,11 +258,12 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
     // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
     const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
     if (mixed || c->chunked_only || c->plan_chunk > 0) {
-        // Many short work items: a launch holds ~20 rounds of the resident
-        // workgroups, so the last, partly filled round and the costlier
-        // items (self-exclusion tiles; near-field tiles of the mixed kernel,
-        // hence its finer split) cost little.  Chunks are never shorter than
-        // one tile.
+        // Chunked split: many short work items, so the last, partly filled
+        // round of resident workgroups and the costlier items
+        // (self-exclusion tiles; the mixed kernel's careful tiles, hence its
+        // finer split) cost little.  Chunks are never shorter than one tile.
+        // (The float64 segment queue below, tried for the mixed kernel, ran
+        // its launch 2-5% slower at level 7 for the same step time.)
         const int wg = c->plan_wg ? c->plan_wg : (mixed ? 131072 : 16384);
         return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
     }

@@ -286,22 +286,21 @@ __device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int 
                                a.t_stride, 0, lds);
 }
 
-// Persistent body of a segmented float64 pass: the workgroup takes segments
-// from a.queue[0] until none are left.  Segment s covers items
-// [s seg_len, (s + 1) seg_len) of the tile-major sequence (item = target tile
-// * n_src_tiles + source tile) and writes row s + t for every target tile t
-// it touches.  The last workgroup to finish clears both counters for the
-// next launch.
-template <int T, int TILE, int R>
-__device__ __forceinline__ void pair_f64_segments(const PairArgs &a, StagedSource *lds,
-                                                  int *seg_slot)
+// Segment queue of the persistent passes: the workgroup takes segments from
+// queue[0] until none are left.  Segment s covers items [s seg_len,
+// (s + 1) seg_len) of the tile-major sequence (item = target tile *
+// n_src_tiles + source tile); range(t, j_begin, j_end, s) runs every maximal
+// run of it inside one target tile t, and writes row s + t (unique: s + t
+// increases along the sequence).  The last workgroup to finish clears both
+// counters for the next launch.
+template <int TILE, class Range>
+__device__ __forceinline__ void run_segments(const PairPlan &p, unsigned *queue, int *seg_slot,
+                                             Range &&range)
 {
-    constexpr int TT = kBlock * T;
-    const PairPlan &p = a.plan;
     const int nst = p.n_src_tiles;
     const int n_items = p.n_tiles * nst;
     for (;;) {
-        if (threadIdx.x == 0) *seg_slot = (int)atomicAdd(&a.queue[0], 1u);
+        if (threadIdx.x == 0) *seg_slot = (int)atomicAdd(&queue[0], 1u);
         __syncthreads();
         const int seg = *seg_slot;
         __syncthreads();
@@ -312,18 +311,29 @@ __device__ __forceinline__ void pair_f64_segments(const PairArgs &a, StagedSourc
             const int t = b / nst;
             const int k0 = b - t * nst;
             const int k1 = min(nst, k0 + (e - b));
-            pair_f64_range<T, TILE, R>(a, t, k0 * TILE, k1 * TILE,
-                                       a.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds);
+            range(t, k0 * TILE, k1 * TILE, seg);
             b += k1 - k0;
         }
     }
     if (threadIdx.x == 0) {
         __threadfence();
-        if (atomicAdd(&a.queue[1], 1u) == gridDim.x - 1) {
-            atomicExch(&a.queue[0], 0u);
-            atomicExch(&a.queue[1], 0u);
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
         }
     }
+}
+
+// Persistent body of a segmented float64 pass.
+template <int T, int TILE, int R>
+__device__ __forceinline__ void pair_f64_segments(const PairArgs &a, StagedSource *lds,
+                                                  int *seg_slot)
+{
+    constexpr int TT = kBlock * T;
+    run_segments<TILE>(a.plan, a.queue, seg_slot, [&](int t, int j0, int j1, int seg) {
+        pair_f64_range<T, TILE, R>(a, t, j0, j1, a.partial + (size_t)(seg + t) * 3 * TT, TT,
+                                   t * TT, lds);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -477,15 +487,17 @@ __device__ __forceinline__ void careful_tile(const MixedArgs &a, const MixedLds 
     S64z += (double)Sz - (double)Cz;
 }
 
+// Sources [j_begin, j_end) (whole tiles) for target tile `ttile`: the sums
+// of target i go to out[i - out_off], out[out_stride + i - out_off] and
+// out[2 out_stride + i - out_off].
 template <int T, bool KAHAN>
-__device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, int ttile,
-                                                MixedLds &lds)
+__device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, int j_begin,
+                                                 int j_end, double *out, int out_stride,
+                                                 int out_off, MixedLds &lds)
 {
     constexpr int TILE = kMixedTile;
-    const PairPlan &plan = a.base.plan;
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    const int stride = a.base.t_stride;
     const TargetClasses cls = a.base.cls;
 
     double xi[T], yi[T], zi[T], ci[T], gx[T], gy[T], gz[T], gr[T];
@@ -515,8 +527,6 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
     }
     const float rnf2 = (float)(a.rnf * a.rnf);
 
-    const int j_begin = chunk * plan.chunk;
-    const int j_end = min(j_begin + plan.chunk, a.base.n_src_pad);
     for (int jt = j_begin; jt < j_end; jt += TILE) {
         __syncthreads();
         lds.pos[threadIdx.x] = a.src32[jt + threadIdx.x];
@@ -556,14 +566,25 @@ __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, i
         }
     }
 
-    double *px = a.base.partial + (size_t)(chunk * 3) * stride;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int i = tile_base + t * kBlock + threadIdx.x;
-        px[i] = S64x[t];
-        px[stride + i] = S64y[t];
-        px[2 * (size_t)stride + i] = S64z[t];
+        const int i = tile_base + t * kBlock + threadIdx.x - out_off;
+        out[i] = S64x[t];
+        out[out_stride + i] = S64y[t];
+        out[2 * (size_t)out_stride + i] = S64z[t];
     }
+}
+
+// Work item (source chunk `chunk`, target tile `ttile`) of a chunked plan.
+template <int T, bool KAHAN>
+__device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, int ttile,
+                                                MixedLds &lds)
+{
+    const int j_begin = chunk * a.base.plan.chunk;
+    const int j_end = min(j_begin + a.base.plan.chunk, a.base.n_src_pad);
+    pair_mixed_range<T, KAHAN>(a, ttile, j_begin, j_end,
+                               a.base.partial + (size_t)(chunk * 3) * a.base.t_stride,
+                               a.base.t_stride, 0, lds);
 }
 
 // ---------------------------------------------------------------------------

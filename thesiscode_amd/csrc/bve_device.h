@@ -449,8 +449,10 @@ __device__ __forceinline__ void moment_tile(const MixedLds &lds, const double4 &
 }
 
 // Careful sum of the source tile in LDS (first source jt) for one target:
-// per-pair near-field test, float64 from the float64 records below the
-// radius, float32 difference form above it, the target's own index skipped.
+// float32 difference form with the pairs below the near-field radius and
+// the target's own index masked out (selects, no branch), and those near
+// pairs in float64 from the float64 records (a branch the wave skips when
+// none of its lanes has one).
 template <bool KAHAN>
 __device__ __forceinline__ void careful_tile(const MixedArgs &a, const MixedLds &lds, int jt,
                                              float ax, float ay, float az, float cif, int self,
@@ -464,22 +466,21 @@ __device__ __forceinline__ void careful_tile(const MixedArgs &a, const MixedLds 
         const float dx = ax - s.x, dy = ay - s.y, dz = az - s.z;
         const float h = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         const int j = jt + k;
-        if (j == self) continue;
-        if (h < rnf2) {
+        const bool near = h < rnf2;
+        const float D = __builtin_fmaf(0.5f, h, cif + lds.c[k]);
+        const float q = (near || j == self) ? 0.f : lds.wgt[k].w * __builtin_amdgcn_rcpf(D);
+        add_f32<KAHAN>(Sx, Cx, -q * dx);
+        add_f32<KAHAN>(Sy, Cy, -q * dy);
+        add_f32<KAHAN>(Sz, Cz, -q * dz);
+        if (near && j != self) {
             const Source sj = a.base.src[j];
             const double ex = sj.x - xi, ey = sj.y - yi, ez = sj.z - zi;
             const double cj64 = 0.5 * (1.0 - (sj.x * sj.x + sj.y * sj.y + sj.z * sj.z));
-            const double D = 0.5 * (ex * ex + ey * ey + ez * ez) + (ci + cj64);
-            const double q = sj.w / D;
-            S64x = __builtin_fma(q, ex, S64x);
-            S64y = __builtin_fma(q, ey, S64y);
-            S64z = __builtin_fma(q, ez, S64z);
-        } else {
-            const float D = __builtin_fmaf(0.5f, h, cif + lds.c[k]);
-            const float q = lds.wgt[k].w * __builtin_amdgcn_rcpf(D);
-            add_f32<KAHAN>(Sx, Cx, -q * dx);
-            add_f32<KAHAN>(Sy, Cy, -q * dy);
-            add_f32<KAHAN>(Sz, Cz, -q * dz);
+            const double D64 = 0.5 * (ex * ex + ey * ey + ez * ez) + (ci + cj64);
+            const double q64 = sj.w / D64;
+            S64x = __builtin_fma(q64, ex, S64x);
+            S64y = __builtin_fma(q64, ey, S64y);
+            S64z = __builtin_fma(q64, ez, S64z);
         }
     }
     S64x += (double)Sx - (double)Cx;

@@ -225,7 +225,8 @@ def ensemble_measurement(n_members=64, levels=(4, 5, 6), steps=10, dt=0.01,
     out = {}
     for mode in modes:
         prec = {"mixed": 1, "fp64": 0, "fp64_fast": 2}[mode]
-        table = ensmod.autotune(sorted(set(levels)), precision=prec)
+        counts = {L: sum(1 for m in members if m["level"] == L) for L in set(levels)}
+        table = ensmod.autotune(sorted(set(levels)), precision=prec, weights=counts)
         el, pairs, ens = ensmod.run(members, dt, steps, prec, table=table, use_graph=True)
         ens.close()
         out[mode] = dict(pair_rate_all_classes=pairs * steps / el, seconds=el,

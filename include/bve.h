@@ -236,8 +236,10 @@ int bve_difference_forward(bve_ctx *ctx, double dt, int n_steps, const double *d
 /* No reference counterpart (the reference runs one problem per MPI job).
  * An ensemble holds n_members panel problems of any sizes and advances all
  * of them with one persistent pair-sum launch per RK4 stage (work items of
- * every member, largest first) and one stage-update launch; the steps can
- * be captured once into a HIP graph and replayed. */
+ * every member, largest first, taken from a queue; one launch per distinct
+ * targets-per-lane value among the members, so one with the autotuner's
+ * plans) and one stage-update launch; the steps can be captured once into a
+ * HIP graph and replayed. */
 typedef struct bve_ensemble bve_ensemble;
 int bve_ensemble_create(bve_ensemble **ens, int device, int n_members);
 int bve_ensemble_destroy(bve_ensemble *ens);
@@ -252,9 +254,11 @@ int bve_ensemble_load_member(bve_ensemble *ens, int member, int n_active, const 
                              const double *x_particles, const double *absvort);
 /* Launch geometry of one member: targets per lane (1, 2, 4) and sources per
  * work item (a multiple of 256); 0 keeps the built-in choice.  Set by the
- * autotuner per problem size. */
+ * autotuner per problem size.  Members with the same targets per lane share
+ * one persistent pair launch (the autotuner gives all members one value). */
 int bve_ensemble_set_member_plan(bve_ensemble *ens, int member, int targets_per_lane, int chunk);
-/* Workgroups of the persistent launch (0 = 4 per CU). */
+/* Workgroups of each persistent pair launch (0 = as many as the kernel's
+ * occupancy keeps resident). */
 int bve_ensemble_set_grid(bve_ensemble *ens, int workgroups);
 /* n_steps RK4 steps of every member; use_graph = 1 captures the steps into
  * a HIP graph (re-captured when dt or n_steps change) and launches it.

@@ -92,7 +92,13 @@ def test_member_plans_keep_parity():
 
 
 def test_autotuner_returns_a_supported_plan():
+    """Supported plans, one targets-per-lane value for every level (the
+    members then share one pair launch), and per level the fastest chunk
+    tried at that value."""
     table = autotune([2, 3], members_per_level=2)
     for L, t in table.items():
         assert t["targets_per_lane"] in (1, 2, 4) and t["chunk"] % 256 == 0
         assert len(t["tried"]) >= 3
+        same_t = [r for r in t["tried"] if r[0] == t["targets_per_lane"]]
+        assert t["ms_per_stage"] == min(r[2] for r in same_t)
+    assert len({t["targets_per_lane"] for t in table.values()}) == 1

@@ -1621,13 +1621,14 @@ struct bve_ensemble {
     hipStream_t stream = nullptr;
     std::string error;
     std::vector<bve_ctx *> members;
-    int grid_wg = 0;   // persistent workgroups, 0 = 4 per CU
+    int grid_wg = 0;   // workgroups per persistent pair launch, 0 = occupancy
     int n_cu = 256;
     int precision = BVE_PRECISION_FP64;
 
     // device tables (rebuilt when a member or a parameter changes)
     bool dirty = true;
     EnsItem *d_items = nullptr;
+    unsigned *d_queue = nullptr;   // pair-launch queue counters, two per item group
     PairArgs *d_pair = nullptr;
     MixedArgs *d_mixed = nullptr;
     StepState *d_states = nullptr;
@@ -1669,6 +1670,7 @@ namespace {
 void ens_free_tables(bve_ensemble *e)
 {
     dev_free(e->d_items);
+    dev_free(e->d_queue);
     dev_free(e->d_pair);
     dev_free(e->d_mixed);
     dev_free(e->d_states);
@@ -1703,7 +1705,8 @@ int ens_build(bve_ensemble *e)
     std::vector<MixedArgs> mixeds;
     std::vector<StepState> states;
     std::vector<int> chunks;
-    std::vector<std::pair<long long, EnsItem>> costed;
+    // (targets-per-lane group, -cost): groups in order, largest items first
+    std::vector<std::pair<std::pair<int, long long>, EnsItem>> costed;
     std::vector<int2> stage_map, src_map, tgt_map;
     for (int m = 0; m < (int)e->members.size(); ++m) {
         bve_ctx *c = e->members[m];
@@ -1724,20 +1727,33 @@ int ens_build(bve_ensemble *e)
         mixeds.push_back(mixed_args_of(c, pa, c->n_targets, c->n_active));
         states.push_back(step_state(c));
         chunks.push_back(c->plan.n_chunks);
-        const long long cost = (long long)kBlock * c->plan.targets_per_lane * c->plan.chunk;
+        const int T = c->plan.targets_per_lane;
+        const int group = T == 1 ? 0 : (T == 2 ? 1 : 2);
+        const long long cost = (long long)kBlock * T * c->plan.chunk;
         for (int ch = 0; ch < c->plan.n_chunks; ++ch)
-            for (int t = 0; t < c->plan.n_tiles; ++t) costed.push_back({cost, EnsItem{m, ch, t}});
+            for (int t = 0; t < c->plan.n_tiles; ++t)
+                costed.push_back({{group, -cost}, EnsItem{m, ch, t}});
         for (int b = 0; b < ceil_div(c->n_targets, 256); ++b) stage_map.push_back(make_int2(m, b));
         for (int b = 0; b < c->n_src_pad / 256; ++b) src_map.push_back(make_int2(m, b));
         for (int b = 0; b < c->t_stride / 256; ++b) tgt_map.push_back(make_int2(m, b));
     }
-    // largest items first; the persistent grid deals them out round-robin
+    // one launch per targets-per-lane group; in a group, largest items first
+    // (the persistent grid takes them from a queue)
     std::stable_sort(costed.begin(), costed.end(),
-                     [](const auto &a, const auto &b) { return a.first > b.first; });
+                     [](const auto &a, const auto &b) { return a.first < b.first; });
     std::vector<EnsItem> items;
     items.reserve(costed.size());
-    for (auto &ci : costed) items.push_back(ci.second);
+    int group_begin[kEnsGroups] = {0, 0, 0}, group_count[kEnsGroups] = {0, 0, 0};
+    for (size_t k = 0; k < costed.size(); ++k) {
+        const int g = costed[k].first.first;
+        if (group_count[g]++ == 0) group_begin[g] = (int)k;
+        items.push_back(costed[k].second);
+    }
     int rc = ens_upload(e, e->d_items, items);
+    if (!rc && !e->d_queue) {
+        HIP_TRY(e, dev_alloc(&e->d_queue, 2 * kEnsGroups));
+        HIP_TRY(e, hipMemsetAsync(e->d_queue, 0, 2 * kEnsGroups * sizeof(unsigned), e->stream));
+    }
     if (!rc) rc = ens_upload(e, e->d_pair, pairs);
     if (!rc) rc = ens_upload(e, e->d_mixed, mixeds);
     if (!rc) rc = ens_upload(e, e->d_states, states);
@@ -1750,14 +1766,22 @@ int ens_build(bve_ensemble *e)
     EnsLaunch &L = e->launch;
     L = EnsLaunch{};
     L.n_members = (int)e->members.size();
-    L.grid = std::max(1, std::min((int)items.size(), e->grid_wg ? e->grid_wg : 4 * e->n_cu));
     L.items = e->d_items;
-    L.n_items = (int)items.size();
+    L.queue = e->d_queue;
     L.pair_members = e->d_pair;
     L.mixed_members = e->d_mixed;
     L.mixed = mixed;
     L.kahan = e->precision == BVE_PRECISION_MIXED_KAHAN;
     L.recip = e->precision == BVE_PRECISION_FP64_FAST ? 4 : 1;
+    static const int kT[kEnsGroups] = {1, 2, 4};
+    for (int g = 0; g < kEnsGroups; ++g) {
+        L.group_begin[g] = group_begin[g];
+        L.group_count[g] = group_count[g];
+        // default grid: every workgroup slot the kernel's occupancy allows
+        const int per_cu = ensemble_pair_blocks_per_cu(kT[g], L.mixed, L.kahan, L.recip);
+        const int slots = e->grid_wg ? e->grid_wg : (per_cu > 0 ? per_cu : 2) * e->n_cu;
+        L.group_grid[g] = std::max(1, std::min(group_count[g], slots));
+    }
     L.states = e->d_states;
     L.n_chunks = e->d_chunks;
     L.stage_map = e->d_stage_map;

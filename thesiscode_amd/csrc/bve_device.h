@@ -286,25 +286,42 @@ __device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int 
                                a.t_stride, 0, lds);
 }
 
-// Segment queue of the persistent passes: the workgroup takes segments from
-// queue[0] until none are left.  Segment s covers items [s seg_len,
+// Work queue of the persistent launches: the workgroup takes item indices
+// from queue[0] until n are handed out, running f(k) for each; the last
+// workgroup to finish clears both counters for the next launch (so graph
+// replays need no memset).
+template <class F>
+__device__ __forceinline__ void run_queue(unsigned *queue, int n, int *slot, F &&f)
+{
+    for (;;) {
+        if (threadIdx.x == 0) *slot = (int)atomicAdd(&queue[0], 1u);
+        __syncthreads();
+        const int k = *slot;
+        __syncthreads();
+        if (k >= n) break;
+        f(k);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+// Segments of the persistent passes: segment s covers items [s seg_len,
 // (s + 1) seg_len) of the tile-major sequence (item = target tile *
 // n_src_tiles + source tile); range(t, j_begin, j_end, s) runs every maximal
 // run of it inside one target tile t, and writes row s + t (unique: s + t
-// increases along the sequence).  The last workgroup to finish clears both
-// counters for the next launch.
+// increases along the sequence).
 template <int TILE, class Range>
 __device__ __forceinline__ void run_segments(const PairPlan &p, unsigned *queue, int *seg_slot,
                                              Range &&range)
 {
     const int nst = p.n_src_tiles;
     const int n_items = p.n_tiles * nst;
-    for (;;) {
-        if (threadIdx.x == 0) *seg_slot = (int)atomicAdd(&queue[0], 1u);
-        __syncthreads();
-        const int seg = *seg_slot;
-        __syncthreads();
-        if (seg >= p.segments) break;
+    run_queue(queue, p.segments, seg_slot, [&](int seg) {
         int b = seg * p.seg_len;
         const int e = min(b + p.seg_len, n_items);
         while (b < e) {
@@ -314,14 +331,7 @@ __device__ __forceinline__ void run_segments(const PairPlan &p, unsigned *queue,
             range(t, k0 * TILE, k1 * TILE, seg);
             b += k1 - k0;
         }
-    }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
-            atomicExch(&queue[0], 0u);
-            atomicExch(&queue[1], 0u);
-        }
-    }
+    });
 }
 
 // Persistent body of a segmented float64 pass.

@@ -4,47 +4,47 @@
 //
 // Members have uneven sizes (e.g. cubed spheres at several levels).  The
 // host lists the work items in decreasing cost, and the persistent grid
-// walks the list round-robin, so each workgroup gets a similar share of
-// pairs; every member keeps its own launch geometry (targets per lane and
-// chunk length), chosen per size by the autotuner.
+// takes them from a queue, so each workgroup gets a similar share of the
+// work whatever the items cost (the mixed kernel's careful tiles cost more
+// than its far-field ones); every member keeps its own chunk length, chosen
+// per size by the autotuner.  The kernels are compiled per targets-per-lane
+// count T, so each holds only its own registers (the mixed loop needs four
+// waves per SIMD): one launch per T present among the members, one in all
+// when the autotuner's common T is used.
 #include "bve_common.h"
 #include "bve_device.h"
 #include "bve_kernels.h"
 
 namespace bve {
 
-template <int R>
+template <int T, int R>
 __global__ __launch_bounds__(kBlock) void ens_pair_f64_kernel(const EnsItem *__restrict__ items,
                                                               int n_items,
-                                                              const PairArgs *__restrict__ members)
+                                                              const PairArgs *__restrict__ members,
+                                                              unsigned *queue)
 {
     __shared__ StagedSource lds[256];
-    for (int k = blockIdx.x; k < n_items; k += gridDim.x) {
+    __shared__ int slot;
+    run_queue(queue, n_items, &slot, [&](int k) {
         const EnsItem it = items[k];
         const PairArgs a = members[it.member];
-        switch (a.plan.targets_per_lane) {
-        case 1: pair_f64_item<1, 256, R>(a, it.chunk, it.ttile, lds); break;
-        case 2: pair_f64_item<2, 256, R>(a, it.chunk, it.ttile, lds); break;
-        default: pair_f64_item<4, 256, R>(a, it.chunk, it.ttile, lds); break;
-        }
-    }
+        pair_f64_item<T, 256, R>(a, it.chunk, it.ttile, lds);
+    });
 }
 
-template <bool KAHAN>
+template <int T, bool KAHAN>
 __global__ __launch_bounds__(kBlock) void ens_pair_mixed_kernel(const EnsItem *__restrict__ items,
                                                                 int n_items,
-                                                                const MixedArgs *__restrict__ members)
+                                                                const MixedArgs *__restrict__ members,
+                                                                unsigned *queue)
 {
     __shared__ MixedLds lds;
-    for (int k = blockIdx.x; k < n_items; k += gridDim.x) {
+    __shared__ int slot;
+    run_queue(queue, n_items, &slot, [&](int k) {
         const EnsItem it = items[k];
         const MixedArgs a = members[it.member];
-        switch (a.base.plan.targets_per_lane) {
-        case 1: pair_mixed_item<1, KAHAN>(a, it.chunk, it.ttile, lds); break;
-        case 2: pair_mixed_item<2, KAHAN>(a, it.chunk, it.ttile, lds); break;
-        default: pair_mixed_item<4, KAHAN>(a, it.chunk, it.ttile, lds); break;
-        }
-    }
+        pair_mixed_item<T, KAHAN>(a, it.chunk, it.ttile, lds);
+    });
 }
 
 __global__ __launch_bounds__(256) void ens_stage_kernel(const int2 *__restrict__ map,
@@ -93,25 +93,53 @@ __global__ __launch_bounds__(256) void ens_mixed_tgt_prep_kernel(
     mixed_target_prep_body(a.base, a.n_targets, a.group_box, mb.y, sh);
 }
 
+// Pair kernel of the ensemble for targets-per-lane T in the ensemble's
+// arithmetic.
+template <int T>
+static const void *ens_pair_kernel(int mixed, int kahan, int recip)
+{
+    if (mixed)
+        return kahan ? (const void *)ens_pair_mixed_kernel<T, true>
+                     : (const void *)ens_pair_mixed_kernel<T, false>;
+    if (recip == 4) return (const void *)ens_pair_f64_kernel<T, 4>;
+    if (recip == 2) return (const void *)ens_pair_f64_kernel<T, 2>;
+    return (const void *)ens_pair_f64_kernel<T, 1>;
+}
+
+static const void *ens_pair_kernel_for(int T, int mixed, int kahan, int recip)
+{
+    switch (T) {
+    case 1: return ens_pair_kernel<1>(mixed, kahan, recip);
+    case 2: return ens_pair_kernel<2>(mixed, kahan, recip);
+    case 4: return ens_pair_kernel<4>(mixed, kahan, recip);
+    default: return nullptr;
+    }
+}
+
+int ensemble_pair_blocks_per_cu(int T, int mixed, int kahan, int recip)
+{
+    const void *k = ens_pair_kernel_for(T, mixed, kahan, recip);
+    int n = 0;
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBlock, 0) != hipSuccess)
+        return 0;
+    return n;
+}
+
 hipError_t launch_ensemble_pairs(const EnsLaunch &e, hipStream_t s)
 {
-    if (e.n_items <= 0) return hipSuccess;
-    if (e.mixed) {
-        if (e.kahan)
-            hipLaunchKernelGGL(ens_pair_mixed_kernel<true>, dim3(e.grid), dim3(kBlock), 0, s,
-                               e.items, e.n_items, e.mixed_members);
-        else
-            hipLaunchKernelGGL(ens_pair_mixed_kernel<false>, dim3(e.grid), dim3(kBlock), 0, s,
-                               e.items, e.n_items, e.mixed_members);
-    } else if (e.recip == 4) {
-        hipLaunchKernelGGL(ens_pair_f64_kernel<4>, dim3(e.grid), dim3(kBlock), 0, s, e.items,
-                           e.n_items, e.pair_members);
-    } else if (e.recip == 2) {
-        hipLaunchKernelGGL(ens_pair_f64_kernel<2>, dim3(e.grid), dim3(kBlock), 0, s, e.items,
-                           e.n_items, e.pair_members);
-    } else {
-        hipLaunchKernelGGL(ens_pair_f64_kernel<1>, dim3(e.grid), dim3(kBlock), 0, s, e.items,
-                           e.n_items, e.pair_members);
+    static const int kT[kEnsGroups] = {1, 2, 4};
+    for (int g = 0; g < kEnsGroups; ++g) {
+        if (e.group_count[g] <= 0) continue;
+        if (!e.queue || e.group_grid[g] <= 0) return hipErrorInvalidValue;
+        const void *k = ens_pair_kernel_for(kT[g], e.mixed, e.kahan, e.recip);
+        const EnsItem *items = e.items + e.group_begin[g];
+        int n = e.group_count[g];
+        unsigned *queue = e.queue + 2 * g;
+        const void *members =
+            e.mixed ? (const void *)e.mixed_members : (const void *)e.pair_members;
+        void *args[] = {(void *)&items, (void *)&n, (void *)&members, (void *)&queue};
+        hipError_t err = hipLaunchKernel(k, dim3(e.group_grid[g]), dim3(kBlock), args, 0, s);
+        if (err != hipSuccess) return err;
     }
     return hipGetLastError();
 }

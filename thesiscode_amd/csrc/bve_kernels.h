@@ -160,11 +160,16 @@ struct EnsItem {
 };
 
 // Device tables of an ensemble launch (all pointers are device memory).
+// Item groups of the ensemble's pair launches: members with 1, 2 and 4
+// targets per lane.
+constexpr int kEnsGroups = 3;
+
 struct EnsLaunch {
     int n_members;
-    int grid;                        // persistent workgroups
-    const EnsItem *items;
-    int n_items;
+    const EnsItem *items;            // grouped by targets per lane, largest first in a group
+    int group_begin[kEnsGroups], group_count[kEnsGroups];
+    int group_grid[kEnsGroups];      // persistent workgroups of each group's launch
+    unsigned *queue;                 // two counters per group, zero between launches
     const PairArgs *pair_members;    // float64 modes
     const MixedArgs *mixed_members;  // mixed modes
     int mixed, kahan, recip;
@@ -178,6 +183,9 @@ struct EnsLaunch {
     int n_tgt_blocks;
 };
 
+// Workgroups of the ensemble pair kernel for T targets per lane that one CU
+// holds at once (0 if the occupancy query fails).
+int ensemble_pair_blocks_per_cu(int T, int mixed, int kahan, int recip);
 hipError_t launch_ensemble_pairs(const EnsLaunch &e, hipStream_t s);
 hipError_t launch_ensemble_mixed_prep(const EnsLaunch &e, hipStream_t s);
 hipError_t launch_ensemble_stage(const EnsLaunch &e, const StageParams &sp, hipStream_t s);

@@ -68,14 +68,18 @@ def candidate_plans(level):
 
 
 def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, omega=ic.OMEGA_EARTH,
-             dt=0.01, device=0, log=None):
+             dt=0.01, device=0, log=None, weights=None):
     """Time every candidate plan on a homogeneous ensemble of each level (one
     RK4 step, HIP events around the pair launches) and return
-    {level: {"targets_per_lane", "chunk", "ms_per_stage", "tried"}}."""
+    {level: {"targets_per_lane", "chunk", "ms_per_stage", "tried"}}.
+
+    Members with the same targets per lane share one pair launch, so the
+    table gives every level one common value: the T whose best chunks
+    minimise sum_L weights[L] * ms_per_stage(L) (weights: members per level,
+    default equal), and per level the fastest chunk at that T."""
     table = {}
     for L in levels:
         mem = make_members(members_per_level, (L,))
-        best = None
         tried = []
         with _lib.Ensemble(members_per_level, device) as ens:
             ens.set_params(omega, 0.05, precision, 0.02)
@@ -91,12 +95,21 @@ def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, ome
                 ens.set_profiling(False)
                 per = ms / max(n, 1)
                 tried.append((T, chunk, per))
-                if best is None or per < best[2]:
-                    best = (T, chunk, per)
-        table[L] = dict(targets_per_lane=best[0], chunk=best[1], ms_per_stage=best[2],
-                        tried=tried)
+        table[L] = dict(tried=tried)
+    weights = weights or {L: 1.0 for L in levels}
+
+    def best_at(L, T):
+        runs = [r for r in table[L]["tried"] if r[0] == T]
+        return min(runs, key=lambda r: r[2]) if runs else None
+
+    common = [T for T in (1, 2, 4) if all(best_at(L, T) for L in levels)]
+    T_best = min(common, key=lambda T: sum(weights.get(L, 1.0) * best_at(L, T)[2]
+                                           for L in levels))
+    for L in levels:
+        T, chunk, ms = best_at(L, T_best)
+        table[L].update(targets_per_lane=T, chunk=chunk, ms_per_stage=ms)
         if log:
-            log(f"autotune level {L}: T={best[0]} chunk={best[1]} {best[2]:.3f} ms/stage")
+            log(f"autotune level {L}: T={T} chunk={chunk} {ms:.3f} ms/stage")
     return table
 
 

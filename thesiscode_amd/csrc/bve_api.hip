@@ -252,9 +252,12 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
     if (c->precision == BVE_PRECISION_REFERENCE) return make_exact_plan(n_targets, n_src_pad);
     const bool mixed = c->precision == BVE_PRECISION_MIXED ||
                        c->precision == BVE_PRECISION_MIXED_KAHAN;
-    // mixed: at most 2 targets per lane (register pressure of the two paths)
+    // mixed: 2 targets per lane from ~130k targets (level 7: 5.3 against
+    // 5.8 ms per launch), 1 below (level 6: 0.40 against 0.45 ms); never 4
+    // (3 waves per SIMD instead of 4)
     const int T = c->plan_t ? c->plan_t
-                            : std::min(mixed ? 2 : 4, choose_targets_per_lane(n_targets));
+                            : (mixed ? (n_targets >= 131072 ? 2 : 1)
+                                     : choose_targets_per_lane(n_targets));
     // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
     const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
     if (mixed || c->chunked_only || c->plan_chunk > 0) {

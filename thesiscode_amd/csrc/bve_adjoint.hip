@@ -12,12 +12,14 @@
 //   G^S_j     = w_j sum_i [ n_i / D_ij + P_ij x_i / D_ij^2 ]
 //   dL/dzeta_j = -(1/4 PI) A_j sum_i P_ij / D_ij
 //
-// The target-major sums (S, T) run with a target per lane and sources
-// streamed through LDS, the source-major sums (H, g) with a source per lane
-// and the cotangent records {x_i, n_i} streamed; both use the forward
-// kernel's tiling, chunking, XCD mapping and batched float64 reciprocals,
-// and write per-chunk partial sums that the combine kernel adds in a fixed
-// order.
+// Targets and sources are the same active panels, so one pass evaluates
+// all four sums: panel k in a lane, the records {x_l, w_l, n_l} of every
+// other panel streamed through LDS; per pair the denominator and its
+// reciprocal are shared by the target-side terms (S, T: k as target, l as
+// source) and the source-side ones (H, g: k as source, l as target).  The
+// pass uses the forward kernel's tiling, chunking, XCD mapping and batched
+// float64 reciprocals, and writes per-chunk partial sums that the combine
+// kernel adds in a fixed order.
 #include "bve_common.h"
 #include "bve_kernels.h"
 
@@ -72,12 +74,12 @@ __device__ __forceinline__ bool diag_tile(int lo, int self_end, int jt, int tile
 }
 
 // ---------------------------------------------------------------------------
-// Target-major pass: S_i and T_i
+// Pair pass: S, T (panel k as target) and H, g (panel k as source)
 // ---------------------------------------------------------------------------
 template <int T, int TILE, int R>
-__global__ __launch_bounds__(kBlock) void vjp_target_kernel(
-    const AdjRec *__restrict__ rec, int stride, int self_end, const Source *__restrict__ src,
-    int n_src_pad, PairPlan plan, double *__restrict__ partial)
+__global__ __launch_bounds__(kBlock) void vjp_kernel(const AdjRec *__restrict__ rec, int stride,
+                                                     int self_end, int n_pad, PairPlan plan,
+                                                     double *__restrict__ partial)
 {
     const int work = xcd_item(blockIdx.x, plan.grid);
     if (work >= plan.n_work) return;
@@ -86,114 +88,31 @@ __global__ __launch_bounds__(kBlock) void vjp_target_kernel(
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
 
-    double nx[T], ny[T], nz[T], ex[T], ey[T], ez[T];
-    double Sx[T], Sy[T], Sz[T], Tx[T], Ty[T], Tz[T];
+    double xk[T], yk[T], zk[T], ex[T], ey[T], ez[T];
+    double Sx[T], Sy[T], Sz[T], Tx[T], Ty[T], Tz[T], Hx[T], Hy[T], Hz[T], G[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const AdjRec a = rec[tile_base + t * kBlock + threadIdx.x];
-        nx[t] = -a.x;
-        ny[t] = -a.y;
-        nz[t] = -a.z;
+        xk[t] = a.x;
+        yk[t] = a.y;
+        zk[t] = a.z;
         ex[t] = a.nx;
         ey[t] = a.ny;
         ez[t] = a.nz;
         Sx[t] = Sy[t] = Sz[t] = Tx[t] = Ty[t] = Tz[t] = 0.0;
+        Hx[t] = Hy[t] = Hz[t] = G[t] = 0.0;
     }
-    __shared__ Source lds[TILE];
+    __shared__ AdjRec lds[TILE];
     const int j_begin = chunk * plan.chunk;
-    const int j_end = min(j_begin + plan.chunk, n_src_pad);
+    const int j_end = min(j_begin + plan.chunk, n_pad);
     for (int jt = j_begin; jt < j_end; jt += TILE) {
         __syncthreads();
-        for (int k = threadIdx.x; k < TILE; k += kBlock) lds[k] = src[jt + k];
+        for (int k = threadIdx.x; k < TILE; k += kBlock) lds[k] = rec[jt + k];
         __syncthreads();
         bool diag = false;
 #pragma unroll
         for (int t = 0; t < T; ++t)
             diag |= diag_tile(tile_base + t * kBlock + wave * 64, self_end, jt, TILE);
-        for (int k = 0; k < TILE; k += R) {
-            Source s[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = lds[k + r];
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const int self = tile_base + t * kBlock + threadIdx.x;
-                double d[R], rc[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    d[r] = __builtin_fma(nx[t], s[r].x, 1.0);
-                    d[r] = __builtin_fma(ny[t], s[r].y, d[r]);
-                    d[r] = __builtin_fma(nz[t], s[r].z, d[r]);
-                    if (diag && jt + k + r == self) d[r] = 1.0;
-                }
-                recips<R>(d, rc);
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    double q = s[r].w * rc[r];
-                    if (diag && jt + k + r == self) q = 0.0;
-                    const double p = __builtin_fma(s[r].z, ez[t], __builtin_fma(s[r].y, ey[t], s[r].x * ex[t]));
-                    const double tq = q * rc[r] * p;
-                    Sx[t] = __builtin_fma(q, s[r].x, Sx[t]);
-                    Sy[t] = __builtin_fma(q, s[r].y, Sy[t]);
-                    Sz[t] = __builtin_fma(q, s[r].z, Sz[t]);
-                    Tx[t] = __builtin_fma(tq, s[r].x, Tx[t]);
-                    Ty[t] = __builtin_fma(tq, s[r].y, Ty[t]);
-                    Tz[t] = __builtin_fma(tq, s[r].z, Tz[t]);
-                }
-            }
-        }
-    }
-    double *p = partial + (size_t)(chunk * 6) * stride;
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-        const int i = tile_base + t * kBlock + threadIdx.x;
-        p[i] = Sx[t];
-        p[stride + i] = Sy[t];
-        p[2 * (size_t)stride + i] = Sz[t];
-        p[3 * (size_t)stride + i] = Tx[t];
-        p[4 * (size_t)stride + i] = Ty[t];
-        p[5 * (size_t)stride + i] = Tz[t];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Source-major pass: H_j and g_j (lane = source j, records {x_i, n_i} streamed)
-// ---------------------------------------------------------------------------
-template <int T, int TILE, int R>
-__global__ __launch_bounds__(kBlock) void vjp_source_kernel(
-    const Source *__restrict__ src, int stride, int self_end, const AdjRec *__restrict__ rec,
-    int n_rec_pad, PairPlan plan, double *__restrict__ partial)
-{
-    const int work = xcd_item(blockIdx.x, plan.grid);
-    if (work >= plan.n_work) return;
-    const int chunk = work / plan.n_tiles;
-    const int ttile = work - chunk * plan.n_tiles;
-    const int tile_base = ttile * (kBlock * T);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-
-    double nx[T], ny[T], nz[T], xj[T], yj[T], zj[T];
-    double Hx[T], Hy[T], Hz[T], G[T];
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-        const Source s = src[tile_base + t * kBlock + threadIdx.x];
-        xj[t] = s.x;
-        yj[t] = s.y;
-        zj[t] = s.z;
-        nx[t] = -s.x;
-        ny[t] = -s.y;
-        nz[t] = -s.z;
-        Hx[t] = Hy[t] = Hz[t] = G[t] = 0.0;
-    }
-    __shared__ AdjRec lds[TILE];
-    const int i_begin = chunk * plan.chunk;
-    const int i_end = min(i_begin + plan.chunk, n_rec_pad);
-    for (int it = i_begin; it < i_end; it += TILE) {
-        __syncthreads();
-        for (int k = threadIdx.x; k < TILE; k += kBlock) lds[k] = rec[it + k];
-        __syncthreads();
-        bool diag = false;
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-            diag |= diag_tile(tile_base + t * kBlock + wave * 64, self_end, it, TILE);
         for (int k = 0; k < TILE; k += R) {
             AdjRec a[R];
 #pragma unroll
@@ -204,18 +123,29 @@ __global__ __launch_bounds__(kBlock) void vjp_source_kernel(
                 double d[R], rc[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    d[r] = __builtin_fma(nx[t], a[r].x, 1.0);
-                    d[r] = __builtin_fma(ny[t], a[r].y, d[r]);
-                    d[r] = __builtin_fma(nz[t], a[r].z, d[r]);
-                    if (diag && it + k + r == self) d[r] = 1.0;
+                    d[r] = __builtin_fma(-xk[t], a[r].x, 1.0);
+                    d[r] = __builtin_fma(-yk[t], a[r].y, d[r]);
+                    d[r] = __builtin_fma(-zk[t], a[r].z, d[r]);
+                    if (diag && jt + k + r == self) d[r] = 1.0;
                 }
                 recips<R>(d, rc);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     double rr = rc[r];
-                    if (diag && it + k + r == self) rr = 0.0;
-                    const double p = __builtin_fma(zj[t], a[r].nz, __builtin_fma(yj[t], a[r].ny, xj[t] * a[r].nx));
-                    const double pr = p * rr;
+                    if (diag && jt + k + r == self) rr = 0.0;
+                    // k as target, l as source: q = w_l / d, P_kl = x_l . n_k
+                    const double q = a[r].w * rr;
+                    const double pt = __builtin_fma(a[r].z, ez[t], __builtin_fma(a[r].y, ey[t], a[r].x * ex[t]));
+                    const double tq = q * rr * pt;
+                    Sx[t] = __builtin_fma(q, a[r].x, Sx[t]);
+                    Sy[t] = __builtin_fma(q, a[r].y, Sy[t]);
+                    Sz[t] = __builtin_fma(q, a[r].z, Sz[t]);
+                    Tx[t] = __builtin_fma(tq, a[r].x, Tx[t]);
+                    Ty[t] = __builtin_fma(tq, a[r].y, Ty[t]);
+                    Tz[t] = __builtin_fma(tq, a[r].z, Tz[t]);
+                    // k as source, l as target: P_lk = x_k . n_l
+                    const double ps = __builtin_fma(zk[t], a[r].nz, __builtin_fma(yk[t], a[r].ny, xk[t] * a[r].nx));
+                    const double pr = ps * rr;
                     const double u = pr * rr;
                     G[t] += pr;
                     Hx[t] = __builtin_fma(rr, a[r].nx, __builtin_fma(u, a[r].x, Hx[t]));
@@ -225,14 +155,13 @@ __global__ __launch_bounds__(kBlock) void vjp_source_kernel(
             }
         }
     }
-    double *p = partial + (size_t)(chunk * 4) * stride;
+    double *p = partial + (size_t)(chunk * kAdjRows) * stride;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const int i = tile_base + t * kBlock + threadIdx.x;
-        p[i] = Hx[t];
-        p[stride + i] = Hy[t];
-        p[2 * (size_t)stride + i] = Hz[t];
-        p[3 * (size_t)stride + i] = G[t];
+        const double v[kAdjRows] = {Sx[t], Sy[t], Sz[t], Tx[t], Ty[t], Tz[t], Hx[t], Hy[t], Hz[t], G[t]};
+#pragma unroll
+        for (int q = 0; q < kAdjRows; ++q) p[(size_t)q * stride + i] = v[q];
     }
 }
 
@@ -247,7 +176,7 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(AdjStage a)
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n_pad) return;
     if (i >= a.n) {
-        a.rec[i] = AdjRec{0, 0, 0, 0, 0, 0};
+        a.rec[i] = AdjRec{0, 0, 0, 0, 0, 0, 0, 0};
         return;
     }
     const int st = a.stride;
@@ -264,27 +193,25 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(AdjStage a)
     a.m[st + i] = my;
     a.m[2 * st + i] = mz;
     // n = m x x
-    a.rec[i] = AdjRec{y.x, y.y, y.z, my * y.z - mz * y.y, mz * y.x - mx * y.z, mx * y.y - my * y.x};
+    a.rec[i] = AdjRec{y.x, y.y, y.z, y.w, my * y.z - mz * y.y, mz * y.x - mx * y.z,
+                      mx * y.y - my * y.x, 0.0};
 }
 
-// Ybar of stage s from the two passes' chunk partials; lam_acc += Ybar.
+// Ybar of stage s from the pair pass's chunk partials; lam_acc += Ybar.
 __global__ __launch_bounds__(256) void adj_combine_kernel(AdjStage a)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n) return;
     const int st = a.stride;
     double S[3] = {0, 0, 0}, Tv[3] = {0, 0, 0}, H[3] = {0, 0, 0}, g = 0.0;
-    for (int c = 0; c < a.n_chunks_t; ++c) {
-        const double *p = a.partial_t + (size_t)(c * 6) * st;
+    for (int c = 0; c < a.n_chunks; ++c) {
+        const double *p = a.partial + (size_t)(c * kAdjRows) * st;
         for (int q = 0; q < 3; ++q) {
             S[q] += p[(size_t)q * st + i];
             Tv[q] += p[(size_t)(q + 3) * st + i];
+            H[q] += p[(size_t)(q + 6) * st + i];
         }
-    }
-    for (int c = 0; c < a.n_chunks_s; ++c) {
-        const double *p = a.partial_s + (size_t)(c * 4) * st;
-        for (int q = 0; q < 3; ++q) H[q] += p[(size_t)q * st + i];
-        g += p[3 * (size_t)st + i];
+        g += p[9 * (size_t)st + i];
     }
     const Source y = a.ys[i];
     const double mx = a.m[i], my = a.m[st + i], mz = a.m[2 * st + i];
@@ -308,12 +235,8 @@ __global__ __launch_bounds__(256) void adj_combine_kernel(AdjStage a)
 template <int T, int TILE, int R>
 static hipError_t launch_vjp3(const AdjStage &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((vjp_target_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0, s,
-                       a.rec, a.stride, a.n, a.ys, a.n_pad, a.plan, a.partial_t);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((vjp_source_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0, s,
-                       a.ys, a.stride, a.n, a.rec, a.n_pad, a.plan, a.partial_s);
+    hipLaunchKernelGGL((vjp_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0, s, a.rec,
+                       a.stride, a.n, a.n_pad, a.plan, a.partial);
     return hipGetLastError();
 }
 

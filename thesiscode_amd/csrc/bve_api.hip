@@ -101,8 +101,8 @@ struct bve_ctx {
     Source *adj_stages = nullptr;   // adj_every steps x 4 stages x adj_pad records
     double *adj_lam = nullptr, *adj_lam_acc = nullptr, *adj_ybar = nullptr, *adj_m = nullptr;
     AdjRec *adj_rec = nullptr;
-    double *adj_pt = nullptr, *adj_ps = nullptr;
-    size_t adj_pt_cap = 0, adj_ps_cap = 0, adj_ckpt_cap = 0, adj_stage_cap = 0;
+    double *adj_pt = nullptr;   // partial rows of the adjoint's pair pass
+    size_t adj_pt_cap = 0, adj_ckpt_cap = 0, adj_stage_cap = 0;
 
     // profiling
     bool profiling = false;
@@ -555,8 +555,7 @@ void free_adjoint(bve_ctx *c)
     dev_free(c->adj_m);
     dev_free(c->adj_rec);
     dev_free(c->adj_pt);
-    dev_free(c->adj_ps);
-    c->adj_pt_cap = c->adj_ps_cap = c->adj_ckpt_cap = c->adj_stage_cap = 0;
+    c->adj_pt_cap = c->adj_ckpt_cap = c->adj_stage_cap = 0;
     c->adj_ready = false;
     c->adj_n = c->adj_pad = 0;
 }
@@ -591,7 +590,7 @@ int copy_active_state(bve_ctx *c, double *rows, bool save)
     return BVE_OK;
 }
 
-// Plans of the adjoint: the active-only forward pass and the VJP passes.
+// Plans of the adjoint: the active-only forward pass and the VJP pass.
 int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
 {
     fwd = plan_for(c, c->n_active, c->n_src_pad);
@@ -601,9 +600,8 @@ int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
     const int wg = c->plan_wg ? c->plan_wg : 16384;
     vjp = make_plan(c->n_active, c->adj_pad, T, 256, wg,
                     c->precision == BVE_PRECISION_FP64_FAST ? 4 : 1);
-    rc = grow(c, c->adj_pt, c->adj_pt_cap, (size_t)vjp.n_chunks * 6 * c->adj_pad, false);
-    if (rc) return rc;
-    return grow(c, c->adj_ps, c->adj_ps_cap, (size_t)vjp.n_chunks * 4 * c->adj_pad, false);
+    return grow(c, c->adj_pt, c->adj_pt_cap, (size_t)vjp.n_chunks * kAdjRows * c->adj_pad,
+                false);
 }
 
 // One reverse RK4 step: lam <- lam + sum_s Ybar_s with the stage inputs in
@@ -632,9 +630,8 @@ int adjoint_reverse_step(bve_ctx *c, const Source *stages, const PairPlan &vjp)
         a.ys = stages + (size_t)s * P;
         a.area = c->area;
         a.plan = vjp;
-        a.partial_t = c->adj_pt;
-        a.partial_s = c->adj_ps;
-        a.n_chunks_t = a.n_chunks_s = vjp.n_chunks;
+        a.partial = c->adj_pt;
+        a.n_chunks = vjp.n_chunks;
         HIP_TRY(c, launch_adjoint_stage(a, c->stream));
     }
     std::swap(c->adj_lam, c->adj_lam_acc);

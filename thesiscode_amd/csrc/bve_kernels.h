@@ -40,10 +40,15 @@ struct MixedArgs {
     int kahan;            // 1: Kahan-compensated float32 sums
 };
 
-// Cotangent record of one target in the adjoint: position and n = m x x.
+// Record of one active panel in the adjoint's pair pass: position, weight
+// w = zeta A and n = m x x (m its velocity cotangent); 64 B.
 struct __align__(16) AdjRec {
-    double x, y, z, nx, ny, nz;
+    double x, y, z, w, nx, ny, nz, pad;
 };
+
+// Partial rows per source chunk of the adjoint's pair pass: S (3), T (3),
+// H (3), g (1).
+constexpr int kAdjRows = 10;
 
 // One reverse RK4 stage of the adjoint on n active panels.  Row arrays have
 // `stride` entries per row (rows x, y, z, zeta); source and record arrays
@@ -61,9 +66,8 @@ struct AdjStage {
     const Source *ys;      // this stage's input as source records {x, zeta A}
     const double *area;
     PairPlan plan;
-    double *partial_t;     // target-major pass: 6 rows per chunk
-    double *partial_s;     // source-major pass: 4 rows per chunk
-    int n_chunks_t, n_chunks_s;
+    double *partial;       // pair pass: kAdjRows rows per chunk
+    int n_chunks;
 };
 
 hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s);

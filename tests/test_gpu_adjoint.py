@@ -152,6 +152,22 @@ def test_checkpoint_interval_does_not_change_the_gradient(ctx):
         assert np.array_equal(gz, out[0][1]) and np.array_equal(gx, out[0][2])
 
 
+def test_repeated_reverse_sweep_gives_the_same_gradient(ctx):
+    """The forward sweep keeps the last checkpoint segment's stage inputs,
+    which the first reverse sweep uses instead of recomputing them and then
+    overwrites with earlier segments; a second reverse sweep after the same
+    forward sweep recomputes them and must give the same bits."""
+    g, z = load(ctx, 3)
+    F = GaussianWeightFunctional()
+    xT, zT = ctx.adjoint_forward(0.01, 6, 2)
+    lam_x, lam_zeta = F.cotangent(xT, zT, g.active_area)
+    first = ctx.adjoint_backward(lam_x, lam_zeta)
+    second = ctx.adjoint_backward(lam_x, lam_zeta)
+    assert all(np.array_equal(a, b) for a, b in zip(first, second))
+    _, gz, gx = gradient(ctx, F, g.active_area, 0.01, 6, 1)
+    assert np.array_equal(gz, first[1]) and np.array_equal(gx, first[0])
+
+
 def test_adjoint_leaves_the_resident_state(ctx):
     g, z = load(ctx, 3)
     before = ctx.store_panels()

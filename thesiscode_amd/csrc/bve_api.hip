@@ -97,6 +97,7 @@ struct bve_ctx {
     int adj_n = 0, adj_pad = 0, adj_steps = 0, adj_every = 0, adj_ckpts = 0;
     double adj_dt = 0.0;
     bool adj_ready = false;
+    bool adj_last_stored = false;   // adj_stages holds the last segment's stages
     double *adj_ckpt = nullptr;     // adj_ckpts x 4 rows (x, y, z, zeta) x adj_pad
     Source *adj_stages = nullptr;   // adj_every steps x 4 stages x adj_pad records
     double *adj_lam = nullptr, *adj_lam_acc = nullptr, *adj_ybar = nullptr, *adj_m = nullptr;
@@ -557,6 +558,7 @@ void free_adjoint(bve_ctx *c)
     dev_free(c->adj_pt);
     c->adj_pt_cap = c->adj_ckpt_cap = c->adj_stage_cap = 0;
     c->adj_ready = false;
+    c->adj_last_stored = false;
     c->adj_n = c->adj_pad = 0;
 }
 
@@ -1380,12 +1382,17 @@ int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every
     PairPlan fwd, vjp;
     rc = adjoint_plans(c, fwd, vjp);
     if (rc) return rc;
+    // the last segment's stage inputs are kept: the reverse sweep starts
+    // there and need not recompute them
+    c->adj_last_stored = false;
     for (int k = 0; k < ckpts; ++k) {
+        const bool last = k == ckpts - 1;
         rc = copy_active_state(c, c->adj_ckpt + (size_t)k * 4 * P, true);
         if (!rc) rc = run_steps_on(c, dt, std::min(every, n_steps - k * every), c->n_active, fwd,
-                                   nullptr, 0);
+                                   last ? c->adj_stages : nullptr, last ? P : 0);
         if (rc) return rc;
     }
+    c->adj_last_stored = true;
     c->downloads.clear();
     rc = download_aos(c, c->x0, c->t_stride, 0, c->n_active, x_T);
     if (!rc && zeta_T) rc = host_down(c, zeta_T, c->zeta0, sizeof(double) * c->n_active);
@@ -1507,9 +1514,12 @@ int bve_adjoint_backward(bve_ctx *c, const double *lambda_x, const double *lambd
     for (int seg = c->adj_ckpts - 1; seg >= 0; --seg) {
         const int n0 = seg * c->adj_every;
         const int len = std::min(c->adj_every, c->adj_steps - n0);
-        rc = copy_active_state(c, c->adj_ckpt + (size_t)seg * 4 * P, false);
-        if (!rc) rc = run_steps_on(c, c->adj_dt, len, c->n_active, fwd, c->adj_stages, P);
-        if (rc) return rc;
+        if (!(seg == c->adj_ckpts - 1 && c->adj_last_stored)) {
+            rc = copy_active_state(c, c->adj_ckpt + (size_t)seg * 4 * P, false);
+            if (!rc) rc = run_steps_on(c, c->adj_dt, len, c->n_active, fwd, c->adj_stages, P);
+            if (rc) return rc;
+        }
+        c->adj_last_stored = false;   // the buffer now goes to earlier segments
         for (int l = len - 1; l >= 0; --l) {
             rc = adjoint_reverse_step(c, c->adj_stages + (size_t)l * 4 * P, vjp);
             if (rc) return rc;

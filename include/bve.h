@@ -114,7 +114,9 @@ int bve_get_launch(const bve_ctx *ctx, int *segments, int *grid, int *n_tiles,
 /* InitRK4 + GatherPanels payload (PanelVelocityParallel.f90:236-318): the
  * active panels' positions, relative vorticity and areas and the passive
  * panels' positions are copied to the device and cached there until
- * bve_reset, exactly as the module caches its private panel copies. */
+ * bve_reset, exactly as the module caches its private panel copies.  The
+ * device holds the passive panels in a spatial order of its own (Hilbert
+ * order per cube face); bve_store_panels returns them in the caller's. */
 int bve_load_panels(bve_ctx *ctx, int n_active, const double *x_active,
                     const double *relvort_active, const double *area_active,
                     int n_passive, const double *x_passive);

@@ -266,9 +266,13 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
         // round of resident workgroups and the costlier items
         // (self-exclusion tiles; the mixed kernel's careful tiles, hence its
         // finer split) cost little.  Chunks are never shorter than one tile.
-        // (The float64 segment queue below, tried for the mixed kernel, ran
-        // its launch 2-5% slower at level 7 for the same step time.)
-        const int wg = c->plan_wg ? c->plan_wg : (mixed ? 131072 : 16384);
+        // The mixed kernel's chunks interleave their source tiles, so its
+        // items cost alike; ~64k of them balance the launch, fewer leave a
+        // slow last round (level 7: 4.98 ms at 57k items, 5.13 at 16k) and
+        // more add partial rows for the stage update to read.  (The float64
+        // segment queue below, tried for the mixed kernel, ran its launch
+        // 2-5% slower at level 7 for the same step time.)
+        const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
         return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
     }
     // float64: items cost the same, so a persistent grid of one workgroup

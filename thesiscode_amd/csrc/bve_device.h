@@ -498,12 +498,12 @@ __device__ __forceinline__ void careful_tile(const MixedArgs &a, const MixedLds 
     S64z += (double)Sz - (double)Cz;
 }
 
-// Sources [j_begin, j_end) (whole tiles) for target tile `ttile`: the sums
-// of target i go to out[i - out_off], out[out_stride + i - out_off] and
-// out[2 out_stride + i - out_off].
+// Source tiles first, first + step, ... below end (tile indices) for target
+// tile `ttile`: the sums of target i go to out[i - out_off],
+// out[out_stride + i - out_off] and out[2 out_stride + i - out_off].
 template <int T, bool KAHAN>
-__device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, int j_begin,
-                                                 int j_end, double *out, int out_stride,
+__device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, int first,
+                                                 int step, int end, double *out, int out_stride,
                                                  int out_off, MixedLds &lds)
 {
     constexpr int TILE = kMixedTile;
@@ -538,7 +538,8 @@ __device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, 
     }
     const float rnf2 = (float)(a.rnf * a.rnf);
 
-    for (int jt = j_begin; jt < j_end; jt += TILE) {
+    for (int st = first; st < end; st += step) {
+        const int jt = st * TILE;
         __syncthreads();
         lds.pos[threadIdx.x] = a.src32[jt + threadIdx.x];
         lds.wgt[threadIdx.x] = a.srcw[jt + threadIdx.x];
@@ -587,13 +588,16 @@ __device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, 
 }
 
 // Work item (source chunk `chunk`, target tile `ttile`) of a chunked plan.
+// Chunk c holds the source tiles c, c + n_chunks, c + 2 n_chunks, ...: the
+// careful tiles of a target group (its neighbours, adjacent in the source
+// order) fall into different items, so items cost alike and can be long
+// (few partial rows) without a tail of slow ones.
 template <int T, bool KAHAN>
 __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, int ttile,
                                                 MixedLds &lds)
 {
-    const int j_begin = chunk * a.base.plan.chunk;
-    const int j_end = min(j_begin + a.base.plan.chunk, a.base.n_src_pad);
-    pair_mixed_range<T, KAHAN>(a, ttile, j_begin, j_end,
+    pair_mixed_range<T, KAHAN>(a, ttile, chunk, a.base.plan.n_chunks,
+                               a.base.n_src_pad / kMixedTile,
                                a.base.partial + (size_t)(chunk * 3) * a.base.t_stride,
                                a.base.t_stride, 0, lds);
 }

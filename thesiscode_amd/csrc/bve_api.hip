@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -181,6 +182,17 @@ PairPlan make_segmented_plan(int n_targets, int n_src_pad, int T, int tile, int 
     p.chunk = p.seg_len * tile;
     p.n_chunks = 0;
     p.n_work = p.segments;
+    p.src_step = 1;
+    return p;
+}
+
+// A stride near 0.618 n that is coprime with n (1 for n <= 2): k -> k * step
+// mod n visits every index once, consecutive k far apart.
+int coprime_step(int n)
+{
+    if (n <= 2) return 1;
+    int p = std::max(1, (int)std::lround(0.618 * n));
+    while (std::gcd(p, n) != 1) ++p;
     return p;
 }
 
@@ -261,17 +273,30 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
                                      : choose_targets_per_lane(n_targets));
     // the mixed kernel's tile is its bounding-sphere granularity: fixed at 256
     const int tile = mixed ? 256 : (c->plan_tile ? c->plan_tile : 256);
+    if (mixed && !c->chunked_only && c->plan_chunk == 0) {
+        // Mixed: the float64 pass's segment queue with the source tiles of a
+        // target tile's row visited in stride order, so that each segment
+        // holds a spread of them and the careful ones (neighbours of the
+        // target group) do not pile up in a few slow segments.  28 segments
+        // per resident workgroup: level 7, 4.90 ms per launch and 19.95 ms
+        // per step, against 4.87 / 20.2 at 56 (twice the partial rows) and
+        // 5.12 / 20.7 at 16; chunked with interleaved chunks 5.02 / 20.75.
+        const int slots =
+            pair_mixed_blocks_per_cu(T, c->precision == BVE_PRECISION_MIXED_KAHAN) * c->n_cu;
+        const int segments = c->plan_wg ? c->plan_wg : 28 * (slots > 0 ? slots : 4 * c->n_cu);
+        PairPlan p = make_segmented_plan(n_targets, n_src_pad, T, tile, segments, 1, slots);
+        p.src_step = coprime_step(p.n_src_tiles);
+        return p;
+    }
     if (mixed || c->chunked_only || c->plan_chunk > 0) {
         // Chunked split: many short work items, so the last, partly filled
         // round of resident workgroups and the costlier items
         // (self-exclusion tiles; the mixed kernel's careful tiles, hence its
         // finer split) cost little.  Chunks are never shorter than one tile.
-        // The mixed kernel's chunks interleave their source tiles, so its
-        // items cost alike; ~64k of them balance the launch, fewer leave a
-        // slow last round (level 7: 4.98 ms at 57k items, 5.13 at 16k) and
-        // more add partial rows for the stage update to read.  (The float64
-        // segment queue below, tried for the mixed kernel, ran its launch
-        // 2-5% slower at level 7 for the same step time.)
+        // The mixed kernel's chunks (ensemble members, explicit chunk)
+        // interleave their source tiles, so its items cost alike; ~64k of
+        // them balance the launch, fewer leave a slow last round (level 7:
+        // 4.98 ms at 57k items, 5.13 at 16k).
         const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
         return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
     }

@@ -52,6 +52,11 @@ struct PairPlan {
     int segments;
     int n_src_tiles;        // n_src_pad / tile
     int seg_len;
+    // Mixed segmented pass: item k of a target tile's row is source tile
+    // (k * src_step) mod n_src_tiles (src_step coprime with n_src_tiles), so
+    // the careful tiles of a target group, adjacent in the source order, fall
+    // into different segments.  Float64 passes: 1 (source order).
+    int src_step;
     // Reference-order pass (exact = 1): one target per lane, every source in
     // index order, the reference's own expression and rounding per pair; one
     // chunk, whose rows hold the sums of the cross-product terms themselves

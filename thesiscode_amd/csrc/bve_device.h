@@ -498,13 +498,13 @@ __device__ __forceinline__ void careful_tile(const MixedArgs &a, const MixedLds 
     S64z += (double)Sz - (double)Cz;
 }
 
-// Source tiles first, first + step, ... below end (tile indices) for target
+// Source tiles (first + k step) mod n_src_tiles for k in [k0, k1), for target
 // tile `ttile`: the sums of target i go to out[i - out_off],
 // out[out_stride + i - out_off] and out[2 out_stride + i - out_off].
 template <int T, bool KAHAN>
 __device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, int first,
-                                                 int step, int end, double *out, int out_stride,
-                                                 int out_off, MixedLds &lds)
+                                                 int step, int k0, int k1, double *out,
+                                                 int out_stride, int out_off, MixedLds &lds)
 {
     constexpr int TILE = kMixedTile;
     const int tile_base = ttile * (kBlock * T);
@@ -538,8 +538,9 @@ __device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, 
     }
     const float rnf2 = (float)(a.rnf * a.rnf);
 
-    for (int st = first; st < end; st += step) {
-        const int jt = st * TILE;
+    const int nst = a.base.n_src_pad / TILE;
+    for (int k = k0; k < k1; ++k) {
+        const int jt = (int)(((long long)k * step + first) % nst) * TILE;
         __syncthreads();
         lds.pos[threadIdx.x] = a.src32[jt + threadIdx.x];
         lds.wgt[threadIdx.x] = a.srcw[jt + threadIdx.x];
@@ -596,10 +597,26 @@ template <int T, bool KAHAN>
 __device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, int ttile,
                                                 MixedLds &lds)
 {
-    pair_mixed_range<T, KAHAN>(a, ttile, chunk, a.base.plan.n_chunks,
-                               a.base.n_src_pad / kMixedTile,
+    const int nst = a.base.n_src_pad / kMixedTile;
+    const int n_chunks = a.base.plan.n_chunks;
+    pair_mixed_range<T, KAHAN>(a, ttile, chunk, n_chunks, 0, ceil_div(nst - chunk, n_chunks),
                                a.base.partial + (size_t)(chunk * 3) * a.base.t_stride,
                                a.base.t_stride, 0, lds);
+}
+
+// Persistent body of a segmented mixed pass (the float64 pass's segment
+// queue and partial rows, source tiles in the plan's stride order).
+template <int T, bool KAHAN>
+__device__ __forceinline__ void pair_mixed_segments(const MixedArgs &a, MixedLds &lds,
+                                                    int *seg_slot)
+{
+    constexpr int TT = kBlock * T;
+    run_segments<kMixedTile>(a.base.plan, a.base.queue, seg_slot,
+                             [&](int t, int j0, int j1, int seg) {
+        pair_mixed_range<T, KAHAN>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile,
+                                   j1 / kMixedTile, a.base.partial + (size_t)(seg + t) * 3 * TT,
+                                   TT, t * TT, lds);
+    });
 }
 
 // ---------------------------------------------------------------------------

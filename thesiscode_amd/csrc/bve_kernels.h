@@ -115,6 +115,9 @@ bool plan_supported(int targets_per_lane, int tile);
 // the occupancy query fails).
 int pair_f64_blocks_per_cu(int targets_per_lane, int tile, int recip);
 hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s);
+// Workgroups of the segmented mixed pair kernel one CU holds at once (0 if
+// the occupancy query fails).
+int pair_mixed_blocks_per_cu(int targets_per_lane, int kahan);
 hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s);
 hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s);
 // Reference-order pass (a.plan.exact): bitwise the reference's sums.

@@ -33,16 +33,52 @@ __global__ __launch_bounds__(kBlock) void pair_sum_mixed_kernel(MixedArgs a)
     pair_mixed_item<T, KAHAN>(a, chunk, work - chunk * a.base.plan.n_tiles, lds);
 }
 
+template <int T, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void pair_sum_mixed_seg_kernel(MixedArgs a)
+{
+    __shared__ MixedLds lds;
+    __shared__ int seg;
+    pair_mixed_segments<T, KAHAN>(a, lds, &seg);
+}
+
+template <int T, bool KAHAN>
+static hipError_t launch_mixed_tk(const MixedArgs &a, hipStream_t s)
+{
+    if (a.base.plan.segments > 0) {
+        if (!a.base.queue || a.base.plan.src_step < 1) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((pair_sum_mixed_seg_kernel<T, KAHAN>), dim3(a.base.plan.grid),
+                           dim3(kBlock), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((pair_sum_mixed_kernel<T, KAHAN>), dim3(a.base.plan.grid), dim3(kBlock),
+                           0, s, a);
+    }
+    return hipGetLastError();
+}
+
 template <int T>
 static hipError_t launch_mixed_t(const MixedArgs &a, hipStream_t s)
 {
-    if (a.kahan)
-        hipLaunchKernelGGL((pair_sum_mixed_kernel<T, true>), dim3(a.base.plan.grid), dim3(kBlock),
-                           0, s, a);
-    else
-        hipLaunchKernelGGL((pair_sum_mixed_kernel<T, false>), dim3(a.base.plan.grid), dim3(kBlock),
-                           0, s, a);
-    return hipGetLastError();
+    return a.kahan ? launch_mixed_tk<T, true>(a, s) : launch_mixed_tk<T, false>(a, s);
+}
+
+template <int T, bool KAHAN>
+static int mixed_seg_blocks()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_sum_mixed_seg_kernel<T, KAHAN>,
+                                                     kBlock, 0) != hipSuccess)
+        return 0;
+    return n;
+}
+
+int pair_mixed_blocks_per_cu(int T, int kahan)
+{
+    switch (T) {
+    case 1: return kahan ? mixed_seg_blocks<1, true>() : mixed_seg_blocks<1, false>();
+    case 2: return kahan ? mixed_seg_blocks<2, true>() : mixed_seg_blocks<2, false>();
+    case 4: return kahan ? mixed_seg_blocks<4, true>() : mixed_seg_blocks<4, false>();
+    default: return 0;
+    }
 }
 
 hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s)

@@ -48,7 +48,8 @@ MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4
 def kernel_name(mode, T, tile, recip, segments):
     """Name of the pair kernel instantiation a mode's launch runs."""
     if mode.startswith("mixed"):
-        return f"pair_sum_mixed_kernel<{T},{str(mode == 'mixed_kahan').lower()}>"
+        kind = "pair_sum_mixed_seg_kernel" if segments > 0 else "pair_sum_mixed_kernel"
+        return f"{kind}<{T},{str(mode == 'mixed_kahan').lower()}>"
     if mode == "reference":
         return "pair_sum_exact_kernel"
     kind = "pair_sum_f64_seg_kernel" if segments > 0 else "pair_sum_f64_kernel"

@@ -69,9 +69,11 @@ def candidate_plans(level):
 
 def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, omega=ic.OMEGA_EARTH,
              dt=0.01, device=0, log=None, weights=None):
-    """Time every candidate plan on a homogeneous ensemble of each level (one
-    RK4 step, HIP events around the pair launches) and return
-    {level: {"targets_per_lane", "chunk", "ms_per_stage", "tried"}}.
+    """Time every candidate plan on a homogeneous ensemble of each level
+    (whole RK4 steps: the pair launches and the stage updates, which sum one
+    partial row per chunk, so short chunks cost there what they save in
+    balance) and return {level: {"targets_per_lane", "chunk",
+    "ms_per_stage", "tried"}}.
 
     Members with the same targets per lane share one pair launch, so the
     table gives every level one common value: the T whose best chunks
@@ -89,11 +91,11 @@ def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, ome
                     ens.set_member_plan(m, T, chunk)
                 ens.advance(dt, 1, use_graph=False)
                 ens.synchronize()
-                ens.set_profiling(True)
-                ens.advance(dt, 1, use_graph=False)
-                n, ms = ens.profile()
-                ens.set_profiling(False)
-                per = ms / max(n, 1)
+                steps = 3
+                t0 = time.perf_counter()
+                ens.advance(dt, steps, use_graph=False)
+                ens.synchronize()
+                per = (time.perf_counter() - t0) * 1e3 / (4 * steps)
                 tried.append((T, chunk, per))
         table[L] = dict(tried=tried)
     weights = weights or {L: 1.0 for L in levels}

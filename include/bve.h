@@ -83,10 +83,10 @@ int bve_set_renormalize(bve_ctx *ctx, int on);
 /* Launch geometry of the pair kernel: targets held per lane (1, 2, 4),
  * sources per LDS tile (256, 512), the work split and, for
  * BVE_PRECISION_FP64_FAST, the pairs sharing one reciprocal (2, 4); 0
- * keeps the built-in choice.  The float64 modes run a persistent launch
- * over `target_workgroups` segments of the (target tile, source tile)
- * sequence; the mixed modes run about `target_workgroups` (target tile,
- * source chunk) work items.  Used by the autotuner. */
+ * keeps the built-in choice.  The float64 and mixed modes run a
+ * persistent launch over `target_workgroups` segments of the (target tile,
+ * source tile) sequence (the mixed modes visit each target tile's source
+ * tiles in a stride order).  Used by the autotuner. */
 int bve_set_plan(bve_ctx *ctx, int targets_per_lane, int tile, int target_workgroups,
                  int recip);
 /* chunk > 0: split the sources into chunks of `chunk` sources (rounded up to

@@ -587,6 +587,53 @@ def test_reference_mode_equals_oracle_on_ragged_inputs(ctx, na, npp, npart):
     assert tot == r["total_ke"]
 
 
+@pytest.mark.parametrize("precision,bound", [(_lib.BVE_PRECISION_FP64_FAST, 1e-12),
+                                             (_lib.BVE_PRECISION_MIXED, 1e-5),
+                                             (_lib.BVE_PRECISION_MIXED_KAHAN, 1e-5)])
+@pytest.mark.parametrize("na,npp,npart", [(1, 0, 0), (7, 3, 5), (300, 0, 257), (1029, 511, 1537)])
+def test_fast_modes_on_ragged_inputs(ctx, precision, bound, na, npp, npart):
+    """The shared-reciprocal and mixed modes on the ragged random inputs: one
+    RK4 step of all three target classes against the oracle, positions to
+    the mode's bar relative to the displacement (sizes that fill no tile,
+    segment or target group evenly; one or two source tiles, so the mixed
+    pass's stride order is the identity or a swap).  Random points come
+    closer than grid panels: the closest pair's 1 - x.y, which the oracle
+    and the GPU round in different orders (up to three roundings each in
+    the dot product), carries up to ~3 eps / d_min relative rounding in
+    either, so the float64 bar is max(1e-12, 16 eps / d_min).  The step is
+    short (dt = 1e-5): a particle ~1e-3 from a panel would otherwise see a
+    velocity difference grow by ~dt w / r^2 per RK4 stage (about 100 at
+    dt = 0.01), and positions would measure that amplification rather than
+    the velocity sums."""
+    rng = np.random.default_rng(na + npp + npart)
+    ax, px, qx = random_sphere(na, 1), random_sphere(npp, 2), random_sphere(npart, 3)
+    zv, area = rng.normal(size=na), rng.uniform(0.5, 1.5, na) * 4 * np.pi / max(na, 1)
+    qa = rng.normal(size=npart)
+    ctx.reset()
+    ctx.set_precision(precision)
+    ctx.set_omega(2 * np.pi)
+    ctx.set_smooth(0.05)
+    try:
+        ctx.load_panels(ax, zv, area, px)
+        qx1, _, _, _ = ctx.rk4_step(1e-5, qx, qa)
+        ax1, _, px1 = ctx.store_panels()
+    finally:
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    r = oracle.rk4_step(ax, zv, area, px, qx, qa, 1e-5, 2 * np.pi, 0.05)
+    t = np.concatenate([ax, px, qx])
+    d = 1.0 - t @ ax.T
+    d[np.arange(na), np.arange(na)] = np.inf
+    bound = max(bound, 16 * np.finfo(float).eps / d.min())
+    for got, x0, key in ((ax1, ax, "active_x"), (px1, px, "passive_x"), (qx1, qx, "particles_x")):
+        if len(got):
+            # each class against its own largest displacement: a random
+            # particle may sit far closer to a panel than any two panels
+            disp = max(np.abs(r[key] - x0).max(), 1e-300)
+            err = np.abs(got - r[key]).max() / disp
+            print(f"{key}: {err:.2e} (bar {bound:.1e})")
+            assert err < bound, key
+
+
 @pytest.mark.parametrize("n", [2, 97, 642])
 def test_reference_mode_voronoi_equals_oracle_on_ragged_inputs(ctx, n):
     """The Voronoi step in the reference order on random generators: bit for

@@ -98,21 +98,30 @@ def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, ome
                 per = (time.perf_counter() - t0) * 1e3 / (4 * steps)
                 tried.append((T, chunk, per))
         table[L] = dict(tried=tried)
-    weights = weights or {L: 1.0 for L in levels}
-
-    def best_at(L, T):
-        runs = [r for r in table[L]["tried"] if r[0] == T]
-        return min(runs, key=lambda r: r[2]) if runs else None
-
-    common = [T for T in (1, 2, 4) if all(best_at(L, T) for L in levels)]
-    T_best = min(common, key=lambda T: sum(weights.get(L, 1.0) * best_at(L, T)[2]
-                                           for L in levels))
-    for L in levels:
-        T, chunk, ms = best_at(L, T_best)
+    for L, (T, chunk, ms) in common_plans({L: t["tried"] for L, t in table.items()},
+                                          weights).items():
         table[L].update(targets_per_lane=T, chunk=chunk, ms_per_stage=ms)
         if log:
             log(f"autotune level {L}: T={T} chunk={chunk} {ms:.3f} ms/stage")
     return table
+
+
+def common_plans(tried, weights=None):
+    """From {level: [(T, chunk, ms per stage), ...]} pick the one T that
+    minimises sum_L weights[L] * (fastest ms at that T on level L) among the
+    values tried on every level, and per level its fastest (T, chunk, ms)."""
+    weights = weights or {L: 1.0 for L in tried}
+
+    def best_at(L, T):
+        runs = [r for r in tried[L] if r[0] == T]
+        return min(runs, key=lambda r: r[2]) if runs else None
+
+    common = [T for T in (1, 2, 4) if all(best_at(L, T) for L in tried)]
+    if not common:
+        raise ValueError("no targets-per-lane value was tried on every level")
+    T_best = min(common, key=lambda T: sum(weights.get(L, 1.0) * best_at(L, T)[2]
+                                           for L in tried))
+    return {L: best_at(L, T_best) for L in tried}
 
 
 def save_table(table, path):

@@ -434,7 +434,12 @@ def main():
             "host_roundtrip": roundtrip,
         },
         "roofline": {
-            "bound": "valu",
+            # compute-bound (the contract's "mfma" class, against the FP64
+            # peak of 78.6 TFLOP/s); the pipe is the vector FP64 VALU: the
+            # pair sum is not a contraction and issues no MFMA instruction
+            # (MI355X's FP64 vector and matrix peaks are the same 78.6)
+            "bound": "mfma",
+            "pipe": "vector FP64 VALU (no MFMA instructions)",
             "achieved": r.get("achieved_tflops"),
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",

@@ -8,11 +8,13 @@ the state resident in HBM (bve_advance); pairs are counted as
 4 (N_a (N_a - 1) + N_p N_a) per step -- active and vertex targets -- and the
 parent-panel targets are computed but not counted.
 
-Each arithmetic mode is timed separately; its normwise velocity error is
-measured against the oracle (float64 in the reference's rounding order) on
-~190 active targets (the first 64, the 64 nearest the vortex centre, 64
-random).  ``value`` is the pair rate of the fastest mode whose error is
-<= 1e-10.
+Each arithmetic mode is timed separately from the initial state; its
+normwise velocity error is measured against the oracle (float64 in the
+reference's rounding order) on ~190 targets of each class (active panels,
+parent panels, vertices: the first 64, the 64 nearest the vortex centre, 64
+random), with the resident state's velocities evaluated by the context's
+own plan -- the kernel instantiation the timed steps run.  ``value`` is the
+pair rate of the fastest mode whose error is <= 1e-10.
 
 Also reported: the adjoint gradient on level 6 (cost against the forward
 solve, finite-difference check), a 64-member ensemble in one HIP graph and
@@ -20,12 +22,13 @@ a level-8 two-vortex single step in every arithmetic mode.
 
 Prints one JSON line.  The roofline block uses HIP events recorded by the
 library around every pair-kernel launch (one launch per RK4 stage) at 20
-flops per pair against the MI355X float64 vector peak.  The CPU baseline
-times the reference's own BVERK4 (compiled unmodified into
-oracle/_ref/ref_bench, MPI over 16 host cores) on three level-6 steps;
-the oracle (the reference's loop restated in C, OpenMP over targets) is
-timed beside it (config.cpu_port) and stands in when the reference build is
-absent.
+flops per pair against the MI355X vector peak of the pipe the mode's pair
+loop runs on (float64 78.6, float32 157.3 TFLOP/s).  The CPU baseline times
+the reference's own BVERK4 (compiled unmodified into oracle/_ref/ref_bench,
+MPI over 16 host cores) on one level-7 step, with three level-6 steps
+beside it; the oracle (the reference's loop restated in C, OpenMP over
+targets) is timed too (config.cpu_port) and stands in when the reference
+build is absent.
 """
 import argparse
 import json
@@ -40,6 +43,7 @@ sys.path.insert(0, ROOT)
 
 FLOPS_PER_PAIR = 20.0
 PEAK_FP64_TFLOPS = 78.6      # MI355X float64 vector
+PEAK_FP32_TFLOPS = 157.3     # MI355X float32 vector
 ERR_BAR = 1e-10
 METRIC = "Biot-Savart pair-interactions/s per RK4 step at <=1e-10 rel err; adjoint/fwd ratio"
 MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4}
@@ -95,15 +99,30 @@ def cpu_baseline(grid, zeta, seconds):
         oracle.smooth_velocity(grid.passive_x, x, zeta, a, 0.05)
         stages += 1
         el = time.perf_counter() - t0
+    model, _ = cpu_model()
     return dict(value=stages * pairs_stage / el, unit="pair-interactions/s", cores=threads,
-                kind="port",
+                kind="port", cpu=model,
                 sample=f"{stages} RK4 stage(s) of the oracle's BVEActiveRHS + BVEPassiveRHS + "
                        f"BVESmoothVelocity (reference rounding order, IEEE divisions) at level "
                        f"{grid.level}: {stages * pairs_stage:.4g} pairs in {el:.1f} s, "
-                       f"{threads} OpenMP threads")
+                       f"{threads} OpenMP threads on {model}")
 
 
-def cpu_baseline_reference(level=6, steps=3, ranks=16):
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry) and its logical CPUs."""
+    name = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 1
+
+
+def cpu_baseline_reference(level=7, steps=1, ranks=16, timeout=300):
     """The reference's own BVERK4 (PanelVelocityParallel.f90, compiled
     unmodified with flang -O2 into oracle/_ref/ref_bench) timed with
     MPI_Wtime on `ranks` MPI ranks of the host (its native parallelisation:
@@ -118,7 +137,7 @@ def cpu_baseline_reference(level=6, steps=3, ranks=16):
     ranks = max(1, min(ranks, os.cpu_count() or 1))
     try:
         r = subprocess.run([mpiexec, "-n", str(ranks), exe, str(level), str(steps)],
-                           capture_output=True, text=True, timeout=180)
+                           capture_output=True, text=True, timeout=timeout)
     except subprocess.TimeoutExpired:
         return None
     line = [ln for ln in r.stdout.splitlines() if ln.startswith("ref_bench")]
@@ -129,10 +148,12 @@ def cpu_baseline_reference(level=6, steps=3, ranks=16):
     seconds, pairs = float(f[7]), float(f[8])
     # the metric's pairs (active and vertex targets) per second of the steps
     primary = 4.0 * nsteps * (n_a * (n_a - 1.0) + n_p * n_a)
+    model, logical = cpu_model()
     return dict(value=primary / seconds, unit="pair-interactions/s", cores=nprocs,
-                kind="reference",
+                kind="reference", cpu=model, level=level,
                 sample=f"{nsteps} BVERK4 step(s) of the reference's PanelVelocityParallelModule "
-                       f"(flang -O2, MPICH, {nprocs} ranks) on the level-{level} cubed sphere "
+                       f"(flang -O2, MPICH, {nprocs} MPI ranks, one per core, on {model}, "
+                       f"{logical} logical CPUs visible) on the level-{level} cubed sphere "
                        f"({n_a} active panels, {n_p} vertices, {n_pp} parents): {pairs:.4g} "
                        f"pairs of all classes in {seconds:.1f} s; all-class rate "
                        f"{pairs / seconds:.4g}/s")
@@ -166,14 +187,23 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
         if particles:
             ctx.load_particles(g.particles_x, absv)
 
-    load(z)
-    ctx.advance(dt, 1)
-    ctx.synchronize()
-    load(z)
-    t0 = time.perf_counter()
-    ctx.advance(dt, steps)
-    ctx.synchronize()
-    t_fwd = time.perf_counter() - t0
+    def forward_seconds(passive, particles):
+        """Wall time of `steps` resident RK4 steps (after one warm-up step)
+        with the given target classes beside the active panels."""
+        ctx.reset()
+        ctx.load_panels(g.active_x, z, g.active_area, g.passive_x if passive else np.zeros((0, 3)))
+        if particles:
+            ctx.load_particles(g.particles_x, absv)
+        ctx.advance(dt, 1)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.advance(dt, steps)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    t_fwd = forward_seconds(True, True)             # all three target classes
+    t_fwd_vertices = forward_seconds(False, True)   # active panels + vertices
+    t_fwd_active = forward_seconds(False, False)    # the map J depends on
     load(z, particles=False)
     gradient(ctx, F, g.active_area, dt, 2, 1)
     t0 = time.perf_counter()
@@ -209,8 +239,13 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
         fd_report[f"eps_{eps:g}_plain"] = report(fd)
     ctx.close()
     return dict(workload=f"rh4_cubed_sphere_L{level}_{steps}steps_dt{dt:g}_checkpoint{every}",
-                mode=mode, n_active=g.n_active, J=J, forward_ms=t_fwd * 1e3,
-                gradient_ms=t_adj * 1e3, adjoint_fwd_ratio=t_adj / t_fwd,
+                mode=mode, n_active=g.n_active, J=J, gradient_ms=t_adj * 1e3,
+                forward_ms_active_only=t_fwd_active * 1e3,
+                forward_ms_active_and_vertices=t_fwd_vertices * 1e3,
+                forward_ms_all_classes=t_fwd * 1e3,
+                adjoint_fwd_ratio_same_map=t_adj / t_fwd_active,
+                adjoint_fwd_ratio_active_and_vertices=t_adj / t_fwd_vertices,
+                adjoint_fwd_ratio_all_classes=t_adj / t_fwd,
                 fd_panels=int(n_fd), fd=fd_report)
 
 
@@ -238,37 +273,67 @@ def ensemble_measurement(n_members=64, levels=(4, 5, 6), steps=10, dt=0.01,
                          f"one HIP graph", modes=out)
 
 
-def error_sample(grid, centres, n=64, seed=7):
-    """Indices of the active targets whose velocity error is measured: the
-    first n, the n nearest to each vortex centre (the steepest part of the
-    field) and n seeded random ones."""
-    idx = [np.arange(n)]
+def class_sample(x, centres, n=64, seed=7):
+    """Indices of the sampled targets of one class: the first n, the n
+    nearest to each vortex centre (the steepest part of the field) and n
+    seeded random ones."""
+    idx = [np.arange(min(n, len(x)))]
     for c in centres:
-        idx.append(np.argsort(1.0 - grid.active_x @ c)[:n])
-    idx.append(np.random.default_rng(seed).choice(grid.n_active, n, replace=False))
+        idx.append(np.argsort(1.0 - x @ c)[:n])
+    idx.append(np.random.default_rng(seed).choice(len(x), min(n, len(x)), replace=False))
     return np.unique(np.concatenate(idx))
 
 
-def velocity_error(ctx, grid, za, sample):
-    """Velocity of the current arithmetic mode at the sampled active targets
-    against the oracle (float64, reference rounding order): the sources are
-    permuted so that the sampled panels come first and self-exclusion by
-    index still applies.  Returns (normwise, max pointwise) relative error:
-    max_i |du_i| / max_i |u_i| and max_i |du_i| / |u_i| over the sample."""
-    import oracle
-    perm = np.concatenate([sample, np.setdiff1d(np.arange(grid.n_active), sample)])
-    xs, zs, As = grid.active_x[perm], za[perm], grid.active_area[perm]
-    k = len(sample)
-    u_gpu = ctx.velocity(xs[:k], xs, zs, As, self_end=k)
-    u_ref = oracle.active_rhs(xs, zs, As, 2 * np.pi, 0, k)[0][:k]
-    du = np.linalg.norm(u_gpu - u_ref, axis=1)
-    un = np.linalg.norm(u_ref, axis=1)
-    return float(du.max() / un.max()), float((du / un).max())
+class ErrorSample:
+    """Sampled targets of all three classes and their oracle velocities
+    (float64, the reference's rounding order: BVEActiveRHS with
+    self-exclusion by index, BVESmoothVelocity, BVEPassiveRHS)."""
+
+    def __init__(self, grid, za, centres, smooth=0.05, n=64):
+        import oracle
+        x, a = grid.active_x, grid.active_area
+        self.ia = class_sample(x, centres, n)
+        self.ip = class_sample(grid.passive_x, centres, n)
+        self.iq = class_sample(grid.particles_x, centres, n)
+        self.ua = np.array([oracle.active_rhs(x, za, a, 0.0, i, i + 1)[0][i] for i in self.ia])
+        self.up = oracle.smooth_velocity(grid.passive_x[self.ip], x, za, a, smooth)
+        self.uq = oracle.passive_rhs(grid.particles_x[self.iq], x, za, a)
+
+    def __len__(self):
+        return len(self.ia) + len(self.ip) + len(self.iq)
+
+    def errors(self, ctx):
+        """Velocities of the resident state evaluated by the context's own
+        pass (the plan and kernel a step times, over all targets at once)
+        against the oracle at the sample.  Returns (normwise, max pointwise,
+        per class): normwise max_i |du_i| / max_i |u_i| over the whole
+        sample, pointwise max_i |du_i| / |u_i| over targets with |u_i| >
+        1e-3 max |u|."""
+        ua, up, uq = ctx.resident_velocity()
+        got = np.concatenate([ua[self.ia], up[self.ip], uq[self.iq]])
+        ref = np.concatenate([self.ua, self.up, self.uq])
+        du = np.linalg.norm(got - ref, axis=1)
+        un = np.linalg.norm(ref, axis=1)
+        keep = un > 1e-3 * un.max()
+        k1, k2 = len(self.ia), len(self.ia) + len(self.ip)
+        per = {cls: float(du[sl].max() / un.max())
+               for cls, sl in (("active", slice(0, k1)), ("passive", slice(k1, k2)),
+                               ("particles", slice(k2, None)))}
+        return float(du.max() / un.max()), float((du[keep] / un[keep]).max()), per
 
 
-def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all, sample=None):
+def peak_of(mode):
+    """Vector peak of the pipe a mode's pair loop runs on (TFLOP/s)."""
+    return PEAK_FP32_TFLOPS if mode.startswith("mixed") else PEAK_FP64_TFLOPS
+
+
+def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
+    """Times args.steps RK4 steps of one arithmetic mode from the initial
+    state (reload() puts it back), then measures the mode's error at the
+    initial state with the same plan."""
     from thesiscode_amd import _lib
     ctx.set_precision(MODES[mode])
+    reload()
     ctx.advance(args.dt, args.warmup)
     ctx.synchronize()
     ctx.set_profiling(True)
@@ -285,19 +350,22 @@ def measure(ctx, grid, za, mode, args, pairs_primary, pairs_all, sample=None):
     # source records (32 B each); for the float64 segmented launch at level 7
     # this is 206 MB against 199 MB from rocprofv3 FETCH_SIZE + WRITE_SIZE
     # (copy-calibrated)
-    traffic = 2.0 * launch["partial_bytes"] + 32.0 * grid.n_active
-    # accuracy of this mode at the sampled active targets vs the oracle
-    err, err_point = velocity_error(ctx, grid, za, sample if sample is not None else np.arange(64))
+    traffic = 2.0 * launch["partial_bytes"] + 32.0 * ctx.n_active
+    # accuracy of this mode vs the oracle: the resident state's velocities
+    # from the same plan and kernel as the timed launches
+    reload()
+    err, err_point, err_class = sample.errors(ctx)
     ctx.set_precision(_lib.BVE_PRECISION_FP64)
     avg_ms = kernel_ms / max(launches, 1)
     achieved = pairs_all / 4.0 * FLOPS_PER_PAIR / (avg_ms * 1e-3) / 1e12
     return dict(pair_rate=pairs_primary * args.steps / el,
                 pair_rate_all_classes=pairs_all * args.steps / el,
                 ms_per_step=el * 1e3 / max(args.steps, 1), rel_err_vs_fp64_oracle=err,
-                max_pointwise_rel_err=err_point,
-                error_sample=int(len(sample)) if sample is not None else 64,
+                max_pointwise_rel_err=err_point, rel_err_per_class=err_class,
+                error_sample=f"{len(sample)} targets (active, passive and particle classes), "
+                             f"velocities of the resident state from the timed plan",
                 launches=launches, avg_launch_ms=avg_ms, achieved_tflops=achieved,
-                roofline_frac=achieved / PEAK_FP64_TFLOPS,
+                peak_tflops=peak_of(mode), roofline_frac=achieved / peak_of(mode),
                 plan=dict(targets_per_lane=T, tile=tile, chunk=chunk, n_chunks=n_chunks,
                           recip=recip, **launch),
                 kernel=kernel_name(mode, T, tile, recip, launch["segments"]),
@@ -349,15 +417,18 @@ def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed
     pairs_primary = 4.0 * (na * (na - 1.0) + npart * na)
     pairs_all = 4.0 * (na * (na - 1.0) + (npart + npp) * na)
     lat, lon = np.deg2rad(20.0), np.deg2rad(10.0)
-    sample = error_sample(g, [ic.latlon_to_xyz(lat, lon), ic.latlon_to_xyz(lat, -lon)])
+    sample = ErrorSample(g, za, [ic.latlon_to_xyz(lat, lon), ic.latlon_to_xyz(lat, -lon)], n=48)
     ctx = _lib.Context(0)
     ctx.set_omega(omega)
-    ctx.load_panels(g.active_x, za, g.active_area, g.passive_x)
-    ctx.load_particles(g.particles_x, absv)
+
+    def reload():
+        ctx.load_panels(g.active_x, za, g.active_area, g.passive_x)
+        ctx.load_particles(g.particles_x, absv)
+
     args = types.SimpleNamespace(dt=dt, warmup=1, steps=1)
     out = {}
     for mode in modes:
-        out[mode] = measure(ctx, g, za, mode, args, pairs_primary, pairs_all, sample)
+        out[mode] = measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample)
         print(f"# large {mode}: {out[mode]}", file=sys.stderr, flush=True)
     ctx.close()
     return dict(workload=f"two_vortices_cubed_sphere_L{level}_rk4_step_dt{dt:g}", level=level,
@@ -394,15 +465,19 @@ def main():
     ctx.set_omega(omega)
     if args.plan:
         ctx.set_plan(*[int(v) for v in args.plan.split(",")])
-    ctx.load_panels(g.active_x, za, g.active_area, g.passive_x)
-    ctx.load_particles(g.particles_x, absv)
+
+    def reload():
+        ctx.load_panels(g.active_x, za, g.active_area, g.passive_x)
+        ctx.load_particles(g.particles_x, absv)
+
+    reload()
     print(f"# level {args.level}: {na} active, {npart} particles, {npp} passive", file=sys.stderr,
           flush=True)
 
-    sample = error_sample(g, [ic.latlon_to_xyz(np.pi / 4, 0.0)])
+    sample = ErrorSample(g, za, [ic.latlon_to_xyz(np.pi / 4, 0.0)])
     results = {}
     for mode in [m for m in args.modes.split(",") if m]:
-        results[mode] = measure(ctx, g, za, mode, args, pairs_primary, pairs_all, sample)
+        results[mode] = measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample)
         print(f"# {mode}: {results[mode]}", file=sys.stderr, flush=True)
     roundtrip = host_roundtrip_measurement(ctx, g, absv, args.dt, max(args.steps // 2, 1),
                                            pairs_primary)
@@ -434,14 +509,14 @@ def main():
             "host_roundtrip": roundtrip,
         },
         "roofline": {
-            # compute-bound (the contract's "mfma" class, against the FP64
-            # peak of 78.6 TFLOP/s); the pipe is the vector FP64 VALU: the
-            # pair sum is not a contraction and issues no MFMA instruction
-            # (MI355X's FP64 vector and matrix peaks are the same 78.6)
-            "bound": "mfma",
-            "pipe": "vector FP64 VALU (no MFMA instructions)",
+            # compute-bound on the vector ALU of the mode's precision (the
+            # float64 pair loop keeps the FP64 VALU busy 94-97% of cycles);
+            # the pair sum is not a contraction and issues no MFMA instruction
+            "bound": "vector_fp32" if (best or "").startswith("mixed") else "vector_fp64",
+            "pipe": ("vector FP32 VALU" if (best or "").startswith("mixed") else "vector FP64 VALU")
+                    + " (no MFMA instructions)",
             "achieved": r.get("achieved_tflops"),
-            "peak": PEAK_FP64_TFLOPS,
+            "peak": r.get("peak_tflops", PEAK_FP64_TFLOPS),
             "unit": "TFLOP/s",
             "frac": r.get("roofline_frac"),
             "traffic": r.get("traffic_bytes_per_launch"),
@@ -457,7 +532,9 @@ def main():
     if args.adjoint_level > 0:
         adj = adjoint_measurement(args.adjoint_level)
         print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
-        out["adjoint_fwd_ratio"] = adj["adjoint_fwd_ratio"]
+        # the gradient against a forward of the map it differentiates (the
+        # active panels alone, the same 20 steps and arithmetic)
+        out["adjoint_fwd_ratio"] = adj["adjoint_fwd_ratio_same_map"]
         out["config"]["adjoint"] = adj
     if args.ensemble_members > 0:
         ens = ensemble_measurement(args.ensemble_members)
@@ -468,9 +545,12 @@ def main():
         out["config"]["large_solve"] = big
     if args.cpu_seconds > 0:
         port = cpu_baseline(g, za, args.cpu_seconds)
-        ref = cpu_baseline_reference()
+        # the headline size (level 7, one step: ~20 s on 16 host cores), and
+        # three level-6 steps beside it
+        ref = cpu_baseline_reference(level=args.level, steps=1)
         out["cpu_baseline"] = ref if ref is not None else port
         out["config"]["cpu_port"] = port
+        out["config"]["cpu_reference_level6"] = cpu_baseline_reference(level=6, steps=3)
     print(json.dumps(out), flush=True)
 
 

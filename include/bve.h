@@ -32,7 +32,12 @@ enum {
     BVE_ERR_STATE = 2,    /* call out of order (e.g. step before panels are loaded) */
     BVE_ERR_DEVICE = 3,   /* a HIP call failed */
     BVE_ERR_MEMORY = 4,   /* device allocation failed */
-    BVE_ERR_IO = 5        /* a file could not be opened or written */
+    BVE_ERR_IO = 5,       /* a file could not be opened or written */
+    BVE_ERR_NUMERIC = 6   /* a velocity or position came out NaN or Inf (e.g. a source
+                             and a target at the same point, 1 - x.y = 0, where the
+                             reference divides 0 by 0); reported by the first
+                             synchronising call after the step, with the first
+                             offending target in bve_last_error */
 };
 
 enum {
@@ -190,6 +195,15 @@ int bve_synchronize(bve_ctx *ctx);
 int bve_velocity(bve_ctx *ctx, int n_targets, const double *x_targets, int self_end,
                  int smooth_begin, int smooth_end, int n_sources, const double *x_sources,
                  const double *vort, const double *area, double *u, double *dvort);
+
+/* Velocities of every resident target at the current state (the step-start
+ * positions and vorticity), evaluated by the same launch plan and
+ * arithmetic as a step's pair passes: BVEActiveRHS / BVEPassiveRHS /
+ * BVESmoothVelocity over the cached panels, as BVERK4's first stage
+ * evaluates them (PanelVelocityParallel.f90:465-473, :1287-1412).
+ * u_active (3*n_active), u_passive (3*n_passive, the caller's passive
+ * order) and u_particles (3*n_particles) may each be NULL. */
+int bve_resident_velocity(bve_ctx *ctx, double *u_active, double *u_passive, double *u_particles);
 
 /* ---- Voronoi path (drop-in for VoronoiVelocity{Serial,Parallel}Module) */
 

@@ -1,0 +1,289 @@
+"""GPU parity at the benchmark workloads, on the launch plans a step
+actually runs there (the timed kernels), against the oracle.
+
+* Level-7 Gaussian vortex (98,304 active panels, 32,766 parents, 98,306
+  vertices): the default plans of the float64 shared-reciprocal mode
+  (pair_sum_f64_seg_kernel<4,512,4>) and of the mixed mode
+  (pair_sum_mixed_seg_kernel<2,false>); velocities of the resident state
+  (bve_resident_velocity, the step's own pass) at sampled targets of all
+  three classes, and one RK4 step.
+* Level-8 two vortices (393,216 active panels): one step in both float64
+  modes, sampled velocities against the oracle plus invariants of the full
+  field.
+* Level-6 adjoint (24,576 active panels, 20 steps, checkpoints every 5):
+  gradient against exact central differences and the tangent-linear
+  dot-product identity.
+* 64-member ensemble (levels 4, 5, 6, 10 mixed steps, one HIP graph):
+  members equal their unbatched runs bit for bit, one member per level
+  matches the oracle's step.
+
+Bars (normwise, max_i |du_i| / max_i |u_i| over the sample of a class):
+float64 modes within 1e-12 of the exact sum (__float128) or no further from
+it than 1.5 times the reference's own rounding, whichever is larger, and
+within 2e-12 of the reference's rounding (the oracle).  At levels 7 and 8
+the reference's 1 - x.y for the closest vertex-panel pairs (angle ~0.008 at
+level 7) carries ~2 eps / angle^2 = 7e-12 relative rounding per term, which
+puts the oracle itself ~1e-12 from the exact sum near the vortex core; the
+golden cases (levels 3-5) hold the float64 modes to 1e-12 of the reference.
+Mixed modes: MIXED_BAR against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import normwise_rel
+from thesiscode_amd import _lib
+from thesiscode_amd import grid as gridmod
+from thesiscode_amd import initial_conditions as ic
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+MIXED_BAR = 1e-6      # float32 far field: 3.7e-7 measured at level 7, 2.4e-7 at level 8
+OMEGA = ic.OMEGA_EARTH
+
+
+def class_sample(x, centres, n=64, seed=7):
+    """Indices of a class's sampled targets: the first n, the n nearest each
+    vortex centre and n seeded random ones."""
+    idx = [np.arange(min(n, len(x)))]
+    for c in centres:
+        idx.append(np.argsort(1.0 - x @ c)[:n])
+    idx.append(np.random.default_rng(seed).choice(len(x), min(n, len(x)), replace=False))
+    return np.unique(np.concatenate(idx))
+
+
+def oracle_sample(g, za, smooth, ia, ip, iq):
+    """Oracle velocities (reference rounding order) at sampled active
+    (self-excluded by index), passive (smoothed) and particle targets, and
+    the exact sums there (__float128 brute force)."""
+    x, a = g.active_x, g.active_area
+    ua = np.array([oracle.active_rhs(x, za, a, OMEGA, i, i + 1)[0][i] for i in ia])
+    up = oracle.smooth_velocity(g.passive_x[ip], x, za, a, smooth)
+    uq = oracle.passive_rhs(g.particles_x[iq], x, za, a)
+    w = za * a
+    # sampled active panels first, so that target k excludes source k
+    perm = np.concatenate([ia, np.setdiff1d(np.arange(len(x)), ia)])
+    qa = oracle.velocity_q(x[perm[:len(ia)]], x[perm], w[perm], self_off=0)
+    qp = oracle.velocity_q(g.passive_x[ip], x, w, eps2=smooth * smooth)
+    qq = oracle.velocity_q(g.particles_x[iq], x, w)
+    return (ua, up, uq), (qa, qp, qq)
+
+
+def check_float64(got, ref, exact):
+    """The float64 bars of the module docstring, per class."""
+    for name, u, r, q in zip(("active", "passive", "particles"), got, ref, exact):
+        e_ref, e_ex, ref_ex = normwise_rel(u, r), normwise_rel(u, q), normwise_rel(r, q)
+        print(f"{name}: vs reference {e_ref:.2e}, vs exact {e_ex:.2e} (reference vs exact "
+              f"{ref_ex:.2e})")
+        assert e_ref < 2e-12, name
+        assert e_ex < max(TOL, 1.5 * ref_ex), name
+
+
+@pytest.fixture(scope="module")
+def level7():
+    g = gridmod.cubed_sphere(7)
+    za, zp, _ = ic.gauss_vortex(g, lat0=np.pi / 4, lon0=0.0, width=0.05)
+    c0 = ic.latlon_to_xyz(np.pi / 4, 0.0)
+    ia = class_sample(g.active_x, [c0])
+    ip = class_sample(g.passive_x, [c0])
+    iq = class_sample(g.particles_x, [c0])
+    ref, exact = oracle_sample(g, za, 0.05, ia, ip, iq)
+    absv = zp + 2 * OMEGA * g.particles_x[:, 2]
+    return dict(g=g, za=za, absv=absv, ia=ia, ip=ip, iq=iq, ref=ref, exact=exact)
+
+
+def resident_context(g, za, absv, precision):
+    c = _lib.Context(0)
+    c.set_omega(OMEGA)
+    c.set_precision(precision)
+    c.load_panels(g.active_x, za, g.active_area, g.passive_x)
+    c.load_particles(g.particles_x, absv)
+    return c
+
+
+@pytest.mark.parametrize("precision,plan,bar", [
+    (_lib.BVE_PRECISION_FP64_FAST, (4, 512, 4), TOL),
+    (_lib.BVE_PRECISION_FP64, (4, 512, 1), TOL),
+    (_lib.BVE_PRECISION_MIXED, (2, 256, 1), MIXED_BAR)])
+def test_level7_timed_plan_velocity(level7, precision, plan, bar):
+    """The step's own pair pass at level 7 (all 229,376 targets in one
+    launch, the plan bench.py times) against the oracle on ~190 sampled
+    targets of each class."""
+    g = level7["g"]
+    print("level 7, precision", precision, flush=True)
+    with resident_context(g, level7["za"], level7["absv"], precision) as c:
+        T, tile, _, _, recip = c.plan()
+        assert (T, tile, recip) == plan
+        assert c.launch()["segments"] > 0
+        ua, up, uq = c.resident_velocity()
+    got = (ua[level7["ia"]], up[level7["ip"]], uq[level7["iq"]])
+    if bar == TOL:
+        check_float64(got, level7["ref"], level7["exact"])
+    else:
+        errs = [normwise_rel(u, r) for u, r in zip(got, level7["ref"])]
+        print(f"active {errs[0]:.3e} passive {errs[1]:.3e} particles {errs[2]:.3e}")
+        assert max(errs) < bar
+
+
+@pytest.mark.parametrize("precision,bar", [(_lib.BVE_PRECISION_FP64_FAST, 2 * TOL),
+                                           (_lib.BVE_PRECISION_MIXED, MIXED_BAR)])
+def test_level7_timed_plan_rk4_step(level7, precision, bar):
+    """One RK4 step at level 7 on the timed plan against the oracle's BVERK4
+    (every active panel, the sampled parents and vertices): positions to
+    the bar relative to the step's displacement, vorticity relative to its
+    change, totalKE (float64: 2e-12, the reference's own rounding at the
+    closest pairs, see the module docstring)."""
+    g, za, absv = level7["g"], level7["za"], level7["absv"]
+    ip, iq = level7["ip"], level7["iq"]
+    dt = 0.005
+    print("level 7 step, precision", precision, flush=True)
+    with resident_context(g, za, absv, precision) as c:
+        c.advance(dt, 1)
+        qx, qrel = c.store_particles(want_relvort=True)
+        ax, av, px = c.store_panels()
+        tot, _ = c.kinetic_energy()
+    r = oracle.rk4_step(g.active_x, za, g.active_area, g.passive_x[ip], g.particles_x[iq],
+                        absv[iq], dt, OMEGA, 0.05)
+    disp = np.linalg.norm(r["active_x"] - g.active_x, axis=1).max()
+    errs = dict(active=np.abs(ax - r["active_x"]).max() / disp,
+                passive=np.abs(px[ip] - r["passive_x"]).max() / disp,
+                particles=np.abs(qx[iq] - r["particles_x"]).max() / disp,
+                relvort=np.abs(av - r["active_relvort"]).max() / np.abs(r["active_relvort"] - za).max())
+    print({k: f"{v:.3e}" for k, v in errs.items()})
+    assert max(errs.values()) < bar
+    assert abs(tot - r["total_ke"]) < bar * abs(r["total_ke"])
+    # particles carry absolute vorticity: relVort = absVort - 2 Omega z exactly as formed
+    assert np.array_equal(qrel, absv - 2 * OMEGA * qx[:, 2])
+
+
+@pytest.fixture(scope="module")
+def level8():
+    g = gridmod.cubed_sphere(8)
+    za, zp, _ = ic.two_vortices(g)
+    lat, lon = np.deg2rad(20.0), np.deg2rad(10.0)
+    cs = [ic.latlon_to_xyz(lat, lon), ic.latlon_to_xyz(lat, -lon)]
+    ia = class_sample(g.active_x, cs, n=48)
+    ip = class_sample(g.passive_x, cs, n=48)
+    iq = class_sample(g.particles_x, cs, n=48)
+    ref, exact = oracle_sample(g, za, 0.05, ia, ip, iq)
+    absv = zp + 2 * OMEGA * g.particles_x[:, 2]
+    return dict(g=g, za=za, absv=absv, ia=ia, ip=ip, iq=iq, ref=ref, exact=exact)
+
+
+@pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_FP64, _lib.BVE_PRECISION_FP64_FAST])
+def test_level8_two_vortices_step(level8, precision):
+    """Large single solve (393,216 active panels, 393,218 vertices, 131,070
+    parents) on the plan a step runs there: sampled velocities of every
+    class against the oracle, tangency u.x = 0 and the active-block
+    momentum sum_i w_i u_i = 0 over the full field, and after one RK4 step
+    the absolute-vorticity carry zeta + 2 Omega z of every active panel."""
+    g, za, absv = level8["g"], level8["za"], level8["absv"]
+    dt = 0.002
+    print("level 8, precision", precision, flush=True)
+    with resident_context(g, za, absv, precision) as c:
+        ua, up, uq = c.resident_velocity()
+        c.advance(dt, 1)
+        ax, av, px = c.store_panels()
+    check_float64((ua[level8["ia"]], up[level8["ip"]], uq[level8["iq"]]), level8["ref"],
+                  level8["exact"])
+    umax = np.linalg.norm(ua, axis=1).max()
+    for u, x in ((ua, g.active_x), (up, g.passive_x), (uq, g.particles_x)):
+        assert np.abs(np.sum(u * x, axis=1)).max() < 1e-13 * umax
+    w = za * g.active_area
+    assert np.abs((w[:, None] * ua).sum(axis=0)).max() < 1e-12 * umax * np.abs(w).sum()
+    carry0 = za + 2 * OMEGA * g.active_x[:, 2]
+    carry1 = av + 2 * OMEGA * ax[:, 2]
+    assert np.abs(carry1 - carry0).max() < 1e-12 * np.abs(carry0).max()
+    assert np.linalg.norm(ax - g.active_x, axis=1).max() > 0
+
+
+def test_level6_adjoint_workload():
+    """Adjoint workload: RH4 at level 6, 20 RK4 steps at dt = 0.01,
+    checkpoints every 5, J = sum_i A_i zeta_i(T) g(x_i(T)).  dJ/dzeta_0
+    against exact central differences (eps = 1e-6, each J difference in
+    difference form) on 8 of the 32 seeded panels, to 1e-6; and the
+    tangent-linear identity grad . dzeta = dJ(dzeta) for a random
+    direction (central quotient of exact differences), to 1e-9."""
+    from thesiscode_amd.adjoint import GaussianWeightFunctional, central_difference, gradient
+    g = gridmod.cubed_sphere(6)
+    z = ic.rh4_wave(g.active_x)
+    F = GaussianWeightFunctional()
+    dt, steps, every = 0.01, 20, 5
+    with _lib.Context(0) as c:
+        c.set_omega(OMEGA)
+        c.load_panels(g.active_x, z, g.active_area, g.passive_x)
+        J, gz, gx = gradient(c, F, g.active_area, dt, steps, every)
+        idx = np.random.default_rng(0).choice(g.n_active, 32, replace=False)[:8]
+        fd = np.array([central_difference(c, F, g.active_area, dt, steps, j, 1e-6) for j in idx])
+        rel = np.abs(gz[idx] - fd) / np.abs(fd)
+        print("adjoint vs central differences:", rel.max())
+        assert rel.max() < 1e-6
+        d = np.random.default_rng(1).normal(size=g.n_active) * 1e-6
+        dJ = []
+        for sgn in (1.0, -1.0):
+            xT, zT, dxT, dzT = c.difference_forward(dt, steps, sgn * d)
+            dJ.append(F.difference(xT, zT, dxT, dzT, g.active_area))
+    # central quotient of exact differences: the second-order terms cancel
+    directional = (dJ[0] - dJ[1]) / 2.0
+    pred = float(np.dot(gz, d))
+    print("tangent-linear identity:", abs(pred - directional) / abs(directional))
+    assert abs(pred - directional) < 1e-9 * abs(directional)
+
+
+# the autotuner's mixed-mode choice for these levels (targets per lane,
+# sources per work item), as bench.py measured it
+ENSEMBLE_PLANS = {4: (1, 256), 5: (1, 256), 6: (1, 512)}
+
+
+def alone_run(d, precision, dt, n_steps, plan):
+    """A member's problem in its own context with the member's launch
+    geometry (256-source tiles, `plan` = (targets per lane, chunk))."""
+    g = d["grid"]
+    with _lib.Context(0) as c:
+        c.set_omega(OMEGA)
+        c.set_precision(precision, 0.02)
+        c.set_plan(plan[0], 256, 0, 0)
+        c.set_chunk(plan[1])
+        c.load_panels(g.active_x, d["zeta_active"], g.active_area, g.passive_x)
+        c.load_particles(g.particles_x, d["absvort"])
+        c.advance(dt, n_steps)
+        qx, _ = c.store_particles()
+        ax, av, px = c.store_panels()
+    return dict(active_x=ax, active_relvort=av, passive_x=px, particles_x=qx)
+
+
+def test_ensemble_workload_64_members():
+    """64 members cycling levels 4, 5, 6 (22 / 21 / 21 members), 10 mixed RK4
+    steps at dt = 0.01 in one HIP graph with the autotuner's plans: every
+    member equals its unbatched run bit for bit; one member per level
+    matches the oracle's first step."""
+    from thesiscode_amd.ensemble import load_members, make_members
+    members = make_members(64, levels=(4, 5, 6))
+    dt, n = 0.01, 10
+    prec = _lib.BVE_PRECISION_MIXED
+    with _lib.Ensemble(len(members)) as ens:
+        ens.set_params(OMEGA, 0.05, prec, 0.02)
+        load_members(ens, members)
+        for m, d in enumerate(members):
+            ens.set_member_plan(m, *ENSEMBLE_PLANS[d["level"]])
+        ens.advance(dt, n, use_graph=True)
+        got = [ens.store_member(m) for m in range(len(members))]
+    for m, d in enumerate(members):
+        ref = alone_run(d, prec, dt, n, ENSEMBLE_PLANS[d["level"]])
+        for k in ("active_x", "active_relvort", "passive_x", "particles_x"):
+            assert np.array_equal(got[m][k], ref[k]), (m, k)
+    with _lib.Ensemble(3) as ens:
+        ens.set_params(OMEGA, 0.05, prec, 0.02)
+        load_members(ens, members[:3])
+        for m in range(3):
+            ens.set_member_plan(m, *ENSEMBLE_PLANS[members[m]["level"]])
+        ens.advance(dt, 1, use_graph=True)
+        first = [ens.store_member(m) for m in range(3)]
+    for m in range(3):
+        d, gm = members[m], members[m]["grid"]
+        r = oracle.rk4_step(gm.active_x, d["zeta_active"], gm.active_area, gm.passive_x,
+                            gm.particles_x, d["absvort"], dt, OMEGA, 0.05)
+        disp = np.abs(r["active_x"] - gm.active_x).max()
+        for k in ("active_x", "passive_x", "particles_x"):
+            err = np.abs(first[m][k] - r[k]).max() / disp
+            print(f"level {d['level']} {k}: {err:.2e}")
+            assert err < 10 * MIXED_BAR

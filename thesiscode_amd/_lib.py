@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbve.so")
 
 BVE_OK = 0
+BVE_ERR_NUMERIC = 6
 BVE_PRECISION_FP64 = 0
 BVE_PRECISION_MIXED = 1
 BVE_PRECISION_FP64_FAST = 2
@@ -45,6 +46,7 @@ SIGNATURES = {
     "bve_get_kinetic_energy": (_i, [_p, _p, _p]),
     "bve_synchronize": (_i, [_p]),
     "bve_velocity": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p]),
+    "bve_resident_velocity": (_i, [_p, _p, _p, _p]),
     "bve_voronoi_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p, _p, _p]),
     "bve_difference_forward": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
     "bve_adjoint_forward": (_i, [_p, _d, _i, _i, _p, _p]),
@@ -76,7 +78,11 @@ _lib = None
 
 
 class BVEError(RuntimeError):
-    pass
+    """A libbve call failed; ``code`` is its status (BVE_ERR_*)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load():
@@ -136,7 +142,8 @@ class Context:
 
     def _check(self, rc, what):
         if rc != BVE_OK:
-            raise BVEError(f"{what} failed ({rc}): {self._L.bve_last_error(self._h).decode()}")
+            raise BVEError(f"{what} failed ({rc}): {self._L.bve_last_error(self._h).decode()}",
+                           rc)
 
     # parameters
     def set_omega(self, omega):
@@ -175,6 +182,7 @@ class Context:
         self._check(self._L.bve_load_panels(self._h, na, _ptr(ax), _ptr(av), _ptr(ar), npp, _ptr(px)),
                     "bve_load_panels")
         self.n_active, self.n_passive = na, npp
+        self.n_particles = 0
 
     def rk4_step(self, dt, particles_x, absvort=None, want_ke=True):
         """One step; returns (new particle x, particle relvort or None, totalKE, ke)."""
@@ -186,6 +194,7 @@ class Context:
         ke = np.zeros(self.n_active) if want_ke else None
         self._check(self._L.bve_rk4_step(self._h, dt, n, _ptr(qx), _ptr(qa), _ptr(rel), _ptr(tot),
                                          _ptr(ke)), "bve_rk4_step")
+        self.n_particles = n
         return qx.reshape(n, 3), rel, float(tot[0]), ke
 
     def rk4_step_into(self, dt, particles_x, absvort, relvort, ke):
@@ -199,6 +208,7 @@ class Context:
         tot = np.zeros(1)
         self._check(self._L.bve_rk4_step(self._h, dt, n, _ptr(particles_x), _ptr(absvort),
                                          _ptr(relvort), _ptr(tot), _ptr(ke)), "bve_rk4_step")
+        self.n_particles = n
         return float(tot[0])
 
     def store_panels_into(self, active_x, active_relvort, passive_x):
@@ -216,6 +226,7 @@ class Context:
         qx = _f64(particles_x, 3 * n).copy()
         self._check(self._L.bve_advection_rk4_step(self._h, t, dt, int(divergent), n, _ptr(qx)),
                     "bve_advection_rk4_step")
+        self.n_particles = n
         return qx.reshape(n, 3)
 
     def load_panel_absvort(self, absvort_active):
@@ -233,6 +244,7 @@ class Context:
         self._check(self._L.bve_stratosphere_rk4_step(self._h, t, dt, n, _ptr(qx), _ptr(qa),
                                                       _ptr(rel), _ptr(tot), _ptr(ke)),
                     "bve_stratosphere_rk4_step")
+        self.n_particles = n
         return qx.reshape(n, 3), rel, float(tot[0]), ke
 
     def store_panels(self):
@@ -272,6 +284,16 @@ class Context:
 
     def synchronize(self):
         self._check(self._L.bve_synchronize(self._h), "bve_synchronize")
+
+    def resident_velocity(self):
+        """Velocities (u_active, u_passive, u_particles) of the resident state,
+        evaluated with the context's own launch plan and arithmetic."""
+        ua = np.zeros((self.n_active, 3))
+        up = np.zeros((self.n_passive, 3))
+        uq = np.zeros((self.n_particles, 3))
+        self._check(self._L.bve_resident_velocity(self._h, _ptr(ua), _ptr(up), _ptr(uq)),
+                    "bve_resident_velocity")
+        return ua, up, uq
 
     # single evaluation
     def velocity(self, targets_x, sources_x, vort, area, self_end=0, smooth_range=(0, 0),
@@ -400,7 +422,8 @@ class Ensemble:
 
     def _check(self, rc, what):
         if rc != BVE_OK:
-            raise BVEError(f"{what} failed ({rc}): {self._L.bve_ensemble_last_error(self._h).decode()}")
+            raise BVEError(f"{what} failed ({rc}): {self._L.bve_ensemble_last_error(self._h).decode()}",
+                           rc)
 
     def set_params(self, omega, smooth=0.05, precision=BVE_PRECISION_FP64, nearfield_radius=0.02):
         self._check(self._L.bve_ensemble_set_params(self._h, omega, smooth, precision,

@@ -54,6 +54,9 @@ struct bve_ctx {
     bool chunked_only = false;   // ensemble members: the batched launch runs chunked plans
     int n_cu = 256;              // compute units of the device
     unsigned *queue = nullptr;   // segment queue of the segmented pair pass (2 counters)
+    // numeric status word (device; see flag_nonfinite) and its host copy
+    unsigned long long *status = nullptr;
+    unsigned long long status_host = 0;
 
     // Pinned host arena for the per-step host transfers: uploads are copied
     // in and sent asynchronously; downloads land here and are copied to the
@@ -298,7 +301,9 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
         // them balance the launch, fewer leave a slow last round (level 7:
         // 4.98 ms at 57k items, 5.13 at 16k).
         const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
-        return make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
+        PairPlan p = make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
+        p.crossed = mixed ? 0 : 1;
+        return p;
     }
     // float64: items cost the same, so a persistent grid of one workgroup
     // per resident slot works through ~8 segments each (the queue evens out
@@ -308,7 +313,9 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
     const int tile64 = c->plan_tile ? c->plan_tile : 512;
     const int slots = pair_f64_blocks_per_cu(T, tile64, recip) * c->n_cu;
     const int segments = c->plan_wg ? c->plan_wg : 8 * (slots > 0 ? slots : 2 * c->n_cu);
-    return make_segmented_plan(n_targets, n_src_pad, T, tile64, segments, recip, slots);
+    PairPlan p = make_segmented_plan(n_targets, n_src_pad, T, tile64, segments, recip, slots);
+    p.crossed = 1;
+    return p;
 }
 
 int update_plan(bve_ctx *c)
@@ -395,6 +402,8 @@ StepState step_state(bve_ctx *c)
     st.particle_absvort = c->particles_have_absvort ? c->absvort : nullptr;
     st.particle_relvort = c->relvort;
     st.unweighted = c->precision == BVE_PRECISION_REFERENCE;
+    st.status = c->status;
+    st.member = 0;
     return st;
 }
 
@@ -756,6 +765,50 @@ int host_down(bve_ctx *c, void *host, const void *dev, size_t bytes, const int *
     return BVE_OK;
 }
 
+// Enqueues the download of the numeric status word; the next host_sync
+// completes it and check_status reads it.
+int enqueue_status(bve_ctx *c)
+{
+    c->status_host = 0;
+    return host_down(c, &c->status_host, c->status, sizeof(unsigned long long));
+}
+
+// After host_sync: BVE_ERR_NUMERIC if a kernel flagged a non-finite result
+// since the last check (the word is cleared).  resident: the flagged index
+// is a row of the resident targets [active | passive | particles], named
+// by class and the caller's index; otherwise a target of the call.
+int check_status(bve_ctx *c, const char *what, bool resident)
+{
+    const unsigned long long w = c->status_host;
+    c->status_host = 0;
+    if (w == 0) return BVE_OK;
+    HIP_TRY(c, hipMemsetAsync(c->status, 0, sizeof(unsigned long long), c->stream));
+    const long long i = (long long)(~w & 0xffffffffull);
+    if (!resident)
+        return c->fail(BVE_ERR_NUMERIC, "%s: non-finite velocity at target %lld", what, i);
+    const long long na = c->n_active, np = c->n_passive;
+    if (i < na)
+        return c->fail(BVE_ERR_NUMERIC, "%s: non-finite velocity or position at active panel %lld "
+                       "(a coincident source?)", what, i);
+    if (i < na + np) {
+        const long long k = i - na;
+        const long long panel = k < (long long)c->passive_order.size() ? c->passive_order[k] : k;
+        return c->fail(BVE_ERR_NUMERIC, "%s: non-finite velocity or position at passive panel "
+                       "%lld", what, panel);
+    }
+    return c->fail(BVE_ERR_NUMERIC, "%s: non-finite velocity or position at particle %lld "
+                   "(a coincident source?)", what, i - na - np);
+}
+
+// host_sync plus the numeric status of the work it waited for.
+int host_sync_checked(bve_ctx *c, const char *what)
+{
+    int rc = enqueue_status(c);
+    if (!rc) rc = host_sync(c);
+    if (!rc) rc = check_status(c, what, true);
+    return rc;
+}
+
 // Spatial order of n points (AoS xyz): indices sorted by
 // cube face (axis of the largest |component| and its sign), then by the
 // Hilbert index of the point's gnomonic face coordinates on a 1024 x 1024
@@ -866,9 +919,12 @@ int bve_create(bve_ctx **out, int device)
     // the segment queue starts at zero and every segmented launch leaves it so
     e = hipMalloc(&c->queue, 2 * sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->queue, 0, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&c->status, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->status, 0, sizeof(unsigned long long));
     if (e != hipSuccess) {
         context_free_error() = std::string("bve_create: queue: ") + hipGetErrorString(e);
         if (c->queue) (void)hipFree(c->queue);
+        if (c->status) (void)hipFree(c->status);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return BVE_ERR_MEMORY;
@@ -892,6 +948,7 @@ int bve_destroy(bve_ctx *c)
     dev_free(c->tile_box);
     dev_free(c->group_box);
     if (c->queue) (void)hipFree(c->queue);
+    if (c->status) (void)hipFree(c->status);
     if (c->pinned) (void)hipHostFree(c->pinned);
     for (auto &ev : c->pending) c->event_pool.push_back(ev);
     for (auto &ev : c->event_pool) {
@@ -1113,7 +1170,7 @@ int finish_step(bve_ctx *c, double *x_particles, double *relvort, double *total_
     c->downloads.clear();
     int rc = enqueue_store_particles(c, x_particles, relvort);
     if (!rc) rc = enqueue_kinetic_energy(c, total_ke, ke);
-    if (!rc) rc = host_sync(c);
+    if (!rc) rc = host_sync_checked(c, "RK4 step");
     if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
@@ -1130,7 +1187,7 @@ int bve_store_particles(bve_ctx *c, double *x, double *relvort)
     HIP_TRY(c, hipSetDevice(c->device));
     c->downloads.clear();
     int rc = enqueue_store_particles(c, x, relvort);
-    if (!rc) rc = host_sync(c);
+    if (!rc) rc = host_sync_checked(c, "bve_store_particles");
     if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
@@ -1143,7 +1200,7 @@ int bve_get_kinetic_energy(bve_ctx *c, double *total_ke, double *ke)
     HIP_TRY(c, hipSetDevice(c->device));
     c->downloads.clear();
     int rc = enqueue_kinetic_energy(c, total_ke, ke);
-    if (!rc) rc = host_sync(c);
+    if (!rc) rc = host_sync_checked(c, "bve_get_kinetic_energy");
     if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
@@ -1153,7 +1210,7 @@ int bve_synchronize(bve_ctx *c)
 {
     if (!c) return BVE_ERR_ARGUMENT;
     HIP_TRY(c, hipSetDevice(c->device));
-    const int rc = host_sync(c);
+    const int rc = host_sync_checked(c, "bve_synchronize");
     if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
@@ -1200,7 +1257,7 @@ int bve_advection_rk4_step(bve_ctx *c, double t, double dt, int divergent, int n
     HIP_TRY(c, launch_advection_step(step_state(c), t, dt, divergent, c->stream));
     c->downloads.clear();
     rc = enqueue_store_particles(c, x_particles, nullptr);
-    if (!rc) rc = host_sync(c);
+    if (!rc) rc = host_sync_checked(c, "advection RK4 step");
     return rc;
 }
 
@@ -1238,7 +1295,7 @@ int bve_store_panels(bve_ctx *c, double *x_active, double *relvort, double *x_pa
         rc = download_aos(c, c->x0, c->t_stride, c->n_active, c->n_passive, x_passive,
                           c->passive_order.data());
     if (!rc && relvort) rc = host_down(c, relvort, c->zeta0, sizeof(double) * c->n_active);
-    if (!rc) rc = host_sync(c);
+    if (!rc) rc = host_sync_checked(c, "bve_store_panels");
     return rc;
 }
 
@@ -1331,13 +1388,47 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
     if (rc) { (void)hipStreamSynchronize(c->stream); cleanup(); return rc; }
     if (dvort && self_end > 0) VTRY(dev_alloc(&dv, self_end));
     VTRY(launch_finalize_velocity(c->partial, plan, t, stride, n_targets, c->omega, uu,
-                                  dv, self_end, c->stream));
+                                  dv, self_end, c->status, c->stream));
     VTRY(hipMemcpyAsync(u, uu, 3 * sizeof(double) * n_targets, hipMemcpyDeviceToHost, c->stream));
     if (dv) VTRY(hipMemcpyAsync(dvort, dv, sizeof(double) * self_end, hipMemcpyDeviceToHost, c->stream));
     VTRY(hipStreamSynchronize(c->stream));
+    VTRY(hipMemcpy(&c->status_host, c->status, sizeof(unsigned long long), hipMemcpyDeviceToHost));
 #undef VTRY
     collect_profile(c);
     cleanup();
+    return check_status(c, "bve_velocity", false);
+}
+
+int bve_resident_velocity(bve_ctx *c, double *u_active, double *u_passive, double *u_particles)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!c->panels_loaded)
+        return c->fail(BVE_ERR_STATE, "bve_resident_velocity: no panels loaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->downloads.clear();
+    int rc = host_sync(c);   // earlier uploads and downloads complete
+    if (rc) return rc;
+    const int n = c->n_targets, na = c->n_active, np = c->n_passive;
+    rc = ensure_staging(c, 3 * (size_t)std::max(n, 1));
+    if (rc) return rc;
+    // the step-start state as the first stage's input, then one pass of
+    // the step's own plan
+    const StepState st = step_state(c);
+    HIP_TRY(c, launch_prepare_step(st, c->stream));
+    rc = pair_pass(c, pair_args_of(c, c->plan), n, na);
+    if (rc) return rc;
+    HIP_TRY(c, launch_finalize_velocity(c->partial, c->plan, c->xs, c->t_stride, n, c->omega,
+                                        c->staging, nullptr, 0, c->status, c->stream));
+    rc = host_down(c, u_active, c->staging, 3 * sizeof(double) * na);
+    if (!rc)
+        rc = host_down(c, u_passive, c->staging + 3 * (size_t)na, 3 * sizeof(double) * np,
+                       c->passive_order.data());
+    if (!rc)
+        rc = host_down(c, u_particles, c->staging + 3 * (size_t)(na + np),
+                       3 * sizeof(double) * (size_t)std::max(c->n_particles, 0));
+    if (!rc) rc = host_sync_checked(c, "bve_resident_velocity");
+    if (rc) return rc;
+    collect_profile(c);
     return BVE_OK;
 }
 
@@ -1429,7 +1520,7 @@ int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every
     if (!rc && zeta_T) rc = host_down(c, zeta_T, c->zeta0, sizeof(double) * c->n_active);
     if (rc) return rc;
     rc = copy_active_state(c, c->adj_ckpt, false);   // the resident state is left at t = 0
-    if (!rc) rc = host_sync(c);
+    if (!rc) rc = host_sync_checked(c, "bve_adjoint_forward");
     if (rc) return rc;
     collect_profile(c);
     c->adj_dt = dt;
@@ -1561,7 +1652,7 @@ int bve_adjoint_backward(bve_ctx *c, const double *lambda_x, const double *lambd
     if (!rc && grad_zeta0) rc = host_down(c, grad_zeta0, c->adj_lam + 3 * P, sizeof(double) * n);
     if (rc) return rc;
     rc = copy_active_state(c, c->adj_ckpt, false);
-    if (!rc) rc = host_sync(c);
+    if (!rc) rc = host_sync_checked(c, "bve_adjoint_backward");
     if (rc) return rc;
     collect_profile(c);
     return BVE_OK;
@@ -1675,6 +1766,7 @@ struct bve_ensemble {
     StepState *d_states = nullptr;
     int *d_chunks = nullptr;
     int2 *d_stage_map = nullptr, *d_src_map = nullptr, *d_tgt_map = nullptr;
+    unsigned long long *d_status = nullptr;   // numeric status of every member's stages
     EnsLaunch launch{};
 
     // captured steps
@@ -1740,6 +1832,10 @@ int ens_build(bve_ensemble *e)
 {
     if (!e->dirty) return BVE_OK;
     ens_free_tables(e);
+    if (!e->d_status) {
+        HIP_TRY(e, dev_alloc(&e->d_status, 1));
+        HIP_TRY(e, hipMemsetAsync(e->d_status, 0, sizeof(unsigned long long), e->stream));
+    }
     const bool mixed = e->precision == BVE_PRECISION_MIXED ||
                        e->precision == BVE_PRECISION_MIXED_KAHAN;
     std::vector<PairArgs> pairs;
@@ -1767,6 +1863,8 @@ int ens_build(bve_ensemble *e)
         pairs.push_back(pa);
         mixeds.push_back(mixed_args_of(c, pa, c->n_targets, c->n_active));
         states.push_back(step_state(c));
+        states.back().status = e->d_status;
+        states.back().member = (unsigned)m;
         chunks.push_back(c->plan.n_chunks);
         const int T = c->plan.targets_per_lane;
         const int group = T == 1 ? 0 : (T == 2 ? 1 : 2);
@@ -1885,6 +1983,7 @@ int ens_enqueue(bve_ensemble *e, double dt, int n_steps, bool timed)
             }
             StageParams sp{};
             sp.stage = s;
+            sp.plan.crossed = L.mixed ? 0 : 1;   // float64 rows hold cross-product sums
             sp.dt = dt;
             sp.omega = omega;
             sp.a_next = a_next[s];
@@ -1939,6 +2038,7 @@ int bve_ensemble_destroy(bve_ensemble *e)
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
     ens_free_tables(e);
+    dev_free(e->d_status);
     for (auto &ev : e->pending) e->pool.push_back(ev);
     for (auto &ev : e->pool) {
         (void)hipEventDestroy(ev.first);
@@ -2071,7 +2171,18 @@ int bve_ensemble_synchronize(bve_ensemble *e)
     HIP_TRY(e, hipSetDevice(e->device));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     ens_collect(e);
-    return BVE_OK;
+    if (!e->d_status) return BVE_OK;
+    unsigned long long w = 0;
+    HIP_TRY(e, hipMemcpy(&w, e->d_status, sizeof w, hipMemcpyDeviceToHost));
+    if (w == 0) return BVE_OK;
+    HIP_TRY(e, hipMemset(e->d_status, 0, sizeof w));
+    const unsigned long long key = ~w;
+    const int m = (int)(key >> 32), i = (int)(key & 0xffffffffull);
+    const bve_ctx *c = e->members[m];
+    const char *cls = i < c->n_active ? "active panel row"
+                      : (i < c->n_active + c->n_passive ? "passive panel row" : "particle row");
+    return e->fail(BVE_ERR_NUMERIC, "bve_ensemble: member %d: non-finite velocity or position at "
+                   "target %d (%s)", m, i, cls);
 }
 
 int bve_ensemble_store_member(bve_ensemble *e, int member, double *x_active, double *relvort,

@@ -1,6 +1,7 @@
 // Shared definitions of the Biot-Savart / RK4 HIP library (gfx950).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 
@@ -62,9 +63,33 @@ struct PairPlan {
     // chunk, whose rows hold the sums of the cross-product terms themselves
     // (u = S / (-4 PI)) rather than the factored S = sum (w/d) x.
     int exact;
+    // Float64 pass (crossed = 1): the rows hold sums of x_i x S_t, each
+    // source tile's factored sum crossed with the target before it is added,
+    // so u = rows / (-4 PI) as in the reference-order pass.
+    int crossed;
 };
 
 __host__ __device__ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// Numeric status word of a context (device memory, 0 = clean).  A kernel
+// that produces a non-finite velocity or position records
+// key = (member << 32) | target with atomicMax(status, ~key), so the word
+// ends up holding the complement of the smallest key: the first offending
+// target of the lowest member.  The host reads and clears it at the next
+// synchronisation.
+__device__ __forceinline__ void flag_nonfinite(unsigned long long *status, unsigned member,
+                                               int target)
+{
+    if (status)
+        atomicMax(status, ~(((unsigned long long)member << 32) | (unsigned)target));
+}
+
+__host__ __device__ inline bool finite3(double a, double b, double c)
+{
+    // false for NaN and +-Inf (NaN compares false)
+    return fabs(a) <= 1.7976931348623157e308 && fabs(b) <= 1.7976931348623157e308 &&
+           fabs(c) <= 1.7976931348623157e308;
+}
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // Doubles of partial sums one pass with plan p writes: chunked, 3 rows of

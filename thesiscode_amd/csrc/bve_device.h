@@ -122,8 +122,10 @@ __device__ __forceinline__ void pair_exact_body(const PairArgs &a, double4 *lds_
 }
 
 // Velocity of target (x, y, z) from its reduced sums: u = (x x S) / (-4 PI)
-// for the factored sums, u = S / (-4 PI) for the reference-order pass, whose
-// sums are of the cross-product terms (PanelVelocityParallel.f90:1332).
+// for the factored sums of the mixed pass, u = S / (-4 PI) when the rows hold
+// cross-product sums already (the reference-order pass, whose terms are the
+// reference's own, PanelVelocityParallel.f90:1332, and the float64 pass,
+// which crosses each source tile's sum).
 __device__ __forceinline__ void velocity_from_sums(int exact, double x, double y, double z,
                                                    double Sx, double Sy, double Sz, double &ux,
                                                    double &uy, double &uz)
@@ -230,6 +232,14 @@ __device__ __forceinline__ void tile_f64(const StagedSource *__restrict__ lds, i
 // Target tile ttile against sources [j_begin, j_end) (multiples of TILE) of
 // a float64 pass; the sums of target i go to out[i - out_shift] and the rows
 // out_stride and 2 out_stride after it.  lds holds TILE records.
+//
+// Each source tile's factored sum S_t = sum_j (w_j / d_ij) x_j is crossed
+// with x_i before it joins the target's running sum, which is therefore
+// sum_t x_i x S_t (the rows hold cross-product sums, plan.crossed).  S_t
+// has a large component along x_i for close pairs (w / d with d ~ r^2 / 2),
+// which x_i x S_t removes; carried across tiles it would round every later
+// addition at eps |S| (2e-12 of max |u| at level 8), crossed per tile it
+// rounds only the additions of its own tile.
 template <int T, int TILE, int R>
 __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int j_begin,
                                                int j_end, double *out, int out_stride,
@@ -239,7 +249,7 @@ __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const TargetClasses cls = a.cls;
 
-    double nx[T], ny[T], nz[T], c0[T], sx[T], sy[T], sz[T];
+    double nx[T], ny[T], nz[T], c0[T], ux[T], uy[T], uz[T];
     int self[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
@@ -249,7 +259,7 @@ __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int
         nz[t] = -a.tz[i];
         c0[t] = (i >= cls.smooth_begin && i < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
         self[t] = (i < cls.self_end) ? i : -1;
-        sx[t] = sy[t] = sz[t] = 0.0;
+        ux[t] = uy[t] = uz[t] = 0.0;
     }
 
     for (int jt = j_begin; jt < j_end; jt += TILE) {
@@ -260,18 +270,28 @@ __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int
 #pragma unroll
         for (int t = 0; t < T; ++t)
             diag |= diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, TILE);
+        double sx[T], sy[T], sz[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) sx[t] = sy[t] = sz[t] = 0.0;
         if (diag)
             tile_f64<T, TILE, R, true>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
         else
             tile_f64<T, TILE, R, false>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
+        // u += x_i x S_t with x_i = -n
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            ux[t] = __builtin_fma(nz[t], sy[t], __builtin_fma(-ny[t], sz[t], ux[t]));
+            uy[t] = __builtin_fma(nx[t], sz[t], __builtin_fma(-nz[t], sx[t], uy[t]));
+            uz[t] = __builtin_fma(ny[t], sx[t], __builtin_fma(-nx[t], sy[t], uz[t]));
+        }
     }
 
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const int i = tile_base + t * kBlock + threadIdx.x - out_shift;
-        out[i] = sx[t];
-        out[out_stride + i] = sy[t];
-        out[2 * (size_t)out_stride + i] = sz[t];
+        out[i] = ux[t];
+        out[out_stride + i] = uy[t];
+        out[2 * (size_t)out_stride + i] = uz[t];
     }
 }
 
@@ -620,6 +640,262 @@ __device__ __forceinline__ void pair_mixed_segments(const MixedArgs &a, MixedLds
 }
 
 // ---------------------------------------------------------------------------
+// Bounded mixed-precision pair body: float32 remainders of a float64 tile
+// expansion, float64 where a rounding bound exceeds the error target
+// ---------------------------------------------------------------------------
+//
+// For target x_i and a source tile with float64 centre c write
+// D_ij = c0_i - x_i.x_j = E - delta_j, with E = c0_i - x_i.c and
+// delta_j = x_i.o_j, o_j = x_j - c (the reference's own denominator,
+// PanelVelocityParallel.f90:1320, :1405, split at the tile centre).  Exactly
+//
+//     1/D = 1/E + delta/E^2 + delta^2 / (E^2 D),
+//
+// so the tile's sum in difference form (x_i x x_i = 0) is
+//
+//     sum_j w_j (o_j - a) / D_ij          (a = x_i - c)
+//       = (m - a W0) / E + [(P x_i + M) - a (m.x_i + Q)] / E^2
+//
+// with the float64 tile moments W0 = sum w, m = sum w o, P = sum w o o^T
+// and the float32 remainder sums M = sum (delta^2/D) w o, Q = sum (delta^2/D) w.
+// The remainder is t^2 of the plain term (t = |delta| / E, at most
+// 2R/rho + 2R^2/rho^2 for a tile of radius R whose centre lies rho from
+// the target), so its float32 rounding is t^2 of a float32 pair's.  A
+// (target group, tile) pair runs the float64 pass instead when the tile
+// holds a target's own index, the group's sphere reaches the tile's, t may
+// exceed kBoundedTmax, or the rounding bound 8 eps32 t^2 sum|w| / rho_min of
+// the remainder exceeds tau = 0.3 err_target S, S being the velocity scale
+// max |x_i x S_i| the probe pass finds (a lower bound of the true maximum,
+// so tau errs on the safe side): vortex-core tiles and near tiles run in
+// float64, the background far field in float32.  Four pairs share one
+// v_rcp_f32 of the product of their denominators (all >= (1 - kBoundedTmax) E,
+// so the product stays far from float32 underflow).
+
+constexpr float kBoundedTmax = 0.9f;
+constexpr double kBoundedEps32 = 5.9604644775390625e-08;   // 2^-24
+
+struct BoundedLds {
+    float4 pos[kMixedTile];         // o_j (.w unused)
+    float4 wgt[kMixedTile];         // w_j o_j, w_j
+    StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
+};
+
+// Remainder sums of the source tile in LDS for N targets.
+template <int N>
+__device__ __forceinline__ void remainder_tile(const BoundedLds &lds, const float *xf,
+                                               const float *yf, const float *zf, const float *Ef,
+                                               float *Mx, float *My, float *Mz, float *Q)
+{
+#pragma unroll 2
+    for (int k = 0; k < kMixedTile; k += 4) {
+        float4 s[4], v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            s[r] = lds.pos[k + r];
+            v[r] = lds.wgt[k + r];
+        }
+#pragma unroll
+        for (int t = 0; t < N; ++t) {
+            float dl[4], D[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                dl[r] = __builtin_fmaf(xf[t], s[r].x, __builtin_fmaf(yf[t], s[r].y, zf[t] * s[r].z));
+                D[r] = Ef[t] - dl[r];
+            }
+            const float p01 = D[0] * D[1], p23 = D[2] * D[3];
+            const float rr = __builtin_amdgcn_rcpf(p01 * p23);
+            const float r01 = rr * p23, r23 = rr * p01;
+            float f[4];
+            f[0] = (dl[0] * dl[0]) * (D[1] * r01);
+            f[1] = (dl[1] * dl[1]) * (D[0] * r01);
+            f[2] = (dl[2] * dl[2]) * (D[3] * r23);
+            f[3] = (dl[3] * dl[3]) * (D[2] * r23);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Mx[t] = __builtin_fmaf(f[r], v[r].x, Mx[t]);
+                My[t] = __builtin_fmaf(f[r], v[r].y, My[t]);
+                Mz[t] = __builtin_fmaf(f[r], v[r].z, Mz[t]);
+                Q[t] = __builtin_fmaf(f[r], v[r].w, Q[t]);
+            }
+        }
+    }
+}
+
+// u += x_i x (tile sum) of a remainder tile: its float64 expansion plus the
+// float32 remainder sums.  mom = {W0, m (3), P (xx yy zz xy xz yz), W}.
+__device__ __forceinline__ void bounded_combine(const double *__restrict__ mom, const double4 &sb,
+                                                double xi, double yi, double zi, double E,
+                                                float Mx, float My, float Mz, float Q,
+                                                double &ux, double &uy, double &uz)
+{
+    const double ax = xi - sb.x, ay = yi - sb.y, az = zi - sb.z;
+    const double W0 = mom[0], m0 = mom[1], m1 = mom[2], m2 = mom[3];
+    const double Pxx = mom[4], Pyy = mom[5], Pzz = mom[6], Pxy = mom[7], Pxz = mom[8],
+                 Pyz = mom[9];
+    const double inv = 1.0 / E;
+    const double h = (m0 * xi + m1 * yi + m2 * zi) + (double)Q;
+    const double Bx = ((Pxx * xi + Pxy * yi + Pxz * zi) + (double)Mx) - ax * h;
+    const double By = ((Pxy * xi + Pyy * yi + Pyz * zi) + (double)My) - ay * h;
+    const double Bz = ((Pxz * xi + Pyz * yi + Pzz * zi) + (double)Mz) - az * h;
+    const double Sx = ((m0 - ax * W0) + Bx * inv) * inv;
+    const double Sy = ((m1 - ay * W0) + By * inv) * inv;
+    const double Sz = ((m2 - az * W0) + Bz * inv) * inv;
+    ux += yi * Sz - zi * Sy;
+    uy += zi * Sx - xi * Sz;
+    uz += xi * Sy - yi * Sx;
+}
+
+// Source tiles (first + k step) mod n_src_tiles, k in [k0, k1), for target
+// tile `ttile` of a bounded pass: the sums x_i x S_i of target i go to
+// out[i - out_off] and the rows out_stride, 2 out_stride after it.
+template <int T>
+__device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile, int first,
+                                                   int step, int k0, int k1, double *out,
+                                                   int out_stride, int out_off, BoundedLds &lds,
+                                                   double tau)
+{
+    const int tile_base = ttile * (kBlock * T);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const TargetClasses cls = a.base.cls;
+
+    double xi[T], yi[T], zi[T], c0[T], gx[T], gy[T], gz[T], gr[T], ux[T], uy[T], uz[T];
+    float xf[T], yf[T], zf[T];
+    int self[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        xi[t] = a.base.tx[i];
+        yi[t] = a.base.ty[i];
+        zi[t] = a.base.tz[i];
+        xf[t] = (float)xi[t];
+        yf[t] = (float)yi[t];
+        zf[t] = (float)zi[t];
+        c0[t] = (i >= cls.smooth_begin && i < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
+        self[t] = (i < cls.self_end) ? i : -1;
+        const double4 gb = a.group_box[(tile_base + t * kBlock) / kGroup + wave];
+        gx[t] = gb.x;
+        gy[t] = gb.y;
+        gz[t] = gb.z;
+        gr[t] = gb.w;
+        ux[t] = uy[t] = uz[t] = 0.0;
+    }
+
+    const int nst = a.base.n_src_pad / kMixedTile;
+    for (int k = k0; k < k1; ++k) {
+        const int tl = (int)(((long long)k * step + first) % nst);
+        const int jt = tl * kMixedTile;
+        __syncthreads();
+        lds.pos[threadIdx.x] = a.src32[jt + threadIdx.x];
+        lds.wgt[threadIdx.x] = a.srcw[jt + threadIdx.x];
+        const double4 sb = a.tile_box[tl];
+        const double *mom = a.tile_mom + (size_t)tl * kTileMoments;
+        const double W = mom[10];
+        bool f64[T];
+        bool any = false;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            // wave-uniform: one target group per wave and t
+            const double dx = gx[t] - sb.x, dy = gy[t] - sb.y, dz = gz[t] - sb.z;
+            const double rho_c = sqrt(dx * dx + dy * dy + dz * dz) - gr[t];
+            const double R = sb.w;
+            bool use64 = diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt,
+                                   kMixedTile) || !(rho_c > R);
+            if (!use64) {
+                const double q = R / rho_c;
+                const double tb = 2.0 * q * (1.0 + q);
+                use64 = tb >= (double)kBoundedTmax ||
+                        8.0 * kBoundedEps32 * tb * tb * W > tau * (rho_c - R);
+            }
+            f64[t] = use64;
+            any |= use64;
+        }
+        if (__syncthreads_or(any ? 1 : 0)) {
+            stage_sources(lds.f64, a.base.src + jt, kMixedTile);
+            __syncthreads();
+        }
+        if (!any) {
+            float Mx[T], My[T], Mz[T], Q[T], Ef[T];
+            double E[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                E[t] = c0[t] - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
+                Ef[t] = (float)E[t];
+                Mx[t] = My[t] = Mz[t] = Q[t] = 0.f;
+            }
+            remainder_tile<T>(lds, xf, yf, zf, Ef, Mx, My, Mz, Q);
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+                bounded_combine(mom, sb, xi[t], yi[t], zi[t], E[t], Mx[t], My[t], Mz[t], Q[t],
+                                ux[t], uy[t], uz[t]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                if (f64[t]) {
+                    double nx[1] = {-xi[t]}, ny[1] = {-yi[t]}, nz[1] = {-zi[t]};
+                    double cc[1] = {c0[t]}, sx[1] = {0.0}, sy[1] = {0.0}, sz[1] = {0.0};
+                    int sf[1] = {self[t]};
+                    if (diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, kMixedTile))
+                        tile_f64<1, kMixedTile, 4, true>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
+                    else
+                        tile_f64<1, kMixedTile, 4, false>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
+                    ux[t] += yi[t] * sz[0] - zi[t] * sy[0];
+                    uy[t] += zi[t] * sx[0] - xi[t] * sz[0];
+                    uz[t] += xi[t] * sy[0] - yi[t] * sx[0];
+                } else {
+                    const double E = c0[t] - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
+                    float Ef[1] = {(float)E}, Mx[1] = {0.f}, My[1] = {0.f}, Mz[1] = {0.f},
+                          Q[1] = {0.f};
+                    remainder_tile<1>(lds, &xf[t], &yf[t], &zf[t], Ef, Mx, My, Mz, Q);
+                    bounded_combine(mom, sb, xi[t], yi[t], zi[t], E, Mx[0], My[0], Mz[0], Q[0],
+                                    ux[t], uy[t], uz[t]);
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int i = tile_base + t * kBlock + threadIdx.x - out_off;
+        out[i] = ux[t];
+        out[out_stride + i] = uy[t];
+        out[2 * (size_t)out_stride + i] = uz[t];
+    }
+}
+
+// Error threshold tau of a bounded pass from its velocity scale.
+__device__ __forceinline__ double bounded_tau(const MixedArgs &a)
+{
+    return 0.3 * a.err_target * *a.scale;
+}
+
+// Work item (source chunk `chunk`, target tile `ttile`) of a chunked bounded
+// plan; chunk c holds the source tiles c, c + n_chunks, ...
+template <int T>
+__device__ __forceinline__ void pair_bounded_item(const MixedArgs &a, int chunk, int ttile,
+                                                  BoundedLds &lds)
+{
+    const int nst = a.base.n_src_pad / kMixedTile;
+    const int n_chunks = a.base.plan.n_chunks;
+    pair_bounded_range<T>(a, ttile, chunk, n_chunks, 0, ceil_div(nst - chunk, n_chunks),
+                          a.base.partial + (size_t)(chunk * 3) * a.base.t_stride,
+                          a.base.t_stride, 0, lds, bounded_tau(a));
+}
+
+// Persistent body of a segmented bounded pass (source tiles in stride order).
+template <int T>
+__device__ __forceinline__ void pair_bounded_segments(const MixedArgs &a, BoundedLds &lds,
+                                                      int *seg_slot)
+{
+    constexpr int TT = kBlock * T;
+    const double tau = bounded_tau(a);
+    run_segments<kMixedTile>(a.base.plan, a.base.queue, seg_slot,
+                             [&](int t, int j0, int j1, int seg) {
+        pair_bounded_range<T>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile, j1 / kMixedTile,
+                              a.base.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds, tau);
+    });
+}
+
+// ---------------------------------------------------------------------------
 // Block reductions (fixed order: deterministic)
 // ---------------------------------------------------------------------------
 
@@ -691,8 +967,10 @@ __device__ __forceinline__ void mixed_target_prep_body(const PairArgs &b, int n_
                                                        double *sh)
 {
     const int i = blk * 256 + threadIdx.x;
-    const bool valid = i < n_targets;
     const double x = b.tx[i], y = b.ty[i], z = b.tz[i];
+    // a non-finite target (already reported) is left out of its group's
+    // sphere, so that it does not spoil its neighbours' sums
+    const bool valid = i < n_targets && finite3(x, y, z);
     const int lane = threadIdx.x & 63, base = threadIdx.x & ~63;
     double v[4] = {valid ? 1.0 : 0.0, valid ? x : 0.0, valid ? y : 0.0, valid ? z : 0.0};
     double m[4];
@@ -717,6 +995,63 @@ __device__ __forceinline__ void mixed_target_prep_body(const PairArgs &b, int n_
     }
     if (lane == 0) group_box[i / kGroup] = make_double4(cx, cy, cz, sh[base]);
     __syncthreads();
+}
+
+// Tile moments of the bounded pass (kMixedTile threads per tile, after
+// mixed_source_prep_body has written the tile centres): W0 = sum w,
+// m = sum w o, P = sum w o o^T, W = sum |w| with o = x - c in float64.
+__device__ __forceinline__ void bounded_moment_prep_body(const Source *__restrict__ src,
+                                                         int n_sources,
+                                                         const double4 *__restrict__ tile_box,
+                                                         double *__restrict__ mom, int tile,
+                                                         double *sh)
+{
+    const int j = tile * kMixedTile + threadIdx.x;
+    const Source s = src[j];
+    const double w = j < n_sources ? s.w : 0.0;
+    const double4 c = tile_box[tile];
+    const double ox = s.x - c.x, oy = s.y - c.y, oz = s.z - c.z;
+    const double v[11] = {w, w * ox, w * oy, w * oz, w * ox * ox, w * oy * oy, w * oz * oz,
+                          w * ox * oy, w * ox * oz, w * oy * oz, fabs(w)};
+    for (int q = 0; q < 11; ++q) {
+        const double b = block_sum(v[q], sh);
+        if (threadIdx.x == 0) mom[(size_t)tile * kTileMoments + q] = b;
+    }
+    if (threadIdx.x == 0) mom[(size_t)tile * kTileMoments + 11] = 0.0;
+}
+
+// Velocity-scale probe of the bounded pass: for target row `row` (the
+// first of a target group), |x_i x S_i| over every source in float64 (the
+// self pair excluded), combined into *scale with atomicMax on the bits of a
+// non-negative double.  One workgroup of 256 threads per probe target.
+__device__ __forceinline__ void bounded_probe_body(const PairArgs &b, int n_targets,
+                                                   int n_sources, int row,
+                                                   unsigned long long *scale_bits, double *sh)
+{
+    if (row >= n_targets) return;
+    const double x = b.tx[row], y = b.ty[row], z = b.tz[row];
+    if (!finite3(x, y, z)) return;
+    const TargetClasses cls = b.cls;
+    const double c0 = (row >= cls.smooth_begin && row < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
+    const int self = row < cls.self_end ? row : -1;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int j = threadIdx.x; j < n_sources; j += 256) {
+        if (j == self) continue;
+        const Source s = b.src[j];
+        const double d = c0 - ((x * s.x + y * s.y) + z * s.z);
+        if (!(d > 0.0)) continue;
+        const double q = s.w / d;
+        sx += q * s.x;
+        sy += q * s.y;
+        sz += q * s.z;
+    }
+    const double Sx = block_sum(sx, sh), Sy = block_sum(sy, sh), Sz = block_sum(sz, sh);
+    if (threadIdx.x == 0) {
+        const double ux = y * Sz - z * Sy, uy = z * Sx - x * Sz, uz = x * Sy - y * Sx;
+        const double m = sqrt(ux * ux + uy * uy + uz * uz);
+        if (m <= 1.7976931348623157e308)
+            atomicMax(scale_bits, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -827,6 +1162,7 @@ __device__ __forceinline__ void advection_step_body(const StepState &st, int i, 
     double q[3];
     for (int c = 0; c < 3; ++c)
         q[c] = p[c] + k[0][c] / 6.0 + k[1][c] / 3.0 + k[2][c] / 3.0 + k[3][c] / 6.0;
+    if (!finite3(q[0], q[1], q[2])) flag_nonfinite(st.status, st.member, i);
     for (int c = 0; c < 3; ++c) {
         st.x0[c * stride + i] = q[c];
         st.xs[c * stride + i] = q[c];
@@ -879,7 +1215,10 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
         const double x = st.xs[i], y = st.xs[stride + i], z = st.xs[2 * stride + i];
         // u = sum_j w_j (x_i x x_j)/d_ij / (-4 PI)    (PanelVelocityParallel.f90:1332)
         double ux, uy, uz;
-        velocity_from_sums(sp.plan.exact, x, y, z, Sx, Sy, Sz, ux, uy, uz);
+        velocity_from_sums(sp.plan.exact | sp.plan.crossed, x, y, z, Sx, Sy, Sz, ux, uy, uz);
+        // a coincident source and target (1 - x.y = 0) makes the sums NaN or
+        // Inf, as the reference's 0/0 does: reported, not silently carried
+        if (!finite3(ux, uy, uz)) flag_nonfinite(st.status, st.member, i);
         const double kx = sp.dt * ux, ky = sp.dt * uy, kz = sp.dt * uz;
         const double *base = (sp.stage == 0) ? st.x0 : st.acc;
         // x + k1/6 + k2/3 + k3/3 + k4/6, left to right (:637-644)

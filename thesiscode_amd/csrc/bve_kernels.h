@@ -38,7 +38,17 @@ struct MixedArgs {
     double4 *group_box;   // per 64 targets: centre, radius
     double rnf;           // near-field radius (float64 below it)
     int kahan;            // 1: Kahan-compensated float32 sums
+    // Bounded mode (BVE_PRECISION_MIXED_BOUNDED): per 256-source tile the
+    // float64 moments {W0, m (3), P (6), W, pad} of kTileMoments doubles
+    // (W0 = sum w, m = sum w o, P = sum w o o^T as xx yy zz xy xz yz,
+    // W = sum |w|, o = x - tile centre), the velocity scale max |x x S| the
+    // probe pass finds, and the error target relative to it.
+    const double *tile_mom;
+    double *scale;
+    double err_target;
 };
+
+constexpr int kTileMoments = 12;
 
 // Record of one active panel in the adjoint's pair pass: position, weight
 // w = zeta A and n = m x x (m its velocity cotangent); 64 B.
@@ -98,6 +108,9 @@ struct StepState {
     // 1: source records carry the vorticity itself (w = zeta; the reference-
     // order pass multiplies by the area term by term), 0: w = zeta * A
     int unweighted;
+    // numeric status word (may be null) and the member index its keys carry
+    unsigned long long *status;
+    unsigned member;
 };
 
 struct StageParams {
@@ -120,6 +133,11 @@ hipError_t launch_pair_sum_f64(const PairArgs &a, hipStream_t s);
 int pair_mixed_blocks_per_cu(int targets_per_lane, int kahan);
 hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s);
 hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s);
+// Bounded mixed mode: preparation (records, tile moments, group spheres,
+// velocity-scale probe) and the pair pass.
+hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s);
+hipError_t launch_pair_sum_bounded(const MixedArgs &a, hipStream_t s);
+int pair_bounded_blocks_per_cu(int targets_per_lane);
 // Reference-order pass (a.plan.exact): bitwise the reference's sums.
 hipError_t launch_pair_sum_exact(const PairArgs &a, hipStream_t s);
 hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s);
@@ -137,7 +155,7 @@ hipError_t launch_make_sources(const double *xyz, const double *vort, const doub
                                int unweighted, Source *src, hipStream_t s);
 hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
-                                    int n_dvort, hipStream_t s);
+                                    int n_dvort, unsigned long long *status, hipStream_t s);
 hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_t s);
 
 // Exact-difference run of the active panels (bve_difference_forward): the

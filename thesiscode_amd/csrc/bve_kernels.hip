@@ -267,7 +267,8 @@ hipError_t launch_make_sources(const double *xyz, const double *vort, const doub
 __global__ void finalize_velocity_kernel(const double *__restrict__ partial, PairPlan plan,
                                          const double *__restrict__ t, int stride, int n,
                                          double omega, double *__restrict__ u,
-                                         double *__restrict__ dvort, int n_dvort)
+                                         double *__restrict__ dvort, int n_dvort,
+                                         unsigned long long *status)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -276,7 +277,8 @@ __global__ void finalize_velocity_kernel(const double *__restrict__ partial, Pai
     reduce_partials(partial, plan, stride, i, Sx, Sy, Sz);
     const double x = t[i], y = t[stride + i], z = t[2 * stride + i];
     double ux, uy, uz;
-    velocity_from_sums(plan.exact, x, y, z, Sx, Sy, Sz, ux, uy, uz);
+    velocity_from_sums(plan.exact | plan.crossed, x, y, z, Sx, Sy, Sz, ux, uy, uz);
+    if (!finite3(ux, uy, uz)) flag_nonfinite(status, 0, i);
     u[3 * i] = ux;
     u[3 * i + 1] = uy;
     u[3 * i + 2] = uz;
@@ -302,11 +304,11 @@ hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_
 
 hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
-                                    int n_dvort, hipStream_t s)
+                                    int n_dvort, unsigned long long *status, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(finalize_velocity_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, partial,
-                       plan, t, stride, n, omega, u, dvort, n_dvort);
+                       plan, t, stride, n, omega, u, dvort, n_dvort, status);
     return hipGetLastError();
 }
 

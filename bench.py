@@ -46,11 +46,15 @@ PEAK_FP64_TFLOPS = 78.6      # MI355X float64 vector
 PEAK_FP32_TFLOPS = 157.3     # MI355X float32 vector
 ERR_BAR = 1e-10
 METRIC = "Biot-Savart pair-interactions/s per RK4 step at <=1e-10 rel err; adjoint/fwd ratio"
-MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4}
+MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4,
+         "mixed_bounded": 5}
 
 
 def kernel_name(mode, T, tile, recip, segments):
     """Name of the pair kernel instantiation a mode's launch runs."""
+    if mode == "mixed_bounded":
+        kind = "pair_sum_bounded_seg_kernel" if segments > 0 else "pair_sum_bounded_kernel"
+        return f"{kind}<{T}>"
     if mode.startswith("mixed"):
         kind = "pair_sum_mixed_seg_kernel" if segments > 0 else "pair_sum_mixed_kernel"
         return f"{kind}<{T},{str(mode == 'mixed_kahan').lower()}>"
@@ -399,7 +403,7 @@ def host_roundtrip_measurement(ctx, grid, absv, dt, steps, pairs_primary, mode="
                 host_bytes_up_per_step=up, host_bytes_down_per_step=down, steps=steps)
 
 
-def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed_kahan"),
+def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed_kahan", "mixed_bounded"),
                             dt=0.002):
     """Large single solve: two co-rotating Gaussian vortices (amplitude 4 PI,
     width 0.1, at 20N +-10E, minus the mean) on the level-8 cubed sphere

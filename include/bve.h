@@ -46,11 +46,17 @@ enum {
                                      float32 sums of each 256-source tile carried in float64 */
     BVE_PRECISION_FP64_FAST = 2,  /* float64 pairs, one reciprocal shared by 2 or 4 pairs */
     BVE_PRECISION_MIXED_KAHAN = 3, /* as MIXED with Kahan-compensated float32 sums */
-    BVE_PRECISION_REFERENCE = 4   /* float64 in the reference's own order and rounding: one
+    BVE_PRECISION_REFERENCE = 4,  /* float64 in the reference's own order and rounding: one
                                      target per lane, sources in index order, each term
                                      ((x_i x x_k) zeta_k) A_k / d_ik with IEEE divisions, no
                                      FMA contraction; BVERK4 results (panel and Voronoi) equal
                                      the reference's bit for bit */
+    BVE_PRECISION_MIXED_BOUNDED = 5 /* float64 tile expansions with float32 remainders: each
+                                     256-source tile's sum is its second-order float64
+                                     expansion about the tile centre plus a float32 sum of
+                                     the remainders delta^2 / (E^2 D); a (target group, tile)
+                                     pair whose remainder rounding could exceed the error
+                                     target (bve_set_error_target) runs in float64 instead */
 };
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -81,6 +87,11 @@ int bve_get_smooth(const bve_ctx *ctx, double *smooth);
  * re-triangulation, which is sensitive to the last bits of the positions).
  * No reference counterpart. */
 int bve_set_precision(bve_ctx *ctx, int precision, double nearfield_radius);
+/* Normwise error target of BVE_PRECISION_MIXED_BOUNDED, relative to the
+ * largest velocity of the pass (default 1e-10): each float32 remainder tile
+ * is held to 0.3 of it times that velocity by a rounding bound, the rest
+ * runs in float64.  No reference counterpart. */
+int bve_set_error_target(bve_ctx *ctx, double rel_err);
 /* 1 = project every position back onto the unit sphere after each step
  * (RenormalizeGrid, ParticlesEdgesPanels2.f90:1690-1702, applied on the
  * device); 0 (default) = the reference's BVERK4, which never does. */

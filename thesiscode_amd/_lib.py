@@ -18,6 +18,7 @@ BVE_PRECISION_MIXED = 1
 BVE_PRECISION_FP64_FAST = 2
 BVE_PRECISION_MIXED_KAHAN = 3
 BVE_PRECISION_REFERENCE = 4
+BVE_PRECISION_MIXED_BOUNDED = 5
 
 # every symbol include/bve.h declares, with its ctypes signature
 _d, _i, _p, _ll = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong
@@ -31,6 +32,7 @@ SIGNATURES = {
     "bve_get_smooth": (_i, [_p, _dp]),
     "bve_set_precision": (_i, [_p, _i, _d]),
     "bve_set_renormalize": (_i, [_p, _i]),
+    "bve_set_error_target": (_i, [_p, _d]),
     "bve_set_plan": (_i, [_p, _i, _i, _i, _i]),
     "bve_set_chunk": (_i, [_p, _i]),
     "bve_load_panels": (_i, [_p, _i, _p, _p, _p, _i, _p]),
@@ -160,6 +162,9 @@ class Context:
     def set_precision(self, precision, nearfield_radius=0.02):
         self._check(self._L.bve_set_precision(self._h, precision, nearfield_radius),
                     "bve_set_precision")
+
+    def set_error_target(self, rel_err):
+        self._check(self._L.bve_set_error_target(self._h, rel_err), "bve_set_error_target")
 
     def set_renormalize(self, on):
         self._check(self._L.bve_set_renormalize(self._h, int(bool(on))), "bve_set_renormalize")

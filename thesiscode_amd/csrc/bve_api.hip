@@ -47,6 +47,7 @@ struct bve_ctx {
     double smooth = 0.05;
     int precision = BVE_PRECISION_FP64;
     double nearfield = 0.02;
+    double err_target = 1e-10;   // BVE_PRECISION_MIXED_BOUNDED
     int renormalize = 0;
     int voronoi = 0;   // 1 while a Voronoi step runs (its module's denominator form)
     int plan_t = 0, plan_tile = 0, plan_wg = 0, plan_recip = 0;
@@ -92,7 +93,9 @@ struct bve_ctx {
     float4 *srcw = nullptr;
     float *srcc = nullptr;
     double4 *tile_box = nullptr, *group_box = nullptr;
-    int mixed_src_cap = 0, mixed_tgt_cap = 0;
+    double *tile_mom = nullptr;   // bounded mode: kTileMoments per 256-source tile
+    double *scale = nullptr;      // bounded mode: velocity scale of the current pass
+    int mixed_src_cap = 0, mixed_tgt_cap = 0, mixed_src_cap_mom = 0;
     size_t staging_cap = 0;      // doubles
     bool particles_have_absvort = false;
     PairPlan plan{};
@@ -266,6 +269,24 @@ PairPlan make_exact_plan(int n_targets, int n_src_pad)
 PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
 {
     if (c->precision == BVE_PRECISION_REFERENCE) return make_exact_plan(n_targets, n_src_pad);
+    if (c->precision == BVE_PRECISION_MIXED_BOUNDED) {
+        // the mixed pass's geometry: 256-source tiles (the expansion and
+        // error-bound granularity), stride-ordered segment queue, or
+        // interleaved chunks when a chunk is set; rows hold x_i x S sums
+        const int T = c->plan_t ? c->plan_t : (n_targets >= 131072 ? 2 : 1);
+        PairPlan p;
+        if (c->plan_chunk > 0 || c->chunked_only) {
+            p = make_plan(n_targets, n_src_pad, T, 256, c->plan_wg ? c->plan_wg : 65536, 1,
+                          c->plan_chunk);
+        } else {
+            const int slots = pair_bounded_blocks_per_cu(T) * c->n_cu;
+            const int segments = c->plan_wg ? c->plan_wg : 28 * (slots > 0 ? slots : 4 * c->n_cu);
+            p = make_segmented_plan(n_targets, n_src_pad, T, 256, segments, 1, slots);
+            p.src_step = coprime_step(p.n_src_tiles);
+        }
+        p.crossed = 1;
+        return p;
+    }
     const bool mixed = c->precision == BVE_PRECISION_MIXED ||
                        c->precision == BVE_PRECISION_MIXED_KAHAN;
     // mixed: 2 targets per lane from ~130k targets (level 7: 5.3 against
@@ -439,6 +460,8 @@ bool is_mixed(const bve_ctx *c)
     return c->precision == BVE_PRECISION_MIXED || c->precision == BVE_PRECISION_MIXED_KAHAN;
 }
 
+bool is_bounded(const bve_ctx *c) { return c->precision == BVE_PRECISION_MIXED_BOUNDED; }
+
 // Buffers of the mixed-precision pass for n_src_pad sources and t_stride targets.
 int ensure_mixed(bve_ctx *c, int n_src_pad, int t_stride)
 {
@@ -458,6 +481,14 @@ int ensure_mixed(bve_ctx *c, int n_src_pad, int t_stride)
         HIP_TRY(c, dev_alloc(&c->group_box, t_stride / 64));
         c->mixed_tgt_cap = t_stride;
     }
+    if (is_bounded(c)) {
+        if (!c->tile_mom || n_src_pad > c->mixed_src_cap_mom) {
+            dev_free(c->tile_mom);
+            HIP_TRY(c, dev_alloc(&c->tile_mom, (size_t)(n_src_pad / 256) * kTileMoments));
+            c->mixed_src_cap_mom = n_src_pad;
+        }
+        if (!c->scale) HIP_TRY(c, dev_alloc(&c->scale, 1));
+    }
     return BVE_OK;
 }
 
@@ -475,6 +506,9 @@ MixedArgs mixed_args_of(const bve_ctx *c, const PairArgs &a, int n_targets, int 
     ma.group_box = c->group_box;
     ma.rnf = c->nearfield;
     ma.kahan = c->precision == BVE_PRECISION_MIXED_KAHAN;
+    ma.tile_mom = c->tile_mom;
+    ma.scale = c->scale;
+    ma.err_target = c->err_target;
     return ma;
 }
 
@@ -508,11 +542,12 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
     a.n_targets = n_targets;
     a.n_sources = n_sources;
     MixedArgs ma{};
-    if (is_mixed(c)) {
+    if (is_mixed(c) || is_bounded(c)) {
         int rc = ensure_mixed(c, a.n_src_pad, a.t_stride);
         if (rc) return rc;
         ma = mixed_args_of(c, a, n_targets, n_sources);
-        HIP_TRY(c, launch_mixed_prep(ma, c->stream));
+        HIP_TRY(c, is_bounded(c) ? launch_bounded_prep(ma, c->stream)
+                                 : launch_mixed_prep(ma, c->stream));
     }
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (c->profiling) {
@@ -521,6 +556,8 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
     }
     if (is_mixed(c))
         HIP_TRY(c, launch_pair_sum_mixed(ma, c->stream));
+    else if (is_bounded(c))
+        HIP_TRY(c, launch_pair_sum_bounded(ma, c->stream));
     else if (a.plan.exact)
         HIP_TRY(c, launch_pair_sum_exact(a, c->stream));
     else
@@ -947,6 +984,8 @@ int bve_destroy(bve_ctx *c)
     dev_free(c->srcc);
     dev_free(c->tile_box);
     dev_free(c->group_box);
+    dev_free(c->tile_mom);
+    dev_free(c->scale);
     if (c->queue) (void)hipFree(c->queue);
     if (c->status) (void)hipFree(c->status);
     if (c->pinned) (void)hipHostFree(c->pinned);
@@ -991,7 +1030,7 @@ int bve_set_precision(bve_ctx *c, int precision, double nearfield_radius)
     if (!c) return BVE_ERR_ARGUMENT;
     if (precision != BVE_PRECISION_FP64 && precision != BVE_PRECISION_MIXED &&
         precision != BVE_PRECISION_FP64_FAST && precision != BVE_PRECISION_MIXED_KAHAN &&
-        precision != BVE_PRECISION_REFERENCE)
+        precision != BVE_PRECISION_REFERENCE && precision != BVE_PRECISION_MIXED_BOUNDED)
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: unknown precision %d", precision);
     if (!(nearfield_radius >= 0.0))
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_precision: nearfield_radius must be >= 0");
@@ -1001,6 +1040,15 @@ int bve_set_precision(bve_ctx *c, int precision, double nearfield_radius)
         HIP_TRY(c, hipSetDevice(c->device));
         return update_plan(c);
     }
+    return BVE_OK;
+}
+
+int bve_set_error_target(bve_ctx *c, double rel_err)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (!(rel_err >= 0.0) || !(rel_err < 1.0))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_set_error_target: target must be in [0, 1)");
+    c->err_target = rel_err;
     return BVE_OK;
 }
 
@@ -2059,9 +2107,9 @@ int bve_ensemble_set_params(bve_ensemble *e, double omega, double smooth, int pr
                             double nearfield_radius)
 {
     if (!e) return BVE_ERR_ARGUMENT;
-    if (precision == BVE_PRECISION_REFERENCE)
-        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_set_params: the reference-order mode "
-                       "runs per context, not in the batched launch");
+    if (precision == BVE_PRECISION_REFERENCE || precision == BVE_PRECISION_MIXED_BOUNDED)
+        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_set_params: the reference-order and "
+                       "bounded mixed modes run per context, not in the batched launch");
     for (bve_ctx *c : e->members) {
         int rc = bve_set_omega(c, omega);
         if (!rc) rc = bve_set_smooth(c, smooth);

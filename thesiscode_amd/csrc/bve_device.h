@@ -758,9 +758,11 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const TargetClasses cls = a.base.cls;
 
-    double xi[T], yi[T], zi[T], c0[T], gx[T], gy[T], gz[T], gr[T], ux[T], uy[T], uz[T];
+    // per-target state kept small (4 waves per SIMD): the group spheres are
+    // re-read per tile (wave-uniform, cached), c0 and the self index
+    // recomputed where needed
+    double xi[T], yi[T], zi[T], ux[T], uy[T], uz[T];
     float xf[T], yf[T], zf[T];
-    int self[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const int i = tile_base + t * kBlock + threadIdx.x;
@@ -770,15 +772,12 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
         xf[t] = (float)xi[t];
         yf[t] = (float)yi[t];
         zf[t] = (float)zi[t];
-        c0[t] = (i >= cls.smooth_begin && i < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
-        self[t] = (i < cls.self_end) ? i : -1;
-        const double4 gb = a.group_box[(tile_base + t * kBlock) / kGroup + wave];
-        gx[t] = gb.x;
-        gy[t] = gb.y;
-        gz[t] = gb.z;
-        gr[t] = gb.w;
         ux[t] = uy[t] = uz[t] = 0.0;
     }
+    auto c0_of = [&](int t) {
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        return (i >= cls.smooth_begin && i < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
+    };
 
     const int nst = a.base.n_src_pad / kMixedTile;
     for (int k = k0; k < k1; ++k) {
@@ -791,12 +790,13 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
         const double *mom = a.tile_mom + (size_t)tl * kTileMoments;
         const double W = mom[10];
         bool f64[T];
-        bool any = false;
+        bool any = false, all = true;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             // wave-uniform: one target group per wave and t
-            const double dx = gx[t] - sb.x, dy = gy[t] - sb.y, dz = gz[t] - sb.z;
-            const double rho_c = sqrt(dx * dx + dy * dy + dz * dz) - gr[t];
+            const double4 gb = a.group_box[(tile_base + t * kBlock) / kGroup + wave];
+            const double dx = gb.x - sb.x, dy = gb.y - sb.y, dz = gb.z - sb.z;
+            const double rho_c = sqrt(dx * dx + dy * dy + dz * dz) - gb.w;
             const double R = sb.w;
             bool use64 = diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt,
                                    kMixedTile) || !(rho_c > R);
@@ -808,6 +808,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             }
             f64[t] = use64;
             any |= use64;
+            all &= use64;
         }
         if (__syncthreads_or(any ? 1 : 0)) {
             stage_sources(lds.f64, a.base.src + jt, kMixedTile);
@@ -818,7 +819,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             double E[T];
 #pragma unroll
             for (int t = 0; t < T; ++t) {
-                E[t] = c0[t] - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
+                E[t] = c0_of(t) - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
                 Ef[t] = (float)E[t];
                 Mx[t] = My[t] = Mz[t] = Q[t] = 0.f;
             }
@@ -827,13 +828,40 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             for (int t = 0; t < T; ++t)
                 bounded_combine(mom, sb, xi[t], yi[t], zi[t], E[t], Mx[t], My[t], Mz[t], Q[t],
                                 ux[t], uy[t], uz[t]);
+        } else if (all) {
+            // every target of the lane in float64: one pass over the records
+            double nx[T], ny[T], nz[T], cc[T], sx[T], sy[T], sz[T];
+            int sf[T];
+            bool diag = false;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int i = tile_base + t * kBlock + threadIdx.x;
+                nx[t] = -xi[t];
+                ny[t] = -yi[t];
+                nz[t] = -zi[t];
+                cc[t] = c0_of(t);
+                sf[t] = i < cls.self_end ? i : -1;
+                sx[t] = sy[t] = sz[t] = 0.0;
+                diag |= diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, kMixedTile);
+            }
+            if (diag)
+                tile_f64<T, kMixedTile, 4, true>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
+            else
+                tile_f64<T, kMixedTile, 4, false>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                ux[t] += yi[t] * sz[t] - zi[t] * sy[t];
+                uy[t] += zi[t] * sx[t] - xi[t] * sz[t];
+                uz[t] += xi[t] * sy[t] - yi[t] * sx[t];
+            }
         } else {
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 if (f64[t]) {
+                    const int i = tile_base + t * kBlock + threadIdx.x;
                     double nx[1] = {-xi[t]}, ny[1] = {-yi[t]}, nz[1] = {-zi[t]};
-                    double cc[1] = {c0[t]}, sx[1] = {0.0}, sy[1] = {0.0}, sz[1] = {0.0};
-                    int sf[1] = {self[t]};
+                    double cc[1] = {c0_of(t)}, sx[1] = {0.0}, sy[1] = {0.0}, sz[1] = {0.0};
+                    int sf[1] = {i < cls.self_end ? i : -1};
                     if (diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, kMixedTile))
                         tile_f64<1, kMixedTile, 4, true>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
                     else
@@ -842,7 +870,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                     uy[t] += zi[t] * sx[0] - xi[t] * sz[0];
                     uz[t] += xi[t] * sy[0] - yi[t] * sx[0];
                 } else {
-                    const double E = c0[t] - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
+                    const double E = c0_of(t) - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
                     float Ef[1] = {(float)E}, Mx[1] = {0.f}, My[1] = {0.f}, Mz[1] = {0.f},
                           Q[1] = {0.f};
                     remainder_tile<1>(lds, &xf[t], &yf[t], &zf[t], Ef, Mx, My, Mz, Q);
@@ -1021,9 +1049,10 @@ __device__ __forceinline__ void bounded_moment_prep_body(const Source *__restric
 }
 
 // Velocity-scale probe of the bounded pass: for target row `row` (the
-// first of a target group), |x_i x S_i| over every source in float64 (the
-// self pair excluded), combined into *scale with atomicMax on the bits of a
-// non-negative double.  One workgroup of 256 threads per probe target.
+// first of a target group), |x_i x S_i| over every source, in float32
+// (a scale, not a result: ~1e-6 relative is plenty), the self pair and
+// coincident points left out; combined into *scale with atomicMax on the
+// bits of a non-negative double.  One workgroup of 256 threads per probe.
 __device__ __forceinline__ void bounded_probe_body(const PairArgs &b, int n_targets,
                                                    int n_sources, int row,
                                                    unsigned long long *scale_bits, double *sh)
@@ -1032,18 +1061,18 @@ __device__ __forceinline__ void bounded_probe_body(const PairArgs &b, int n_targ
     const double x = b.tx[row], y = b.ty[row], z = b.tz[row];
     if (!finite3(x, y, z)) return;
     const TargetClasses cls = b.cls;
-    const double c0 = (row >= cls.smooth_begin && row < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
+    const float e2 = (row >= cls.smooth_begin && row < cls.smooth_end) ? (float)cls.eps2 : 0.f;
     const int self = row < cls.self_end ? row : -1;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
     for (int j = threadIdx.x; j < n_sources; j += 256) {
-        if (j == self) continue;
         const Source s = b.src[j];
-        const double d = c0 - ((x * s.x + y * s.y) + z * s.z);
-        if (!(d > 0.0)) continue;
-        const double q = s.w / d;
-        sx += q * s.x;
-        sy += q * s.y;
-        sz += q * s.z;
+        const float dx = (float)(s.x - x), dy = (float)(s.y - y), dz = (float)(s.z - z);
+        const float h = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const float q = (j == self || !(h > 0.f)) ? 0.f
+                                                  : (float)s.w * __builtin_amdgcn_rcpf(0.5f * h + e2);
+        sx = __builtin_fmaf(q, dx, sx);
+        sy = __builtin_fmaf(q, dy, sy);
+        sz = __builtin_fmaf(q, dz, sz);
     }
     const double Sx = block_sum(sx, sh), Sy = block_sum(sy, sh), Sz = block_sum(sz, sh);
     if (threadIdx.x == 0) {

@@ -43,7 +43,7 @@ struct MixedArgs {
     // (W0 = sum w, m = sum w o, P = sum w o o^T as xx yy zz xy xz yz,
     // W = sum |w|, o = x - tile centre), the velocity scale max |x x S| the
     // probe pass finds, and the error target relative to it.
-    const double *tile_mom;
+    double *tile_mom;
     double *scale;
     double err_target;
 };

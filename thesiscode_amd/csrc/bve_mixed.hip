@@ -106,4 +106,121 @@ hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Bounded mixed mode
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kMixedTile) void bounded_moment_prep_kernel(
+    const Source *__restrict__ src, int n_sources, const double4 *__restrict__ tile_box,
+    double *__restrict__ mom)
+{
+    __shared__ double sh[kMixedTile];
+    bounded_moment_prep_body(src, n_sources, tile_box, mom, blockIdx.x, sh);
+}
+
+// The probe's maximum starts from zero every pass (a kernel, not a memset,
+// so that captured graphs replay it).
+__global__ void bounded_scale_reset_kernel(double *scale)
+{
+    if (threadIdx.x == 0) *scale = 0.0;
+}
+
+__global__ __launch_bounds__(256) void bounded_probe_kernel(const PairArgs b, int n_targets,
+                                                            int n_sources, int group_stride,
+                                                            double *scale)
+{
+    __shared__ double sh[256];
+    bounded_probe_body(b, n_targets, n_sources, blockIdx.x * group_stride * kGroup,
+                       (unsigned long long *)scale, sh);
+}
+
+// 4 waves per SIMD (at most 128 VGPRs): the remainder loop needs the
+// latency hiding more than the registers
+template <int T>
+__global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_kernel(MixedArgs a)
+{
+    const int work = xcd_work_item(blockIdx.x, a.base.plan.grid);
+    if (work >= a.base.plan.n_work) return;
+    __shared__ BoundedLds lds;
+    const int chunk = work / a.base.plan.n_tiles;
+    pair_bounded_item<T>(a, chunk, work - chunk * a.base.plan.n_tiles, lds);
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(MixedArgs a)
+{
+    __shared__ BoundedLds lds;
+    __shared__ int seg;
+    pair_bounded_segments<T>(a, lds, &seg);
+}
+
+template <int T>
+static hipError_t launch_bounded_t(const MixedArgs &a, hipStream_t s)
+{
+    if (a.base.plan.segments > 0) {
+        if (!a.base.queue || a.base.plan.src_step < 1) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T>), dim3(a.base.plan.grid),
+                           dim3(kBlock), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((pair_sum_bounded_kernel<T>), dim3(a.base.plan.grid), dim3(kBlock),
+                           0, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <int T>
+static int bounded_seg_blocks()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_sum_bounded_seg_kernel<T>, kBlock,
+                                                     0) != hipSuccess)
+        return 0;
+    return n;
+}
+
+int pair_bounded_blocks_per_cu(int T)
+{
+    switch (T) {
+    case 1: return bounded_seg_blocks<1>();
+    case 2: return bounded_seg_blocks<2>();
+    case 4: return bounded_seg_blocks<4>();
+    default: return 0;
+    }
+}
+
+hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s)
+{
+    if (!a.tile_mom || !a.scale) return hipErrorInvalidValue;
+    hipError_t e = launch_mixed_prep(a, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bounded_moment_prep_kernel, dim3(a.base.n_src_pad / kMixedTile),
+                       dim3(kMixedTile), 0, s, a.base.src, a.n_sources, a.tile_box, a.tile_mom);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bounded_scale_reset_kernel, dim3(1), dim3(64), 0, s, a.scale);
+    e = hipGetLastError();
+    if (e != hipSuccess || a.n_targets <= 0) return e;
+    // the first target of every group_stride-th group of 64: at most ~1024
+    // probes, a target every ~220 at level 7 (the groups are spatially
+    // compact runs, so the probes cover the sphere); ~0.05 ms at level 7
+    const int groups = ceil_div(a.n_targets, kGroup);
+    const int stride = ceil_div(groups, 1024);
+    hipLaunchKernelGGL(bounded_probe_kernel, dim3(ceil_div(groups, stride)), dim3(256), 0, s,
+                       a.base, a.n_targets, a.n_sources, stride, a.scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_sum_bounded(const MixedArgs &a, hipStream_t s)
+{
+    if (a.base.plan.tile != kMixedTile || a.base.n_src_pad % kMixedTile != 0 ||
+        a.base.t_stride % 256 != 0 || !a.tile_mom || !a.scale)
+        return hipErrorInvalidValue;
+    switch (a.base.plan.targets_per_lane) {
+    case 1: return launch_bounded_t<1>(a, s);
+    case 2: return launch_bounded_t<2>(a, s);
+    case 4: return launch_bounded_t<4>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace bve

@@ -207,6 +207,22 @@ int bve_velocity(bve_ctx *ctx, int n_targets, const double *x_targets, int self_
                  int smooth_begin, int smooth_end, int n_sources, const double *x_sources,
                  const double *vort, const double *area, double *u, double *dvort);
 
+/* No counterpart in the reference.  bve_velocity for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops): the same
+ * sums, but every pointer is device memory (x_targets 3*n_targets and
+ * x_sources 3*n_sources AoS, vort and area n_sources, u 3*n_targets,
+ * dvort self_end or NULL), the work is enqueued on the context's stream
+ * without waiting (bve_synchronize waits and reports numeric status), and
+ * nothing is allocated: the kernels' scratch (target rows, source records,
+ * queue, partial sums) is a static device array of ~46 MB per GPU, held
+ * until the process ends and shared by every call on that GPU, so calls
+ * that overlap on one GPU race.  At most 262144 targets and 262144
+ * sources; float64, float64-fast and reference-order modes (the others
+ * return BVE_ERR_ARGUMENT).  Segmented plans are capped at 1024 segments. */
+int bve_velocity_noalloc(bve_ctx *ctx, int n_targets, const double *x_targets, int self_end,
+                         int smooth_begin, int smooth_end, int n_sources, const double *x_sources,
+                         const double *vort, const double *area, double *u, double *dvort);
+
 /* Velocities of every resident target at the current state (the step-start
  * positions and vorticity), evaluated by the same launch plan and
  * arithmetic as a step's pair passes: BVEActiveRHS / BVEPassiveRHS /

@@ -49,6 +49,7 @@ SIGNATURES = {
     "bve_synchronize": (_i, [_p]),
     "bve_velocity": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p]),
     "bve_resident_velocity": (_i, [_p, _p, _p, _p]),
+    "bve_velocity_noalloc": (_i, [_p, _i, _p, _i, _i, _i, _i, _p, _p, _p, _p, _p]),
     "bve_voronoi_rk4_step": (_i, [_p, _d, _i, _p, _p, _p, _p, _p, _p, _p]),
     "bve_difference_forward": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
     "bve_adjoint_forward": (_i, [_p, _d, _i, _i, _p, _p]),
@@ -299,6 +300,15 @@ class Context:
         self._check(self._L.bve_resident_velocity(self._h, _ptr(ua), _ptr(up), _ptr(uq)),
                     "bve_resident_velocity")
         return ua, up, uq
+
+    def velocity_noalloc(self, n_targets, x_targets, n_sources, x_sources, vort, area, u,
+                         self_end=0, smooth_range=(0, 0), dvort=None):
+        """bve_velocity_noalloc on device addresses (ints): enqueued on the
+        context's stream, no allocation, no wait (synchronize() waits)."""
+        self._check(self._L.bve_velocity_noalloc(self._h, n_targets, x_targets, self_end,
+                                                 smooth_range[0], smooth_range[1], n_sources,
+                                                 x_sources, vort, area, u, dvort),
+                    "bve_velocity_noalloc")
 
     # single evaluation
     def velocity(self, targets_x, sources_x, vort, area, self_end=0, smooth_range=(0, 0),

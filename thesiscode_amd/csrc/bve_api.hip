@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -37,6 +38,68 @@ void dev_free(P *&p)
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free single evaluation (bve_velocity_noalloc):
+// one file-scope device array per GPU, held until the process ends.  Host
+// code never uses the array's name as an address: each device's address
+// comes from hipGetSymbolAddress, resolved once per device with that device
+// current (bve_create) and kept in scratch_addr.
+//
+// Layout for at most kNoallocMaxTargets targets and kNoallocMaxSources
+// sources: target rows (3 x stride doubles, stride = targets rounded up to
+// 1024), source records (sources rounded up to 512), the segment queue
+// (2 words, zeroed by a kernel at every call) and the partial rows of a
+// segmented pass of at most kNoallocMaxSegments segments (for the
+// reference-order pass, 3 x stride doubles).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoallocMaxTargets = 262144;
+constexpr int kNoallocMaxSources = 262144;
+constexpr int kNoallocMaxSegments = 1024;
+constexpr size_t kNoallocTargetBytes = 3 * sizeof(double) * (size_t)kNoallocMaxTargets;
+constexpr size_t kNoallocSourceBytes = sizeof(Source) * (size_t)kNoallocMaxSources;
+constexpr size_t kNoallocQueueBytes = 256;
+// (segments + target tiles) rows of 3 x kBlock T doubles, largest at T = 4
+constexpr size_t kNoallocPartialBytes =
+    (size_t)(kNoallocMaxSegments + kNoallocMaxTargets / (4 * kBlock)) * 3 * 4 * kBlock *
+    sizeof(double);
+constexpr size_t kNoallocScratchBytes =
+    kNoallocTargetBytes + kNoallocSourceBytes + kNoallocQueueBytes + kNoallocPartialBytes;
+constexpr int kMaxDevices = 64;
+
+std::mutex scratch_mutex;
+void *scratch_addr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+// The scratch array's address on `device` (which must be current); resolved
+// once per device.  Null with an error string if the lookup fails.
+void *scratch_for_device(int device, std::string &err)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        err = "device index beyond the scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_addr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        scratch_addr[device] = ptr;
+    }
+    return scratch_addr[device];
+}
+
+}  // namespace
+
 struct bve_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -55,6 +118,10 @@ struct bve_ctx {
     bool chunked_only = false;   // ensemble members: the batched launch runs chunked plans
     int n_cu = 256;              // compute units of the device
     unsigned *queue = nullptr;   // segment queue of the segmented pair pass (2 counters)
+    // this device's address of the allocation-free scratch (null if the
+    // symbol lookup failed; scratch_error says why)
+    unsigned char *scratch = nullptr;
+    std::string scratch_error;
     // numeric status word (device; see flag_nonfinite) and its host copy
     unsigned long long *status = nullptr;
     unsigned long long status_host = 0;
@@ -953,6 +1020,8 @@ int bve_create(bve_ctx **out, int device)
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         cus > 0)
         c->n_cu = cus;
+    // set-up of the allocation-free entry point on this device (current)
+    c->scratch = (unsigned char *)scratch_for_device(device, c->scratch_error);
     // the segment queue starts at zero and every segmented launch leaves it so
     e = hipMalloc(&c->queue, 2 * sizeof(unsigned));
     if (e == hipSuccess) e = hipMemset(c->queue, 0, 2 * sizeof(unsigned));
@@ -1445,6 +1514,77 @@ int bve_velocity(bve_ctx *c, int n_targets, const double *x_targets, int self_en
     collect_profile(c);
     cleanup();
     return check_status(c, "bve_velocity", false);
+}
+
+int bve_velocity_noalloc(bve_ctx *c, int n_targets, const double *x_targets, int self_end,
+                         int smooth_begin, int smooth_end, int n_sources, const double *x_sources,
+                         const double *vort, const double *area, double *u, double *dvort)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (n_targets < 0 || n_sources < 0 || (n_targets > 0 && (!x_targets || !u)) ||
+        (n_sources > 0 && (!x_sources || !vort || !area)))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_velocity_noalloc: bad sizes or null arrays");
+    if (n_targets > kNoallocMaxTargets || n_sources > kNoallocMaxSources)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_velocity_noalloc: %d targets / %d sources exceed the "
+                       "scratch's maximum of %d / %d", n_targets, n_sources, kNoallocMaxTargets,
+                       kNoallocMaxSources);
+    if (self_end < 0 || self_end > n_targets || self_end > n_sources || smooth_begin < 0 ||
+        smooth_end < smooth_begin || smooth_end > n_targets)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_velocity_noalloc: bad target classes (self_end=%d, "
+                       "smooth=[%d,%d), n_targets=%d, n_sources=%d)", self_end, smooth_begin,
+                       smooth_end, n_targets, n_sources);
+    if (c->precision != BVE_PRECISION_FP64 && c->precision != BVE_PRECISION_FP64_FAST &&
+        c->precision != BVE_PRECISION_REFERENCE)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_velocity_noalloc: runs the float64 and "
+                       "reference-order modes (precision %d)", c->precision);
+    if (!c->scratch)
+        return c->fail(BVE_ERR_DEVICE, "bve_velocity_noalloc: no scratch on device %d: %s",
+                       c->device, c->scratch_error.c_str());
+    if (n_targets == 0) return BVE_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int stride = round_up(n_targets, kTargetAlign);
+    const int ns_pad = round_up(std::max(n_sources, 1), kSourceAlign);
+    double *t = (double *)c->scratch;
+    Source *src = (Source *)(c->scratch + kNoallocTargetBytes);
+    unsigned *queue = (unsigned *)(c->scratch + kNoallocTargetBytes + kNoallocSourceBytes);
+    double *partial = (double *)(c->scratch + kNoallocTargetBytes + kNoallocSourceBytes +
+                                 kNoallocQueueBytes);
+    PairPlan plan = plan_for(c, n_targets, ns_pad);
+    if (!plan.exact && plan.segments > kNoallocMaxSegments) {
+        // the same geometry with fewer, longer segments
+        plan = make_segmented_plan(n_targets, ns_pad, plan.targets_per_lane, plan.tile,
+                                   kNoallocMaxSegments, plan.recip,
+                                   std::min(plan.grid, kNoallocMaxSegments));
+        plan.crossed = 1;
+    }
+    if (partial_doubles(plan, stride) * sizeof(double) > kNoallocPartialBytes)
+        return c->fail(BVE_ERR_ARGUMENT, "bve_velocity_noalloc: the launch plan's partial sums "
+                       "exceed the scratch (use the default plan)");
+    HIP_TRY(c, launch_aos_to_soa_padded(x_targets, n_targets, t, stride, c->stream));
+    HIP_TRY(c, launch_make_sources_padded(x_sources, vort, area, n_sources, ns_pad,
+                                          c->precision == BVE_PRECISION_REFERENCE, src,
+                                          c->stream));
+    HIP_TRY(c, launch_zero_words(queue, 2, c->stream));
+    PairArgs pa{};
+    pa.tx = t;
+    pa.ty = t + stride;
+    pa.tz = t + 2 * (size_t)stride;
+    pa.t_stride = stride;
+    pa.cls.self_end = self_end;
+    pa.cls.smooth_begin = smooth_begin;
+    pa.cls.smooth_end = smooth_end;
+    pa.cls.eps2 = c->smooth * c->smooth;
+    pa.src = src;
+    pa.n_src_pad = ns_pad;
+    pa.plan = plan;
+    pa.partial = partial;
+    pa.queue = queue;
+    pa.area = area;
+    const int rc = pair_pass(c, pa, n_targets, n_sources);
+    if (rc) return rc;
+    HIP_TRY(c, launch_finalize_velocity(partial, plan, t, stride, n_targets, c->omega, u,
+                                        dvort, dvort ? self_end : 0, c->status, c->stream));
+    return BVE_OK;
 }
 
 int bve_resident_velocity(bve_ctx *c, double *u_active, double *u_passive, double *u_particles)

@@ -157,6 +157,15 @@ hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, unsigned long long *status, hipStream_t s);
 hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_t s);
+// Allocation-free single evaluation: AoS targets into SoA rows of `stride`
+// (zeros beyond n), source records for n_pad slots (zero weight beyond n),
+// and zeroed queue counters, all by kernels (no memset nodes).
+hipError_t launch_aos_to_soa_padded(const double *aos, int n, double *dst, int stride,
+                                    hipStream_t s);
+hipError_t launch_make_sources_padded(const double *xyz, const double *vort, const double *area,
+                                      int n, int n_pad, int unweighted, Source *src,
+                                      hipStream_t s);
+hipError_t launch_zero_words(unsigned *p, int n, hipStream_t s);
 
 // Exact-difference run of the active panels (bve_difference_forward): the
 // trajectory of (x, zeta) and the difference to the trajectory of

@@ -302,6 +302,61 @@ hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_
     return hipGetLastError();
 }
 
+__global__ void aos_to_soa_padded_kernel(const double *__restrict__ aos, int n,
+                                         double *__restrict__ dst, int stride)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= stride) return;
+    const bool live = i < n;
+    dst[i] = live ? aos[3 * i] : 0.0;
+    dst[stride + i] = live ? aos[3 * i + 1] : 0.0;
+    dst[2 * (size_t)stride + i] = live ? aos[3 * i + 2] : 0.0;
+}
+
+hipError_t launch_aos_to_soa_padded(const double *aos, int n, double *dst, int stride,
+                                    hipStream_t s)
+{
+    if (stride <= 0) return hipSuccess;
+    hipLaunchKernelGGL(aos_to_soa_padded_kernel, dim3(ceil_div(stride, 256)), dim3(256), 0, s, aos,
+                       n, dst, stride);
+    return hipGetLastError();
+}
+
+__global__ void make_sources_padded_kernel(const double *__restrict__ xyz,
+                                           const double *__restrict__ vort,
+                                           const double *__restrict__ area, int n, int n_pad,
+                                           int unweighted, Source *__restrict__ src)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_pad) return;
+    src[i] = i < n ? Source{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
+                            unweighted ? vort[i] : vort[i] * area[i]}
+                   : Source{0.0, 0.0, 0.0, 0.0};
+}
+
+hipError_t launch_make_sources_padded(const double *xyz, const double *vort, const double *area,
+                                      int n, int n_pad, int unweighted, Source *src,
+                                      hipStream_t s)
+{
+    if (n_pad <= 0) return hipSuccess;
+    hipLaunchKernelGGL(make_sources_padded_kernel, dim3(ceil_div(n_pad, 256)), dim3(256), 0, s,
+                       xyz, vort, area, n, n_pad, unweighted, src);
+    return hipGetLastError();
+}
+
+__global__ void zero_words_kernel(unsigned *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_words(unsigned *p, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_words_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, unsigned long long *status, hipStream_t s)

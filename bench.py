@@ -71,7 +71,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--level", type=int, default=7)
     p.add_argument("--dt", type=float, default=0.005)
-    p.add_argument("--modes", type=str, default="fp64,fp64_fast,mixed,reference")
+    p.add_argument("--modes", type=str, default="fp64,fp64_fast,mixed,mixed_bounded,reference")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="target duration of the CPU-baseline sample (0 = skip)")
     p.add_argument("--plan", type=str, default="",
@@ -82,8 +82,6 @@ def parse():
                    help="members of the ensemble workload (0 = skip)")
     p.add_argument("--large-level", type=int, default=8,
                    help="cubed-sphere level of the two-vortex large solve (0 = skip)")
-    p.add_argument("--copy-probe-mib", type=int, default=0,
-                   help="only run a plain copy of this many MiB (byte-counter calibration)")
     return p.parse_args()
 
 
@@ -376,6 +374,52 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
                 traffic_bytes_per_launch=traffic)
 
 
+def pareto_curve(ctx, reload, sample, dt, steps, base_launch_ms, base_step_ms):
+    """Error against speed of the reduced-precision modes on one problem:
+    the float32 mixed pass at near-field radii 0.02 to 0.5, with and
+    without Kahan sums, and the bounded pass at error targets 1e-8 to
+    1e-11.  Speed-ups are against the plain float64 pass (per launch and
+    per RK4 step); errors are normwise (and max pointwise) over the
+    sample's three classes, from the timed plan."""
+    from thesiscode_amd import _lib
+    points = [("mixed", r) for r in (0.02, 0.05, 0.1, 0.2, 0.5)]
+    points += [("mixed_kahan", r) for r in (0.02, 0.1, 0.5)]
+    points += [("mixed_bounded", t) for t in (1e-8, 1e-9, 1e-10, 1e-11)]
+    out = []
+    for mode, param in points:
+        if mode == "mixed_bounded":
+            ctx.set_precision(MODES[mode])
+            ctx.set_error_target(param)
+        else:
+            ctx.set_precision(MODES[mode], param)
+        reload()
+        ctx.advance(dt, 1)
+        ctx.synchronize()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        ctx.advance(dt, steps)
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        launches, kernel_ms = ctx.profile()
+        ctx.set_profiling(False)
+        reload()
+        err, err_point, _ = sample.errors(ctx)
+        launch_ms = kernel_ms / max(launches, 1)
+        step_ms = el * 1e3 / steps
+        out.append(dict(mode=mode, **({"error_target": param} if mode == "mixed_bounded"
+                                      else {"nearfield_radius": param}),
+                        ms_per_launch=launch_ms, ms_per_step=step_ms,
+                        speedup_vs_fp64_launch=base_launch_ms / launch_ms,
+                        speedup_vs_fp64_step=base_step_ms / step_ms,
+                        rel_err=err, max_pointwise_rel_err=err_point))
+        print(f"# pareto {out[-1]}", file=sys.stderr, flush=True)
+    ctx.set_error_target(1e-10)
+    ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    ok = [p for p in out if p["rel_err"] <= ERR_BAR]
+    best = max(ok, key=lambda p: p["speedup_vs_fp64_step"]) if ok else None
+    return {"points": out, "fastest_within_1e-10": best}
+
+
 def host_roundtrip_measurement(ctx, grid, absv, dt, steps, pairs_primary, mode="fp64_fast"):
     """The drop-in's per-step pattern (the Fortran BVERK4): particles and
     their absolute vorticity uploaded, one RK4 step, particles, their
@@ -434,10 +478,14 @@ def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed
     for mode in modes:
         out[mode] = measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample)
         print(f"# large {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    curve = None
+    if "fp64" in out:
+        curve = pareto_curve(ctx, reload, sample, dt, 1, out["fp64"]["avg_launch_ms"],
+                             out["fp64"]["ms_per_step"])
     ctx.close()
     return dict(workload=f"two_vortices_cubed_sphere_L{level}_rk4_step_dt{dt:g}", level=level,
                 n_active=na, n_particles=npart, n_passive=npp, pairs_per_step=pairs_primary,
-                pairs_per_step_all_classes=pairs_all, modes=out)
+                pairs_per_step_all_classes=pairs_all, modes=out, mixed_pareto=curve)
 
 
 def main():
@@ -450,12 +498,6 @@ def main():
     from thesiscode_amd import _lib
     from thesiscode_amd import grid as gridmod
     from thesiscode_amd import initial_conditions as ic
-
-    if args.copy_probe_mib > 0:
-        with _lib.Context(0) as c:
-            c.copy_probe(args.copy_probe_mib << 20)
-        print(json.dumps({"copy_probe_bytes": args.copy_probe_mib << 20}), flush=True)
-        return
 
     g = gridmod.cubed_sphere(args.level)
     za, zp, zq = ic.gauss_vortex(g, lat0=np.pi / 4, lon0=0.0, width=0.05)
@@ -483,6 +525,11 @@ def main():
     for mode in [m for m in args.modes.split(",") if m]:
         results[mode] = measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample)
         print(f"# {mode}: {results[mode]}", file=sys.stderr, flush=True)
+    if "fp64" in results:
+        curve = pareto_curve(ctx, reload, sample, args.dt, max(2, min(args.steps, 5)),
+                             results["fp64"]["avg_launch_ms"], results["fp64"]["ms_per_step"])
+    else:
+        curve = None
     roundtrip = host_roundtrip_measurement(ctx, g, absv, args.dt, max(args.steps // 2, 1),
                                            pairs_primary)
     print(f"# host round trip: {roundtrip}", file=sys.stderr, flush=True)
@@ -510,6 +557,7 @@ def main():
             "dt": args.dt, "mode": best,
             "pairs_per_step": pairs_primary, "pairs_per_step_all_classes": pairs_all,
             "modes": results,
+            "mixed_pareto": curve,
             "host_roundtrip": roundtrip,
         },
         "roofline": {

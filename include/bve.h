@@ -357,10 +357,6 @@ int bve_vtk_write_frame(const char *filename, int n_particles, int ld_particles,
  * since profiling was switched on. */
 int bve_set_profiling(bve_ctx *ctx, int on);
 int bve_get_profile(bve_ctx *ctx, long long *launches, double *total_ms);
-/* Copies `bytes` (a multiple of 16) between two fresh device buffers with a
- * plain streaming kernel: the known-traffic workload against which hardware
- * byte counters are calibrated. */
-int bve_copy_probe(bve_ctx *ctx, long long bytes);
 /* Pairs evaluated by one RK4 step of the loaded problem: 4 (N_a (N_a - 1) + (N_p + N_pp) N_a). */
 int bve_pairs_per_step(const bve_ctx *ctx, double *pairs);
 

@@ -249,8 +249,10 @@ def test_mixed_with_everything_near_is_float64(ctx, precision):
 @pytest.mark.parametrize("rnf", [0.0, 0.02, 0.1])
 def test_mixed_error(ctx, precision, rnf):
     """float32 far field: the normwise error against the float64 oracle is
-    bounded by the float32 term rounding (~1e-7 per term); the bound here
-    is loose, the measured values are reported by the benchmark."""
+    set by the float32 term rounding, for every class held to about twice
+    the value measured at level 5: with no float64 near field (rnf = 0)
+    2.6e-6 (active) and 3.0e-6 (particles), bar 6e-6; with rnf >= 0.02 the
+    bar is 1e-6."""
     g = gridmod.cubed_sphere(5)
     za, zp, zq = ic.gauss_vortex(g, width=0.05)
     na, npp = g.n_active, g.n_passive
@@ -262,10 +264,14 @@ def test_mixed_error(ctx, precision, rnf):
         u = ctx.velocity(t, g.active_x, za, g.active_area, self_end=na, smooth_range=(na, na + npp))
     finally:
         ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    ref_s = oracle.smooth_velocity(g.passive_x, g.active_x, za, g.active_area, 0.05)
     ea = normwise_rel(u[:na], ref_a)
+    es = normwise_rel(u[na:na + npp], ref_s)
     ep = normwise_rel(u[na + npp:], ref_p)
-    print(f"mixed precision={precision} rnf={rnf}: active {ea:.3e} particles {ep:.3e}")
-    assert ea < 1e-5 and ep < 1e-5
+    print(f"mixed precision={precision} rnf={rnf}: active {ea:.3e} parents {es:.3e} "
+          f"particles {ep:.3e}")
+    bar = 6e-6 if rnf == 0.0 else 1e-6
+    assert ea < bar and es < bar and ep < bar
 
 
 def test_level7_invariants(ctx):
@@ -456,18 +462,41 @@ def test_resident_advance_matches_stepwise(ctx):
 
 
 def test_renormalize_option(ctx):
+    """set_renormalize(True): BVERK4 followed by RenormalizeGrid
+    (ParticlesEdgesPanels2.f90:1690-1702) on every particle and panel, two
+    steps (the second starts from renormalised source records), against
+    the oracle's bvo_rk4_step + bvo_renormalize: positions to 1e-12 of the
+    steps' displacement, |x| = 1 to rounding."""
     g = gridmod.cubed_sphere(3)
     z = ic.rh4_wave(g.active_x)
+    dt, omega = 0.05, 2 * np.pi
     ctx.reset()
+    ctx.set_omega(omega)
     ctx.set_renormalize(True)
     try:
         ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
-        qx, _, _, _ = ctx.rk4_step(0.05, g.particles_x)
-        ax, _, px = ctx.store_panels()
+        qx = g.particles_x
+        for _ in range(2):
+            qx, _, _, _ = ctx.rk4_step(dt, qx)
+        ax, av, px = ctx.store_panels()
     finally:
         ctx.set_renormalize(False)
-    for p in (qx, ax, px):
-        assert np.abs(np.linalg.norm(p, axis=1) - 1).max() < 4e-16
+    ra, rz, rp, rq = g.active_x, z, g.passive_x, g.particles_x
+    for _ in range(2):
+        r = oracle.rk4_step(ra, rz, g.active_area, rp, rq, np.zeros(len(rq)), dt, omega, 0.05)
+        ra, rp, rq = (oracle.renormalize(r[k]) for k in ("active_x", "passive_x", "particles_x"))
+        rz = r["active_relvort"]
+    disp = np.linalg.norm(ra - g.active_x, axis=1).max()
+    for got, want in ((ax, ra), (px, rp), (qx, rq)):
+        assert np.abs(got - want).max() < TOL * disp
+        assert np.abs(np.linalg.norm(got, axis=1) - 1).max() < 4e-16
+    assert np.abs(av - rz).max() < TOL * np.abs(rz - z).max()
+    # a wrong vector normalised to unit length would pass the norm check
+    # but not the comparison: the renormalisation moved points by far more
+    # than the bar
+    raw = oracle.rk4_step(g.active_x, z, g.active_area, g.passive_x, g.particles_x,
+                          np.zeros(g.n_particles), dt, omega, 0.05)["active_x"]
+    assert np.abs(raw - oracle.renormalize(raw)).max() > 1e3 * TOL * disp
 
 
 def test_small_and_empty_classes(ctx):

@@ -69,7 +69,6 @@ SIGNATURES = {
     "bve_ensemble_pairs_per_step": (_i, [_p, _dp]),
     "bve_set_profiling": (_i, [_p, _i]),
     "bve_get_profile": (_i, [_p, ctypes.POINTER(_ll), _dp]),
-    "bve_copy_probe": (_i, [_p, _ll]),
     "bve_get_plan": (_i, [_p, _p, _p, _p, _p, _p]),
     "bve_get_launch": (_i, [_p, _p, _p, _p, _p]),
     "bve_pairs_per_step": (_i, [_p, _dp]),
@@ -385,8 +384,6 @@ class Context:
         self._check(self._L.bve_pairs_per_step(self._h, ctypes.byref(v)), "bve_pairs_per_step")
         return v.value
 
-    def copy_probe(self, nbytes):
-        self._check(self._L.bve_copy_probe(self._h, nbytes), "bve_copy_probe")
 
     def plan(self):
         """(targets_per_lane, tile, chunk, n_chunks, recip) of the pair kernel."""

@@ -71,7 +71,23 @@ def build(verbose=False, extra_flags=()):
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_fortran_host(verbose)
+    build_copy_probe(verbose)
     return LIB
+
+
+COPY_PROBE = os.path.join(HERE, "copy_probe")
+
+
+def build_copy_probe(verbose=False):
+    """The byte-counter calibration tool (csrc/copy_probe.hip; not part of
+    libbve), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE."""
+    tmp = COPY_PROBE + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, os.path.join(CSRC, "copy_probe.hip"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, COPY_PROBE)
+    return COPY_PROBE
 
 
 FORTRAN = os.path.join(HERE, "fortran")

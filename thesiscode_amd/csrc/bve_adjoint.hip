@@ -21,56 +21,29 @@
 // float64 reciprocals, and writes per-chunk partial sums that the combine
 // kernel adds in a fixed order.
 #include "bve_common.h"
+#include "bve_device.h"
 #include "bve_kernels.h"
 
 namespace bve {
 
-__device__ __forceinline__ double rcp_newton1(double p)
-{
-    const double r = __builtin_amdgcn_rcp(p);
-    const double e = __builtin_fma(-p, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-
-__device__ __forceinline__ double rcp_newton2(double d)
-{
-    double r = __builtin_amdgcn_rcp(d);
-    double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-d, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-
-// 1/d_r for R denominators: R = 1 each refined twice, R = 4 one shared
-// reciprocal of the product refined once (as in the forward kernel).
+// 1/d_r for R denominators with the forward pass's reciprocals: R = 1 each
+// refined twice (recip_f64), R = 4 one shared reciprocal of the product
+// refined once (recip1_f64), as in tile_f64.
 template <int R>
 __device__ __forceinline__ void recips(const double (&d)[R], double (&r)[R])
 {
     if constexpr (R == 1) {
-        r[0] = rcp_newton2(d[0]);
+        r[0] = recip_f64(d[0]);
     } else {
         static_assert(R == 4, "R must be 1 or 4");
         const double p01 = d[0] * d[1], p23 = d[2] * d[3];
-        const double rr = rcp_newton1(p01 * p23);
+        const double rr = recip1_f64(p01 * p23);
         const double r01 = rr * p23, r23 = rr * p01;
         r[0] = d[1] * r01;
         r[1] = d[0] * r01;
         r[2] = d[3] * r23;
         r[3] = d[2] * r23;
     }
-}
-
-__device__ __forceinline__ int xcd_item(int g, int grid)
-{
-    return (g % 8) * (grid / 8) + g / 8;
-}
-
-// Wave-uniform: does source tile [jt, jt + tile) hold the own index of one
-// of this wave's lanes (lanes carry consecutive indices from lo)?
-__device__ __forceinline__ bool diag_tile(int lo, int self_end, int jt, int tile)
-{
-    const int hi = min(lo + 64, self_end);
-    return (lo < hi) && (lo < jt + tile) && (jt < hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -81,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void vjp_kernel(const AdjRec *__restrict__ 
                                                      int self_end, int n_pad, PairPlan plan,
                                                      double *__restrict__ partial)
 {
-    const int work = xcd_item(blockIdx.x, plan.grid);
+    const int work = xcd_work_item(blockIdx.x, plan.grid);
     if (work >= plan.n_work) return;
     const int chunk = work / plan.n_tiles;
     const int ttile = work - chunk * plan.n_tiles;
@@ -285,7 +258,7 @@ hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s)
 // One (target tile, source chunk) item: base sums S and their differences.
 __global__ __launch_bounds__(kBlock) void diff_pair_kernel(DiffState st, PairPlan plan)
 {
-    const int work = xcd_item(blockIdx.x, plan.grid);
+    const int work = xcd_work_item(blockIdx.x, plan.grid);
     if (work >= plan.n_work) return;
     const int chunk = work / plan.n_tiles;
     const int i = (work - chunk * plan.n_tiles) * kBlock + threadIdx.x;

@@ -1903,25 +1903,6 @@ int bve_get_launch(const bve_ctx *c, int *segments, int *grid, int *n_tiles,
     return BVE_OK;
 }
 
-int bve_copy_probe(bve_ctx *c, long long bytes)
-{
-    if (!c) return BVE_ERR_ARGUMENT;
-    if (bytes < 16 || bytes % 16 != 0)
-        return c->fail(BVE_ERR_ARGUMENT, "bve_copy_probe: bytes must be a positive multiple of 16");
-    HIP_TRY(c, hipSetDevice(c->device));
-    char *in = nullptr, *out = nullptr;
-    HIP_TRY(c, dev_alloc(&in, (size_t)bytes));
-    hipError_t e = dev_alloc(&out, (size_t)bytes);
-    if (e == hipSuccess) e = hipMemsetAsync(in, 1, (size_t)bytes, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = launch_copy_probe(in, out, (size_t)bytes, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    dev_free(in);
-    dev_free(out);
-    if (e != hipSuccess) return c->hip_fail(e, "bve_copy_probe");
-    return BVE_OK;
-}
-
 int bve_pairs_per_step(const bve_ctx *c, double *pairs)
 {
     if (!c || !pairs) return BVE_ERR_ARGUMENT;

@@ -156,7 +156,6 @@ hipError_t launch_make_sources(const double *xyz, const double *vort, const doub
 hipError_t launch_finalize_velocity(const double *partial, const PairPlan &plan, const double *t,
                                     int stride, int n, double omega, double *u, double *dvort,
                                     int n_dvort, unsigned long long *status, hipStream_t s);
-hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_t s);
 // Allocation-free single evaluation: AoS targets into SoA rows of `stride`
 // (zeros beyond n), source records for n_pad slots (zero weight beyond n),
 // and zeroed queue counters, all by kernels (no memset nodes).

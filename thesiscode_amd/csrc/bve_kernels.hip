@@ -285,23 +285,6 @@ __global__ void finalize_velocity_kernel(const double *__restrict__ partial, Pai
     if (dvort && i < n_dvort) dvort[i] = -(2.0 * omega * uz);
 }
 
-// Plain streaming copy (16 B per lane per access), the reference workload
-// for calibrating the hardware byte counters against known traffic.
-__global__ __launch_bounds__(256) void copy_probe_kernel(const double2 *__restrict__ in,
-                                                         double2 *__restrict__ out, size_t n)
-{
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
-        out[i] = in[i];
-}
-
-hipError_t launch_copy_probe(const void *in, void *out, size_t bytes, hipStream_t s)
-{
-    const size_t n = bytes / sizeof(double2);
-    hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, s, (const double2 *)in,
-                       (double2 *)out, n);
-    return hipGetLastError();
-}
-
 __global__ void aos_to_soa_padded_kernel(const double *__restrict__ aos, int n,
                                          double *__restrict__ dst, int stride)
 {

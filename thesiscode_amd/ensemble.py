@@ -87,6 +87,10 @@ def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, ome
             ens.set_params(omega, 0.05, precision, 0.02)
             load_members(ens, mem)
             for T, chunk in candidate_plans(L):
+                # every candidate times the same steps from the initial state
+                # (carried on through all candidates, ~80 steps, a perturbed
+                # state rolls up and leaves the sphere)
+                load_members(ens, mem)
                 for m in range(members_per_level):
                     ens.set_member_plan(m, T, chunk)
                 ens.advance(dt, 1, use_graph=False)

@@ -741,13 +741,17 @@ int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
     int rc = ensure_partial(c, partial_doubles(fwd, c->t_stride));
     if (rc) return rc;
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_active);
-    const int wg = c->plan_wg ? c->plan_wg : 16384;
     // The adjoint's pair pass shares one float64 reciprocal (refined once)
     // among four pairs in both float64 modes: ~1e-13 relative, far inside
-    // what a gradient needs, and 4-6% off the gradient at level 6.
-    vjp = make_plan(c->n_active, c->adj_pad, T, 256, wg, 4);
-    return grow(c, c->adj_pt, c->adj_pt_cap, (size_t)vjp.n_chunks * kAdjRows * c->adj_pad,
-                false);
+    // what a gradient needs, and 4-6% off the gradient at level 6.  It runs
+    // the forward pass's segment queue with 2 segments per resident
+    // workgroup: its items are few and alike (level 6: 4,608 of them for
+    // 768 slots), and every segment writes 10 partial rows per tile.
+    const int slots = vjp_blocks_per_cu(T, 4) * c->n_cu;
+    const int segments = c->plan_wg ? c->plan_wg : 2 * (slots > 0 ? slots : 2 * c->n_cu);
+    vjp = make_segmented_plan(c->n_active, c->adj_pad, T, 256, segments, 4, slots);
+    return grow(c, c->adj_pt, c->adj_pt_cap,
+                (size_t)(vjp.segments + vjp.n_tiles) * kAdjRows * kBlock * T, false);
 }
 
 // One reverse RK4 step: lam <- lam + sum_s Ybar_s with the stage inputs in
@@ -777,7 +781,7 @@ int adjoint_reverse_step(bve_ctx *c, const Source *stages, const PairPlan &vjp)
         a.area = c->area;
         a.plan = vjp;
         a.partial = c->adj_pt;
-        a.n_chunks = vjp.n_chunks;
+        a.queue = c->queue;
         HIP_TRY(c, launch_adjoint_stage(a, c->stream));
     }
     std::swap(c->adj_lam, c->adj_lam_acc);

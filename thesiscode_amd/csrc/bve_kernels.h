@@ -56,8 +56,8 @@ struct __align__(16) AdjRec {
     double x, y, z, w, nx, ny, nz, pad;
 };
 
-// Partial rows per source chunk of the adjoint's pair pass: S (3), T (3),
-// H (3), g (1).
+// Partial rows per (segment, target tile) of the adjoint's pair pass:
+// S (3), V (3), N (3), g (1).
 constexpr int kAdjRows = 10;
 
 // One reverse RK4 stage of the adjoint on n active panels.  Row arrays have
@@ -75,12 +75,14 @@ struct AdjStage {
     AdjRec *rec;
     const Source *ys;      // this stage's input as source records {x, zeta A}
     const double *area;
-    PairPlan plan;
-    double *partial;       // pair pass: kAdjRows rows per chunk
-    int n_chunks;
+    PairPlan plan;         // segmented plan of the pair pass over n_pad records
+    double *partial;       // pair pass: kAdjRows rows per (segment + tile)
+    unsigned *queue;       // segment queue (two counters, zero between launches)
 };
 
 hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s);
+// Workgroups of the adjoint pair kernel one CU holds at once.
+int vjp_blocks_per_cu(int targets_per_lane, int recip);
 
 // Device state of a panel RK4 step.  Targets are [active | passive | particles]
 // in SoA rows of length t_stride.

@@ -105,18 +105,19 @@ int bve_set_renormalize(bve_ctx *ctx, int on);
  * tiles in a stride order).  Used by the autotuner. */
 int bve_set_plan(bve_ctx *ctx, int targets_per_lane, int tile, int target_workgroups,
                  int recip);
-/* chunk > 0: split the sources into chunks of `chunk` sources (rounded up to
- * whole tiles), one work item per (target tile, chunk), in every arithmetic
- * mode -- the split ensemble members use, so a problem run alone with the
- * same chunk and targets per lane reproduces a member bit for bit.  0 (the
- * default) restores the built-in split. */
+/* chunk > 0: segments of `chunk` sources' worth of work (chunk / tile
+ * consecutive (target tile, source tile) items of the tile-major sequence,
+ * chunk rounded up to whole 256-source tiles), in every arithmetic mode --
+ * the split ensemble members use, so a problem run alone with the same
+ * chunk and targets per lane reproduces a member bit for bit.  0 (the
+ * default) restores the built-in segment count. */
 int bve_set_chunk(bve_ctx *ctx, int chunk);
 
 /* The pair kernel's current launch geometry for the loaded problem:
- * targets per lane, sources per LDS tile, sources per chunk, chunks (each
- * chunk's partial sums are written once and read once per stage), pairs per
- * shared reciprocal.  For a segmented (float64) plan, chunk is the sources
- * per segment and n_chunks the number of segments.  Any pointer may be NULL. */
+ * targets per lane, sources per LDS tile, sources' worth of work per segment
+ * (segment length x tile), segments (for the reference-order pass: one
+ * chunk of all sources), pairs per shared reciprocal.  Any pointer may be
+ * NULL. */
 int bve_get_plan(const bve_ctx *ctx, int *targets_per_lane, int *tile, int *chunk,
                  int *n_chunks, int *recip);
 /* Launch shape of the pair kernel: segments (0 for a chunked plan),

@@ -80,13 +80,15 @@ def test_target_zero_is_float64():
 @pytest.mark.parametrize("chunk", [0, 1024])
 def test_level7_gauss_within_target(chunk):
     """config[1]: level-7 Gaussian vortex, resident state on the default
-    (segmented) plan and on a chunked plan."""
+    plan and on one with explicit 1024-source segments."""
     g = gridmod.cubed_sphere(7)
     za, zp, _ = ic.gauss_vortex(g, lat0=np.pi / 4, lon0=0.0, width=0.05)
     absv = zp + 2 * OMEGA * g.particles_x[:, 2]
     with bounded_context(g, za, absv) as c:
         c.set_chunk(chunk)
-        assert (c.launch()["segments"] > 0) == (chunk == 0)
+        assert c.launch()["segments"] > 0
+        if chunk:
+            assert c.plan()[2] == chunk
         u = c.resident_velocity()
     err = sampled_errors(g, za, u, [ic.latlon_to_xyz(np.pi / 4, 0.0)])
     print(f"level 7 bounded (chunk {chunk}): {err:.3e}")

@@ -518,12 +518,12 @@ def test_small_and_empty_classes(ctx):
 @pytest.mark.parametrize("precision", [_lib.BVE_PRECISION_FP64, _lib.BVE_PRECISION_MIXED])
 def test_work_split_per_mode(ctx, precision):
     """Float64 and mixed passes run the segmented persistent launch (the
-    mixed one visiting each row's source tiles in stride order); an explicit
-    chunk runs chunked (the mixed chunks interleaving their tiles); both
-    splits give the same step to the float64 bar in either arithmetic (the
-    mixed pass forms its float32 sums per source tile and target group, the
-    same in every split), and for float64 the segmented launch writes far
-    fewer partial sums."""
+    mixed one visiting each row's source tiles in stride order); an
+    explicit chunk sets the segments' length (chunk / tile items each, the
+    split ensemble members use); both give the same step to the float64
+    bar in either arithmetic (the mixed pass forms its float32 sums per
+    source tile and target group, the same in every split), and either
+    launch writes one partial row per (segment, target tile) piece."""
     g = gridmod.cubed_sphere(5)
     z = ic.rh4_wave(g.active_x)
     ctx.reset()
@@ -534,16 +534,20 @@ def test_work_split_per_mode(ctx, precision):
         seg = ctx.launch()
         assert seg["segments"] > 0 and 0 < seg["grid"] <= seg["segments"]
         x_seg = ctx.rk4_step(0.01, g.particles_x)[0]
-        ctx.set_chunk(512)
+        ctx.set_chunk(768)
         ch = ctx.launch()
-        assert ch["segments"] == 0
-        if precision == _lib.BVE_PRECISION_FP64:
-            assert ch["partial_bytes"] > seg["partial_bytes"]
+        T, tile, chunk, n_seg, _ = ctx.plan()
+        assert chunk == 768 and tile == 256 and n_seg == ch["segments"]
+        items = ch["n_tiles"] * (-(-g.n_active // 512) * 512 // tile)
+        assert ch["segments"] == -(-items // 3)
         ctx.reset()
         ctx.load_panels(g.active_x, z, g.active_area, g.passive_x)
         x_ch = ctx.rk4_step(0.01, g.particles_x)[0]
         disp = np.abs(x_seg - g.particles_x).max()
         assert np.abs(x_seg - x_ch).max() < 1e-12 * disp
+        for launch in (seg, ch):
+            rows = launch["partial_bytes"] // (3 * 256 * T * 8)
+            assert rows <= launch["segments"] + launch["n_tiles"]
     finally:
         ctx.set_chunk(0)
         ctx.set_precision(_lib.BVE_PRECISION_FP64)

@@ -259,6 +259,29 @@ PairPlan make_segmented_plan(int n_targets, int n_src_pad, int T, int tile, int 
     return p;
 }
 
+// Segmented geometry with segments of an explicit length: seg_len items of
+// the tile-major sequence each (an explicit chunk of seg_len * tile
+// sources), run by a persistent grid of at most `slots` workgroups.
+PairPlan make_len_plan(int n_targets, int n_src_pad, int T, int tile, int seg_len, int recip,
+                       int slots)
+{
+    PairPlan p{};
+    p.targets_per_lane = T;
+    p.tile = tile;
+    p.recip = recip;
+    p.n_tiles = std::max(1, ceil_div(n_targets, kBlock * T));
+    p.n_src_tiles = n_src_pad / tile;
+    const int n_items = p.n_tiles * p.n_src_tiles;
+    p.seg_len = std::max(1, std::min(seg_len, n_items));
+    p.segments = ceil_div(n_items, p.seg_len);
+    p.grid = slots > 0 ? std::min(slots, p.segments) : p.segments;
+    p.chunk = p.seg_len * tile;
+    p.n_chunks = 0;
+    p.n_work = p.segments;
+    p.src_step = 1;
+    return p;
+}
+
 // A stride near 0.618 n that is coprime with n (1 for n <= 2): k -> k * step
 // mod n visits every index once, consecutive k far apart.
 int coprime_step(int n)
@@ -341,16 +364,16 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
         // error-bound granularity), stride-ordered segment queue, or
         // interleaved chunks when a chunk is set; rows hold x_i x S sums
         const int T = c->plan_t ? c->plan_t : (n_targets >= 131072 ? 2 : 1);
+        const int slots = pair_bounded_blocks_per_cu(T) * c->n_cu;
         PairPlan p;
-        if (c->plan_chunk > 0 || c->chunked_only) {
-            p = make_plan(n_targets, n_src_pad, T, 256, c->plan_wg ? c->plan_wg : 65536, 1,
-                          c->plan_chunk);
+        if (c->plan_chunk > 0) {
+            p = make_len_plan(n_targets, n_src_pad, T, 256, round_up(c->plan_chunk, 256) / 256, 1,
+                              slots);
         } else {
-            const int slots = pair_bounded_blocks_per_cu(T) * c->n_cu;
             const int segments = c->plan_wg ? c->plan_wg : 28 * (slots > 0 ? slots : 4 * c->n_cu);
             p = make_segmented_plan(n_targets, n_src_pad, T, 256, segments, 1, slots);
-            p.src_step = coprime_step(p.n_src_tiles);
         }
+        p.src_step = coprime_step(p.n_src_tiles);
         p.crossed = 1;
         return p;
     }
@@ -380,16 +403,22 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
         return p;
     }
     if (mixed || c->chunked_only || c->plan_chunk > 0) {
-        // Chunked split: many short work items, so the last, partly filled
-        // round of resident workgroups and the costlier items
-        // (self-exclusion tiles; the mixed kernel's careful tiles, hence its
-        // finer split) cost little.  Chunks are never shorter than one tile.
-        // The mixed kernel's chunks (ensemble members, explicit chunk)
-        // interleave their source tiles, so its items cost alike; ~64k of
-        // them balance the launch, fewer leave a slow last round (level 7:
-        // 4.98 ms at 57k items, 5.13 at 16k).
+        // Segments of an explicit length (an explicit chunk, or the
+        // ensemble's members, whose segments the batched launch deals out):
+        // chunk / tile items each, the length the former chunked split gave
+        // its work items by default (~64k items for the mixed pass, ~16k for
+        // float64), so every member's rows number segments + tiles.  The
+        // mixed pass visits each target tile's source tiles in stride order.
         const int wg = c->plan_wg ? c->plan_wg : (mixed ? 65536 : 16384);
-        PairPlan p = make_plan(n_targets, n_src_pad, T, tile, wg, recip_for(c), c->plan_chunk);
+        const int chunk = c->plan_chunk > 0
+                              ? round_up(c->plan_chunk, tile)
+                              : make_plan(n_targets, n_src_pad, T, tile, wg, 1).chunk;
+        const int recip = recip_for(c);
+        const int slots = (mixed ? pair_mixed_blocks_per_cu(T, c->precision == BVE_PRECISION_MIXED_KAHAN)
+                                 : pair_f64_blocks_per_cu(T, tile, recip)) * c->n_cu;
+        PairPlan p = make_len_plan(n_targets, n_src_pad, T, tile, chunk / tile, mixed ? 1 : recip,
+                                   slots);
+        if (mixed) p.src_step = coprime_step(p.n_src_tiles);
         p.crossed = mixed ? 0 : 1;
         return p;
     }
@@ -1937,7 +1966,7 @@ struct bve_ensemble {
     PairArgs *d_pair = nullptr;
     MixedArgs *d_mixed = nullptr;
     StepState *d_states = nullptr;
-    int *d_chunks = nullptr;
+    PairPlan *d_plans = nullptr;
     int2 *d_stage_map = nullptr, *d_src_map = nullptr, *d_tgt_map = nullptr;
     unsigned long long *d_status = nullptr;   // numeric status of every member's stages
     EnsLaunch launch{};
@@ -1980,7 +2009,7 @@ void ens_free_tables(bve_ensemble *e)
     dev_free(e->d_pair);
     dev_free(e->d_mixed);
     dev_free(e->d_states);
-    dev_free(e->d_chunks);
+    dev_free(e->d_plans);
     dev_free(e->d_stage_map);
     dev_free(e->d_src_map);
     dev_free(e->d_tgt_map);
@@ -2014,7 +2043,7 @@ int ens_build(bve_ensemble *e)
     std::vector<PairArgs> pairs;
     std::vector<MixedArgs> mixeds;
     std::vector<StepState> states;
-    std::vector<int> chunks;
+    std::vector<PairPlan> plans;
     // (targets-per-lane group, -cost): groups in order, largest items first
     std::vector<std::pair<std::pair<int, long long>, EnsItem>> costed;
     std::vector<int2> stage_map, src_map, tgt_map;
@@ -2038,13 +2067,12 @@ int ens_build(bve_ensemble *e)
         states.push_back(step_state(c));
         states.back().status = e->d_status;
         states.back().member = (unsigned)m;
-        chunks.push_back(c->plan.n_chunks);
+        plans.push_back(c->plan);
         const int T = c->plan.targets_per_lane;
         const int group = T == 1 ? 0 : (T == 2 ? 1 : 2);
         const long long cost = (long long)kBlock * T * c->plan.chunk;
-        for (int ch = 0; ch < c->plan.n_chunks; ++ch)
-            for (int t = 0; t < c->plan.n_tiles; ++t)
-                costed.push_back({{group, -cost}, EnsItem{m, ch, t}});
+        for (int sg = 0; sg < c->plan.segments; ++sg)
+            costed.push_back({{group, -cost}, EnsItem{m, sg}});
         for (int b = 0; b < ceil_div(c->n_targets, 256); ++b) stage_map.push_back(make_int2(m, b));
         for (int b = 0; b < c->n_src_pad / 256; ++b) src_map.push_back(make_int2(m, b));
         for (int b = 0; b < c->t_stride / 256; ++b) tgt_map.push_back(make_int2(m, b));
@@ -2069,7 +2097,7 @@ int ens_build(bve_ensemble *e)
     if (!rc) rc = ens_upload(e, e->d_pair, pairs);
     if (!rc) rc = ens_upload(e, e->d_mixed, mixeds);
     if (!rc) rc = ens_upload(e, e->d_states, states);
-    if (!rc) rc = ens_upload(e, e->d_chunks, chunks);
+    if (!rc) rc = ens_upload(e, e->d_plans, plans);
     if (!rc) rc = ens_upload(e, e->d_stage_map, stage_map);
     if (!rc) rc = ens_upload(e, e->d_src_map, src_map);
     if (!rc) rc = ens_upload(e, e->d_tgt_map, tgt_map);
@@ -2095,7 +2123,7 @@ int ens_build(bve_ensemble *e)
         L.group_grid[g] = std::max(1, std::min(group_count[g], slots));
     }
     L.states = e->d_states;
-    L.n_chunks = e->d_chunks;
+    L.plans = e->d_plans;
     L.stage_map = e->d_stage_map;
     L.n_stage_blocks = (int)stage_map.size();
     L.src_map = e->d_src_map;
@@ -2156,7 +2184,6 @@ int ens_enqueue(bve_ensemble *e, double dt, int n_steps, bool timed)
             }
             StageParams sp{};
             sp.stage = s;
-            sp.plan.crossed = L.mixed ? 0 : 1;   // float64 rows hold cross-product sums
             sp.dt = dt;
             sp.omega = omega;
             sp.a_next = a_next[s];

@@ -295,17 +295,6 @@ __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int
     }
 }
 
-// Work item (target tile ttile, source chunk `chunk`) of a chunked float64 pass.
-template <int T, int TILE, int R>
-__device__ __forceinline__ void pair_f64_item(const PairArgs &a, int chunk, int ttile,
-                                              StagedSource *lds)
-{
-    const int j_begin = chunk * a.plan.chunk;
-    const int j_end = min(j_begin + a.plan.chunk, a.n_src_pad);
-    pair_f64_range<T, TILE, R>(a, ttile, j_begin, j_end, a.partial + (size_t)(chunk * 3) * a.t_stride,
-                               a.t_stride, 0, lds);
-}
-
 // Work queue of the persistent launches: the workgroup takes item indices
 // from queue[0] until n are handed out, running f(k) for each; the last
 // workgroup to finish clears both counters for the next launch (so graph
@@ -330,27 +319,44 @@ __device__ __forceinline__ void run_queue(unsigned *queue, int n, int *slot, F &
     }
 }
 
-// Segments of the persistent passes: segment s covers items [s seg_len,
-// (s + 1) seg_len) of the tile-major sequence (item = target tile *
-// n_src_tiles + source tile); range(t, j_begin, j_end, s) runs every maximal
-// run of it inside one target tile t, and writes row s + t (unique: s + t
-// increases along the sequence).
+// Segment s of a segmented pass: items [s seg_len, (s + 1) seg_len) of the
+// tile-major sequence (item = target tile * n_src_tiles + source tile);
+// range(t, j_begin, j_end, s) runs every maximal run of it inside one
+// target tile t, and writes row s + t (unique: s + t increases along the
+// sequence).
+template <int TILE, class Range>
+__device__ __forceinline__ void run_segment(const PairPlan &p, int seg, Range &&range)
+{
+    const int nst = p.n_src_tiles;
+    const int n_items = p.n_tiles * nst;
+    int b = seg * p.seg_len;
+    const int e = min(b + p.seg_len, n_items);
+    while (b < e) {
+        const int t = b / nst;
+        const int k0 = b - t * nst;
+        const int k1 = min(nst, k0 + (e - b));
+        range(t, k0 * TILE, k1 * TILE, seg);
+        b += k1 - k0;
+    }
+}
+
+// The persistent passes take their segments from a queue.
 template <int TILE, class Range>
 __device__ __forceinline__ void run_segments(const PairPlan &p, unsigned *queue, int *seg_slot,
                                              Range &&range)
 {
-    const int nst = p.n_src_tiles;
-    const int n_items = p.n_tiles * nst;
-    run_queue(queue, p.segments, seg_slot, [&](int seg) {
-        int b = seg * p.seg_len;
-        const int e = min(b + p.seg_len, n_items);
-        while (b < e) {
-            const int t = b / nst;
-            const int k0 = b - t * nst;
-            const int k1 = min(nst, k0 + (e - b));
-            range(t, k0 * TILE, k1 * TILE, seg);
-            b += k1 - k0;
-        }
+    run_queue(queue, p.segments, seg_slot, [&](int seg) { run_segment<TILE>(p, seg, range); });
+}
+
+// Segment `seg` of a float64 pass (one work item of the persistent and the
+// ensemble launches).
+template <int T, int TILE, int R>
+__device__ __forceinline__ void pair_f64_segment(const PairArgs &a, int seg, StagedSource *lds)
+{
+    constexpr int TT = kBlock * T;
+    run_segment<TILE>(a.plan, seg, [&](int t, int j0, int j1, int s) {
+        pair_f64_range<T, TILE, R>(a, t, j0, j1, a.partial + (size_t)(s + t) * 3 * TT, TT, t * TT,
+                                   lds);
     });
 }
 
@@ -359,11 +365,8 @@ template <int T, int TILE, int R>
 __device__ __forceinline__ void pair_f64_segments(const PairArgs &a, StagedSource *lds,
                                                   int *seg_slot)
 {
-    constexpr int TT = kBlock * T;
-    run_segments<TILE>(a.plan, a.queue, seg_slot, [&](int t, int j0, int j1, int seg) {
-        pair_f64_range<T, TILE, R>(a, t, j0, j1, a.partial + (size_t)(seg + t) * 3 * TT, TT,
-                                   t * TT, lds);
-    });
+    run_queue(a.queue, a.plan.segments, seg_slot,
+              [&](int seg) { pair_f64_segment<T, TILE, R>(a, seg, lds); });
 }
 
 // ---------------------------------------------------------------------------
@@ -608,35 +611,28 @@ __device__ __forceinline__ void pair_mixed_range(const MixedArgs &a, int ttile, 
     }
 }
 
-// Work item (source chunk `chunk`, target tile `ttile`) of a chunked plan.
-// Chunk c holds the source tiles c, c + n_chunks, c + 2 n_chunks, ...: the
-// careful tiles of a target group (its neighbours, adjacent in the source
-// order) fall into different items, so items cost alike and can be long
-// (few partial rows) without a tail of slow ones.
+// Segment `seg` of a mixed pass, each target tile's source tiles visited in
+// the plan's stride order (tile (k src_step) mod n_src_tiles for item k):
+// the careful tiles of a target group, its neighbours in the source order,
+// fall into different segments, so segments cost alike.
 template <int T, bool KAHAN>
-__device__ __forceinline__ void pair_mixed_item(const MixedArgs &a, int chunk, int ttile,
-                                                MixedLds &lds)
+__device__ __forceinline__ void pair_mixed_segment(const MixedArgs &a, int seg, MixedLds &lds)
 {
-    const int nst = a.base.n_src_pad / kMixedTile;
-    const int n_chunks = a.base.plan.n_chunks;
-    pair_mixed_range<T, KAHAN>(a, ttile, chunk, n_chunks, 0, ceil_div(nst - chunk, n_chunks),
-                               a.base.partial + (size_t)(chunk * 3) * a.base.t_stride,
-                               a.base.t_stride, 0, lds);
+    constexpr int TT = kBlock * T;
+    run_segment<kMixedTile>(a.base.plan, seg, [&](int t, int j0, int j1, int s) {
+        pair_mixed_range<T, KAHAN>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile,
+                                   j1 / kMixedTile, a.base.partial + (size_t)(s + t) * 3 * TT, TT,
+                                   t * TT, lds);
+    });
 }
 
-// Persistent body of a segmented mixed pass (the float64 pass's segment
-// queue and partial rows, source tiles in the plan's stride order).
+// Persistent body of a segmented mixed pass.
 template <int T, bool KAHAN>
 __device__ __forceinline__ void pair_mixed_segments(const MixedArgs &a, MixedLds &lds,
                                                     int *seg_slot)
 {
-    constexpr int TT = kBlock * T;
-    run_segments<kMixedTile>(a.base.plan, a.base.queue, seg_slot,
-                             [&](int t, int j0, int j1, int seg) {
-        pair_mixed_range<T, KAHAN>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile,
-                                   j1 / kMixedTile, a.base.partial + (size_t)(seg + t) * 3 * TT,
-                                   TT, t * TT, lds);
-    });
+    run_queue(a.base.queue, a.base.plan.segments, seg_slot,
+              [&](int seg) { pair_mixed_segment<T, KAHAN>(a, seg, lds); });
 }
 
 // ---------------------------------------------------------------------------
@@ -894,19 +890,6 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
 __device__ __forceinline__ double bounded_tau(const MixedArgs &a)
 {
     return 0.3 * a.err_target * *a.scale;
-}
-
-// Work item (source chunk `chunk`, target tile `ttile`) of a chunked bounded
-// plan; chunk c holds the source tiles c, c + n_chunks, ...
-template <int T>
-__device__ __forceinline__ void pair_bounded_item(const MixedArgs &a, int chunk, int ttile,
-                                                  BoundedLds &lds)
-{
-    const int nst = a.base.n_src_pad / kMixedTile;
-    const int n_chunks = a.base.plan.n_chunks;
-    pair_bounded_range<T>(a, ttile, chunk, n_chunks, 0, ceil_div(nst - chunk, n_chunks),
-                          a.base.partial + (size_t)(chunk * 3) * a.base.t_stride,
-                          a.base.t_stride, 0, lds, bounded_tau(a));
 }
 
 // Persistent body of a segmented bounded pass (source tiles in stride order).

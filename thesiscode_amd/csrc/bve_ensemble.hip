@@ -1,5 +1,6 @@
 // Batched ensemble launches: one persistent pair-sum launch over every
-// member's (target tile, source chunk) work items, and block-mapped launches
+// member's segments (runs of its tile-major (target tile, source tile)
+// sequence, the single-context pass's work items), and block-mapped launches
 // of the RK4 stage update and the per-stage preparation for all members.
 //
 // Members have uneven sizes (e.g. cubed spheres at several levels).  The
@@ -28,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void ens_pair_f64_kernel(const EnsItem *__r
     run_queue(queue, n_items, &slot, [&](int k) {
         const EnsItem it = items[k];
         const PairArgs a = members[it.member];
-        pair_f64_item<T, 256, R>(a, it.chunk, it.ttile, lds);
+        pair_f64_segment<T, 256, R>(a, it.segment, lds);
     });
 }
 
@@ -43,20 +44,19 @@ __global__ __launch_bounds__(kBlock) void ens_pair_mixed_kernel(const EnsItem *_
     run_queue(queue, n_items, &slot, [&](int k) {
         const EnsItem it = items[k];
         const MixedArgs a = members[it.member];
-        pair_mixed_item<T, KAHAN>(a, it.chunk, it.ttile, lds);
+        pair_mixed_segment<T, KAHAN>(a, it.segment, lds);
     });
 }
 
 __global__ __launch_bounds__(256) void ens_stage_kernel(const int2 *__restrict__ map,
                                                         const StepState *__restrict__ states,
-                                                        const int *__restrict__ n_chunks,
+                                                        const PairPlan *__restrict__ plans,
                                                         StageParams sp)
 {
     __shared__ double sh[256];
     const int2 mb = map[blockIdx.x];
-    // ensemble members run chunked plans
-    sp.plan.segments = 0;
-    sp.plan.n_chunks = n_chunks[mb.x];
+    // each member's own segmented plan (its rows and their order)
+    sp.plan = plans[mb.x];
     stage_update_body(states[mb.x], sp, mb.y, sh);
 }
 
@@ -158,7 +158,7 @@ hipError_t launch_ensemble_mixed_prep(const EnsLaunch &e, hipStream_t s)
 hipError_t launch_ensemble_stage(const EnsLaunch &e, const StageParams &sp, hipStream_t s)
 {
     hipLaunchKernelGGL(ens_stage_kernel, dim3(e.n_stage_blocks), dim3(256), 0, s, e.stage_map,
-                       e.states, e.n_chunks, sp);
+                       e.states, e.plans, sp);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || sp.stage != 0) return err;
     hipLaunchKernelGGL(ens_ke_kernel, dim3(e.n_members), dim3(256), 0, s, e.states);

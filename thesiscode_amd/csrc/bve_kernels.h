@@ -189,9 +189,10 @@ hipError_t launch_diff_pair(const DiffState &st, const PairPlan &plan, hipStream
 hipError_t launch_diff_stage(const DiffState &st, const PairPlan &plan, int stage, double dt,
                              double omega, hipStream_t s);
 
-// Ensemble: one work item of the persistent pair launch.
+// Ensemble: one work item of the persistent pair launch, a segment of a
+// member's segmented plan.
 struct EnsItem {
-    int member, chunk, ttile;
+    int member, segment;
 };
 
 // Device tables of an ensemble launch (all pointers are device memory).
@@ -209,7 +210,7 @@ struct EnsLaunch {
     const MixedArgs *mixed_members;  // mixed modes
     int mixed, kahan, recip;
     const StepState *states;
-    const int *n_chunks;             // per member
+    const PairPlan *plans;           // per member (segmented)
     const int2 *stage_map;           // (member, block) of every stage-update block
     int n_stage_blocks;
     const int2 *src_map;             // (member, source tile) of the mixed preparation

@@ -8,18 +8,8 @@
 namespace bve {
 
 // ---------------------------------------------------------------------------
-// float64 pair pass: one workgroup per (target tile, source chunk)
+// float64 pair pass
 // ---------------------------------------------------------------------------
-
-template <int T, int TILE, int R>
-__global__ __launch_bounds__(kBlock) void pair_sum_f64_kernel(PairArgs a)
-{
-    const int work = xcd_work_item(blockIdx.x, a.plan.grid);
-    if (work >= a.plan.n_work) return;
-    __shared__ StagedSource lds[TILE];
-    const int chunk = work / a.plan.n_tiles;
-    pair_f64_item<T, TILE, R>(a, chunk, work - chunk * a.plan.n_tiles, lds);
-}
 
 // Segmented pass: a persistent grid (about one workgroup per resident slot)
 // takes segments from the queue.
@@ -34,14 +24,9 @@ __global__ __launch_bounds__(kBlock) void pair_sum_f64_seg_kernel(PairArgs a)
 template <int T, int TILE, int R>
 static hipError_t launch_f64(const PairArgs &a, hipStream_t s)
 {
-    if (a.plan.segments > 0) {
-        if (!a.queue) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((pair_sum_f64_seg_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock),
-                           0, s, a);
-    } else {
-        hipLaunchKernelGGL((pair_sum_f64_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0,
-                           s, a);
-    }
+    if (a.plan.segments <= 0 || !a.queue) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((pair_sum_f64_seg_kernel<T, TILE, R>), dim3(a.plan.grid), dim3(kBlock), 0,
+                       s, a);
     return hipGetLastError();
 }
 

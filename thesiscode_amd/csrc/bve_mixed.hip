@@ -24,16 +24,6 @@ __global__ __launch_bounds__(256) void mixed_target_prep_kernel(const PairArgs b
 }
 
 template <int T, bool KAHAN>
-__global__ __launch_bounds__(kBlock) void pair_sum_mixed_kernel(MixedArgs a)
-{
-    const int work = xcd_work_item(blockIdx.x, a.base.plan.grid);
-    if (work >= a.base.plan.n_work) return;
-    __shared__ MixedLds lds;
-    const int chunk = work / a.base.plan.n_tiles;
-    pair_mixed_item<T, KAHAN>(a, chunk, work - chunk * a.base.plan.n_tiles, lds);
-}
-
-template <int T, bool KAHAN>
 __global__ __launch_bounds__(kBlock) void pair_sum_mixed_seg_kernel(MixedArgs a)
 {
     __shared__ MixedLds lds;
@@ -44,14 +34,10 @@ __global__ __launch_bounds__(kBlock) void pair_sum_mixed_seg_kernel(MixedArgs a)
 template <int T, bool KAHAN>
 static hipError_t launch_mixed_tk(const MixedArgs &a, hipStream_t s)
 {
-    if (a.base.plan.segments > 0) {
-        if (!a.base.queue || a.base.plan.src_step < 1) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((pair_sum_mixed_seg_kernel<T, KAHAN>), dim3(a.base.plan.grid),
-                           dim3(kBlock), 0, s, a);
-    } else {
-        hipLaunchKernelGGL((pair_sum_mixed_kernel<T, KAHAN>), dim3(a.base.plan.grid), dim3(kBlock),
-                           0, s, a);
-    }
+    if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((pair_sum_mixed_seg_kernel<T, KAHAN>), dim3(a.base.plan.grid), dim3(kBlock),
+                       0, s, a);
     return hipGetLastError();
 }
 
@@ -137,16 +123,6 @@ __global__ __launch_bounds__(256) void bounded_probe_kernel(const PairArgs b, in
 // 4 waves per SIMD (at most 128 VGPRs): the remainder loop needs the
 // latency hiding more than the registers
 template <int T>
-__global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_kernel(MixedArgs a)
-{
-    const int work = xcd_work_item(blockIdx.x, a.base.plan.grid);
-    if (work >= a.base.plan.n_work) return;
-    __shared__ BoundedLds lds;
-    const int chunk = work / a.base.plan.n_tiles;
-    pair_bounded_item<T>(a, chunk, work - chunk * a.base.plan.n_tiles, lds);
-}
-
-template <int T>
 __global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(MixedArgs a)
 {
     __shared__ BoundedLds lds;
@@ -157,14 +133,10 @@ __global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(MixedAr
 template <int T>
 static hipError_t launch_bounded_t(const MixedArgs &a, hipStream_t s)
 {
-    if (a.base.plan.segments > 0) {
-        if (!a.base.queue || a.base.plan.src_step < 1) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T>), dim3(a.base.plan.grid),
-                           dim3(kBlock), 0, s, a);
-    } else {
-        hipLaunchKernelGGL((pair_sum_bounded_kernel<T>), dim3(a.base.plan.grid), dim3(kBlock),
-                           0, s, a);
-    }
+    if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T>), dim3(a.base.plan.grid), dim3(kBlock), 0,
+                       s, a);
     return hipGetLastError();
 }
 

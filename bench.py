@@ -166,8 +166,10 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
     dt = 0.01, checkpoints every 5, J = sum_i A_i zeta_i(T) g(x_i(T)) with a
     width-0.2 Gaussian weight at 30N 90E.  Times the gradient (forward sweep
     with checkpoints + reverse sweep) against the forward solve of the same
-    steps with all three target classes, and checks dJ/dzeta_j(0) against
-    central differences of the GPU forward map on n_fd seeded panels."""
+    steps on the active panels alone (the map J depends on), with vertices,
+    and with all three target classes; times it again with every stage
+    input kept (no recomputation), and checks dJ/dzeta_j(0) against central
+    differences of the GPU forward map on n_fd seeded panels."""
     import math
     from thesiscode_amd import _lib
     from thesiscode_amd import grid as gridmod
@@ -211,6 +213,13 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
     t0 = time.perf_counter()
     J, gz, gx = gradient(ctx, F, g.active_area, dt, steps, every)
     t_adj = time.perf_counter() - t0
+    # every stage input kept from the forward sweep (one checkpoint): no
+    # recomputation, 56 bytes per panel and stage resident in HBM
+    gradient(ctx, F, g.active_area, dt, steps, steps)
+    t0 = time.perf_counter()
+    J1, gz1, _ = gradient(ctx, F, g.active_area, dt, steps, steps)
+    t_adj_stored = time.perf_counter() - t0
+    stored_diff = float(np.abs(gz1 - gz).max() / np.abs(gz).max())
 
     idx = np.random.default_rng(0).choice(g.n_active, n_fd, replace=False)
 
@@ -246,6 +255,9 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
                 forward_ms_active_and_vertices=t_fwd_vertices * 1e3,
                 forward_ms_all_classes=t_fwd * 1e3,
                 adjoint_fwd_ratio_same_map=t_adj / t_fwd_active,
+                gradient_ms_all_stages_stored=t_adj_stored * 1e3,
+                adjoint_fwd_ratio_same_map_all_stages_stored=t_adj_stored / t_fwd_active,
+                all_stages_stored_vs_checkpointed_max_rel_diff=stored_diff,
                 adjoint_fwd_ratio_active_and_vertices=t_adj / t_fwd_vertices,
                 adjoint_fwd_ratio_all_classes=t_adj / t_fwd,
                 fd_panels=int(n_fd), fd=fd_report)

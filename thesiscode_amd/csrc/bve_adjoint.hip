@@ -13,13 +13,13 @@
 //   dL/dzeta_j = -(1/4 PI) A_j sum_i P_ij / D_ij
 //
 // Targets and sources are the same active panels, so one pass evaluates
-// all four sums: panel k in a lane, the records {x_l, w_l, n_l} of every
+// the pair sums: panel k in a lane, the records {x_l, w_l, n_l} of every
 // other panel streamed through LDS; per pair the denominator and its
-// reciprocal are shared by the target-side terms (S, T: k as target, l as
-// source) and the source-side ones (H, g: k as source, l as target).  The
-// pass uses the forward kernel's tiling, chunking, XCD mapping and batched
-// float64 reciprocals, and writes per-chunk partial sums that the combine
-// kernel adds in a fixed order.
+// reciprocal are shared by the target-side terms (T: k as target, l as
+// source) and the source-side ones (H: k as source, l as target).  The
+// pass runs the forward kernel's segment queue, tiling and batched float64
+// reciprocals, and writes per-segment partial sums that the combine kernel
+// adds in a fixed order; S comes from the forward sweep.
 #include "bve_common.h"
 #include "bve_device.h"
 #include "bve_kernels.h"
@@ -47,7 +47,7 @@ __device__ __forceinline__ void recips(const double (&d)[R], double (&r)[R])
 }
 
 // ---------------------------------------------------------------------------
-// Pair pass: S, V, N and g of panel k (as target and as source)
+// Pair pass: V and N of panel k (as target and as source)
 // ---------------------------------------------------------------------------
 //
 // With the target-side T_k = sum_l (w_l P_kl / d^2) x_l and the source-side
@@ -56,14 +56,59 @@ __device__ __forceinline__ void recips(const double (&d)[R], double (&r)[R])
 //
 //     V_k = sum_l (w_l P_kl + w_k P_lk) / d^2  x_l,    N_k = sum_l n_l / d,
 //
-// (P_kl = x_l . n_k, P_lk = x_k . n_l, d = 1 - x_k . x_l), so a pair adds
-// to four sums S, V, N, g instead of five (26.75 instead of 29.75 float64
-// operations per pair with four pairs sharing a reciprocal).
+// (P_kl = x_l . n_k, P_lk = x_k . n_l, d = 1 - x_k . x_l).  The two other
+// sums of the gradient come without a pass of their own: S_k is the forward
+// sum of the same stage input, kept by the forward sweep, and
+// g_k = sum_l P_lk / d = x_k . N_k.  A pair costs 22 float64 operations
+// with four pairs sharing a reciprocal.
+
+// Source tile at jt (in LDS) against the T targets of this lane; MASK: the
+// tile holds the own index of some lane's target (self pairs are dropped).
+template <int T, int R, bool MASK>
+__device__ __forceinline__ void vjp_tile(const AdjRec *lds, int jt, const int (&self)[T],
+                                         const double (&xk)[T], const double (&yk)[T],
+                                         const double (&zk)[T], const double (&wk)[T],
+                                         const double (&ex)[T], const double (&ey)[T],
+                                         const double (&ez)[T], double (&Vx)[T], double (&Vy)[T],
+                                         double (&Vz)[T], double (&Nx)[T], double (&Ny)[T],
+                                         double (&Nz)[T])
+{
+    constexpr int TILE = 256;
+    for (int k = 0; k < TILE; k += R) {
+        AdjRec a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = lds[k + r];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            double d[R], rc[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                d[r] = __builtin_fma(-xk[t], a[r].x, 1.0);
+                d[r] = __builtin_fma(-yk[t], a[r].y, d[r]);
+                d[r] = __builtin_fma(-zk[t], a[r].z, d[r]);
+                if (MASK && jt + k + r == self[t]) d[r] = 1.0;
+            }
+            recips<R>(d, rc);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double rr = rc[r];
+                if (MASK && jt + k + r == self[t]) rr = 0.0;
+                const double pt = __builtin_fma(a[r].z, ez[t], __builtin_fma(a[r].y, ey[t], a[r].x * ex[t]));
+                const double ps = __builtin_fma(zk[t], a[r].nz, __builtin_fma(yk[t], a[r].ny, xk[t] * a[r].nx));
+                const double gam = (rr * rr) * __builtin_fma(a[r].w, pt, wk[t] * ps);
+                Vx[t] = __builtin_fma(gam, a[r].x, Vx[t]);
+                Vy[t] = __builtin_fma(gam, a[r].y, Vy[t]);
+                Vz[t] = __builtin_fma(gam, a[r].z, Vz[t]);
+                Nx[t] = __builtin_fma(rr, a[r].nx, Nx[t]);
+                Ny[t] = __builtin_fma(rr, a[r].ny, Ny[t]);
+                Nz[t] = __builtin_fma(rr, a[r].nz, Nz[t]);
+            }
+        }
+    }
+}
 
 // Target tile `ttile` against source tiles [j_begin, j_end) (multiples of
-// 256): rows q of target i go to out[q * out_stride + i - out_shift].  The
-// diagonal mask is a run-time select (a compile-time split of the loop, as
-// in the forward pass, takes this kernel from 162 to 256 VGPRs).
+// 256): rows q of target i go to out[q * out_stride + i - out_shift].
 template <int T, int R>
 __device__ __forceinline__ void vjp_range(const AdjRec *__restrict__ rec, int self_end, int ttile,
                                           int j_begin, int j_end, double *out, int out_stride,
@@ -73,10 +118,12 @@ __device__ __forceinline__ void vjp_range(const AdjRec *__restrict__ rec, int se
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     double xk[T], yk[T], zk[T], wk[T], ex[T], ey[T], ez[T];
-    double Sx[T], Sy[T], Sz[T], Vx[T], Vy[T], Vz[T], Nx[T], Ny[T], Nz[T], G[T];
+    double Vx[T], Vy[T], Vz[T], Nx[T], Ny[T], Nz[T];
+    int self[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const AdjRec a = rec[tile_base + t * kBlock + threadIdx.x];
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        const AdjRec a = rec[i];
         xk[t] = a.x;
         yk[t] = a.y;
         zk[t] = a.z;
@@ -84,8 +131,8 @@ __device__ __forceinline__ void vjp_range(const AdjRec *__restrict__ rec, int se
         ex[t] = a.nx;
         ey[t] = a.ny;
         ez[t] = a.nz;
-        Sx[t] = Sy[t] = Sz[t] = Vx[t] = Vy[t] = Vz[t] = 0.0;
-        Nx[t] = Ny[t] = Nz[t] = G[t] = 0.0;
+        self[t] = i < self_end ? i : -1;
+        Vx[t] = Vy[t] = Vz[t] = Nx[t] = Ny[t] = Nz[t] = 0.0;
     }
     for (int jt = j_begin; jt < j_end; jt += TILE) {
         __syncthreads();
@@ -95,59 +142,28 @@ __device__ __forceinline__ void vjp_range(const AdjRec *__restrict__ rec, int se
 #pragma unroll
         for (int t = 0; t < T; ++t)
             diag |= diag_tile(tile_base + t * kBlock + wave * 64, self_end, jt, TILE);
-        for (int k = 0; k < TILE; k += R) {
-            AdjRec a[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) a[r] = lds[k + r];
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const int self = tile_base + t * kBlock + threadIdx.x;
-                double d[R], rc[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    d[r] = __builtin_fma(-xk[t], a[r].x, 1.0);
-                    d[r] = __builtin_fma(-yk[t], a[r].y, d[r]);
-                    d[r] = __builtin_fma(-zk[t], a[r].z, d[r]);
-                    if (diag && jt + k + r == self) d[r] = 1.0;
-                }
-                recips<R>(d, rc);
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    double rr = rc[r];
-                    if (diag && jt + k + r == self) rr = 0.0;
-                    const double pt = __builtin_fma(a[r].z, ez[t], __builtin_fma(a[r].y, ey[t], a[r].x * ex[t]));
-                    const double ps = __builtin_fma(zk[t], a[r].nz, __builtin_fma(yk[t], a[r].ny, xk[t] * a[r].nx));
-                    const double q = a[r].w * rr;
-                    const double gam = (rr * rr) * __builtin_fma(a[r].w, pt, wk[t] * ps);
-                    Sx[t] = __builtin_fma(q, a[r].x, Sx[t]);
-                    Sy[t] = __builtin_fma(q, a[r].y, Sy[t]);
-                    Sz[t] = __builtin_fma(q, a[r].z, Sz[t]);
-                    Vx[t] = __builtin_fma(gam, a[r].x, Vx[t]);
-                    Vy[t] = __builtin_fma(gam, a[r].y, Vy[t]);
-                    Vz[t] = __builtin_fma(gam, a[r].z, Vz[t]);
-                    Nx[t] = __builtin_fma(rr, a[r].nx, Nx[t]);
-                    Ny[t] = __builtin_fma(rr, a[r].ny, Ny[t]);
-                    Nz[t] = __builtin_fma(rr, a[r].nz, Nz[t]);
-                    G[t] = __builtin_fma(ps, rr, G[t]);
-                }
-            }
-        }
+        if (diag)
+            vjp_tile<T, R, true>(lds, jt, self, xk, yk, zk, wk, ex, ey, ez, Vx, Vy, Vz, Nx, Ny, Nz);
+        else
+            vjp_tile<T, R, false>(lds, jt, self, xk, yk, zk, wk, ex, ey, ez, Vx, Vy, Vz, Nx, Ny, Nz);
     }
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const int i = tile_base + t * kBlock + threadIdx.x - out_shift;
-        const double v[kAdjRows] = {Sx[t], Sy[t], Sz[t], Vx[t], Vy[t], Vz[t], Nx[t], Ny[t], Nz[t], G[t]};
+        const double v[kAdjRows] = {Vx[t], Vy[t], Vz[t], Nx[t], Ny[t], Nz[t]};
 #pragma unroll
         for (int q = 0; q < kAdjRows; ++q) out[(size_t)q * out_stride + i] = v[q];
     }
 }
 
 // Segmented persistent pass (the forward pass's queue and row layout:
-// segment s writes rows (s + tile) of kAdjRows x kBlock T entries).
+// segment s writes rows (s + tile) of kAdjRows x kBlock T entries).  The
+// register cap keeps 3 waves per SIMD at T = 2 (168 VGPRs, no spills) and 2
+// at T = 4.
 template <int T, int R>
-__global__ __launch_bounds__(kBlock) void vjp_kernel(const AdjRec *__restrict__ rec, int self_end,
-                                                     PairPlan plan, double *__restrict__ partial,
-                                                     unsigned *queue)
+__global__ __launch_bounds__(kBlock, T == 4 ? 2 : 3) void vjp_kernel(
+    const AdjRec *__restrict__ rec, int self_end, PairPlan plan, double *__restrict__ partial,
+    unsigned *queue)
 {
     constexpr int TT = kBlock * T;
     __shared__ AdjRec lds[256];
@@ -205,7 +221,8 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(AdjStage a)
                       mx * y.y - my * y.x, 0.0};
 }
 
-// Ybar of stage s from the pair pass's segment rows; lam_acc += Ybar.
+// Ybar of stage s from the pair pass's segment rows and the forward sums S
+// of the stage input; lam_acc += Ybar.
 __global__ __launch_bounds__(256) void adj_combine_kernel(AdjStage a)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -226,14 +243,16 @@ __global__ __launch_bounds__(256) void adj_combine_kernel(AdjStage a)
         for (int q = 0; q < kAdjRows; ++q) v[q] += row[(size_t)q * tt];
     }
     const Source y = a.ys[i];
+    const double Sx = a.sums[i], Sy = a.sums[st + i], Sz = a.sums[2 * st + i];
     const double mx = a.m[i], my = a.m[st + i], mz = a.m[2 * st + i];
     const double c4 = -4.0 * kPi;
-    // dL/dx = (S x m + V + w N) / (-4 PI),  dL/dzeta = A g / (-4 PI)
-    const double gx = (v[1] * mz - v[2] * my) + v[3] + y.w * v[6];
-    const double gy = (v[2] * mx - v[0] * mz) + v[4] + y.w * v[7];
-    const double gz = (v[0] * my - v[1] * mx) + v[5] + y.w * v[8];
+    // dL/dx = (S x m + V + w N) / (-4 PI),  dL/dzeta = A (x . N) / (-4 PI)
+    const double gx = (Sy * mz - Sz * my) + v[0] + y.w * v[3];
+    const double gy = (Sz * mx - Sx * mz) + v[1] + y.w * v[4];
+    const double gz = (Sx * my - Sy * mx) + v[2] + y.w * v[5];
+    const double g = (y.x * v[3] + y.y * v[4]) + y.z * v[5];
     const double bx = gx / c4, by = gy / c4, bz = gz / c4;
-    const double bzeta = a.area[i] * v[9] / c4;
+    const double bzeta = a.area[i] * g / c4;
     a.ybar[i] = bx;
     a.ybar[st + i] = by;
     a.ybar[2 * st + i] = bz;
@@ -262,7 +281,7 @@ static hipError_t launch_vjp_t(const AdjStage &a, hipStream_t s)
 hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s)
 {
     if (a.plan.tile != 256 || a.n_pad % 256 != 0 || a.stride < a.n_pad || a.plan.segments <= 0 ||
-        !a.queue || a.plan.n_tiles * kBlock * a.plan.targets_per_lane > a.n_pad)
+        !a.queue || !a.sums || a.plan.n_tiles * kBlock * a.plan.targets_per_lane > a.n_pad)
         return hipErrorInvalidValue;
     const int blocks = ceil_div(a.n_pad, 256);
     hipLaunchKernelGGL(adj_prep_kernel, dim3(blocks), dim3(256), 0, s, a);

@@ -174,10 +174,11 @@ struct bve_ctx {
     bool adj_last_stored = false;   // adj_stages holds the last segment's stages
     double *adj_ckpt = nullptr;     // adj_ckpts x 4 rows (x, y, z, zeta) x adj_pad
     Source *adj_stages = nullptr;   // adj_every steps x 4 stages x adj_pad records
+    double *adj_sums = nullptr;     // their forward sums S: adj_every x 4 x 3 rows of adj_pad
     double *adj_lam = nullptr, *adj_lam_acc = nullptr, *adj_ybar = nullptr, *adj_m = nullptr;
     AdjRec *adj_rec = nullptr;
     double *adj_pt = nullptr;   // partial rows of the adjoint's pair pass
-    size_t adj_pt_cap = 0, adj_ckpt_cap = 0, adj_stage_cap = 0;
+    size_t adj_pt_cap = 0, adj_ckpt_cap = 0, adj_stage_cap = 0, adj_sums_cap = 0;
 
     // profiling
     bool profiling = false;
@@ -668,11 +669,15 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
 // n_steps RK4 steps of the first n_targets targets ([active | passive |
 // particles] prefix) with launch plan `plan`.  If stage_store is given, the
 // source records of every stage input are copied there, stage_stride records
-// apart, in step-major order (the adjoint's recomputation).  forced: the
-// StratosphereRK4 variant, step n starting at time t + n dt.
+// apart, in step-major order, and the active targets' factored sums S of
+// every stage go to sum_store, 3 rows of stage_stride per stage (the
+// adjoint's forward sweep and recomputation; the plan must not be crossed).
+// forced: the StratosphereRK4 variant, step n starting at time t + n dt.
 int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPlan &plan,
-                 Source *stage_store, size_t stage_stride, bool forced = false, double t = 0.0)
+                 Source *stage_store, size_t stage_stride, bool forced = false, double t = 0.0,
+                 double *sum_store = nullptr)
 {
+    if (sum_store && (plan.crossed || plan.exact)) return c->fail(BVE_ERR_STATE, "run_steps_on: sums of a crossed plan");
     StepState st = step_state(c);
     st.n_targets = n_targets;
     if (forced) {
@@ -704,6 +709,10 @@ int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPl
             sp.renormalize = c->renormalize;
             // t + dt/2, t + dt/2, t + dt, t + dt (PanelVelocityParallel.f90:746, :794, :839, :908)
             sp.t_next = t_frac[s] == 1.0 ? tn + dt : tn + t_frac[s] * dt;
+            if (sum_store) {
+                sp.sums_out = sum_store + (size_t)(4 * n + s) * 3 * stage_stride;
+                sp.sums_stride = (int)stage_stride;
+            }
             HIP_TRY(c, launch_stage_update(st, sp, c->stream));
         }
     }
@@ -721,13 +730,14 @@ void free_adjoint(bve_ctx *c)
 {
     dev_free(c->adj_ckpt);
     dev_free(c->adj_stages);
+    dev_free(c->adj_sums);
     dev_free(c->adj_lam);
     dev_free(c->adj_lam_acc);
     dev_free(c->adj_ybar);
     dev_free(c->adj_m);
     dev_free(c->adj_rec);
     dev_free(c->adj_pt);
-    c->adj_pt_cap = c->adj_ckpt_cap = c->adj_stage_cap = 0;
+    c->adj_pt_cap = c->adj_ckpt_cap = c->adj_stage_cap = c->adj_sums_cap = 0;
     c->adj_ready = false;
     c->adj_last_stored = false;
     c->adj_n = c->adj_pad = 0;
@@ -767,6 +777,7 @@ int copy_active_state(bve_ctx *c, double *rows, bool save)
 int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
 {
     fwd = plan_for(c, c->n_active, c->n_src_pad);
+    fwd.crossed = 0;   // the rows hold S, which the reverse sweep reuses
     int rc = ensure_partial(c, partial_doubles(fwd, c->t_stride));
     if (rc) return rc;
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_active);
@@ -784,8 +795,9 @@ int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
 }
 
 // One reverse RK4 step: lam <- lam + sum_s Ybar_s with the stage inputs in
-// `stages` (4 x adj_pad source records).
-int adjoint_reverse_step(bve_ctx *c, const Source *stages, const PairPlan &vjp)
+// `stages` (4 x adj_pad source records) and their forward sums in `sums`
+// (4 x 3 rows of adj_pad).
+int adjoint_reverse_step(bve_ctx *c, const Source *stages, const double *sums, const PairPlan &vjp)
 {
     static const double B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     static const double C[4] = {0.5, 0.5, 1.0, 0.0};   // a of the next stage's input
@@ -807,6 +819,7 @@ int adjoint_reverse_step(bve_ctx *c, const Source *stages, const PairPlan &vjp)
         a.m = c->adj_m;
         a.rec = c->adj_rec;
         a.ys = stages + (size_t)s * P;
+        a.sums = sums + (size_t)s * 3 * P;
         a.area = c->area;
         a.plan = vjp;
         a.partial = c->adj_pt;
@@ -1709,6 +1722,7 @@ int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every
     const size_t P = (size_t)c->adj_pad;
     int rc = grow(c, c->adj_ckpt, c->adj_ckpt_cap, (size_t)ckpts * 4 * P, true);
     if (!rc) rc = grow(c, c->adj_stages, c->adj_stage_cap, (size_t)every * 4 * P, true);
+    if (!rc) rc = grow(c, c->adj_sums, c->adj_sums_cap, (size_t)every * 4 * 3 * P, true);
     size_t cap4 = 0;
     if (!rc && !c->adj_lam) {
         rc = grow(c, c->adj_lam, cap4, 4 * P, true);
@@ -1732,7 +1746,8 @@ int bve_adjoint_forward(bve_ctx *c, double dt, int n_steps, int checkpoint_every
         const bool last = k == ckpts - 1;
         rc = copy_active_state(c, c->adj_ckpt + (size_t)k * 4 * P, true);
         if (!rc) rc = run_steps_on(c, dt, std::min(every, n_steps - k * every), c->n_active, fwd,
-                                   last ? c->adj_stages : nullptr, last ? P : 0);
+                                   last ? c->adj_stages : nullptr, last ? P : 0, false, 0.0,
+                                   last ? c->adj_sums : nullptr);
         if (rc) return rc;
     }
     c->adj_last_stored = true;
@@ -1859,12 +1874,14 @@ int bve_adjoint_backward(bve_ctx *c, const double *lambda_x, const double *lambd
         const int len = std::min(c->adj_every, c->adj_steps - n0);
         if (!(seg == c->adj_ckpts - 1 && c->adj_last_stored)) {
             rc = copy_active_state(c, c->adj_ckpt + (size_t)seg * 4 * P, false);
-            if (!rc) rc = run_steps_on(c, c->adj_dt, len, c->n_active, fwd, c->adj_stages, P);
+            if (!rc) rc = run_steps_on(c, c->adj_dt, len, c->n_active, fwd, c->adj_stages, P, false,
+                                       0.0, c->adj_sums);
             if (rc) return rc;
         }
         c->adj_last_stored = false;   // the buffer now goes to earlier segments
         for (int l = len - 1; l >= 0; --l) {
-            rc = adjoint_reverse_step(c, c->adj_stages + (size_t)l * 4 * P, vjp);
+            rc = adjoint_reverse_step(c, c->adj_stages + (size_t)l * 4 * P,
+                                      c->adj_sums + (size_t)l * 4 * 3 * P, vjp);
             if (rc) return rc;
         }
     }

@@ -233,13 +233,14 @@ __device__ __forceinline__ void tile_f64(const StagedSource *__restrict__ lds, i
 // a float64 pass; the sums of target i go to out[i - out_shift] and the rows
 // out_stride and 2 out_stride after it.  lds holds TILE records.
 //
-// Each source tile's factored sum S_t = sum_j (w_j / d_ij) x_j is crossed
-// with x_i before it joins the target's running sum, which is therefore
-// sum_t x_i x S_t (the rows hold cross-product sums, plan.crossed).  S_t
-// has a large component along x_i for close pairs (w / d with d ~ r^2 / 2),
-// which x_i x S_t removes; carried across tiles it would round every later
-// addition at eps |S| (2e-12 of max |u| at level 8), crossed per tile it
-// rounds only the additions of its own tile.
+// With plan.crossed, each source tile's factored sum S_t = sum_j (w_j /
+// d_ij) x_j is crossed with x_i before it joins the target's running sum,
+// which is therefore sum_t x_i x S_t (the rows hold cross-product sums).
+// S_t has a large component along x_i for close pairs (w / d with d ~ r^2 /
+// 2), which x_i x S_t removes; carried across tiles it would round every
+// later addition at eps |S| (2e-12 of max |u| at level 8), crossed per tile
+// it rounds only the additions of its own tile.  Without it the rows hold S
+// itself (the adjoint's forward sweep keeps S for its reverse sweep).
 template <int T, int TILE, int R>
 __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int j_begin,
                                                int j_end, double *out, int out_stride,
@@ -277,12 +278,21 @@ __device__ __forceinline__ void pair_f64_range(const PairArgs &a, int ttile, int
             tile_f64<T, TILE, R, true>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
         else
             tile_f64<T, TILE, R, false>(lds, jt, nx, ny, nz, c0, self, sx, sy, sz);
-        // u += x_i x S_t with x_i = -n
+        if (a.plan.crossed) {
+            // u += x_i x S_t with x_i = -n
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            ux[t] = __builtin_fma(nz[t], sy[t], __builtin_fma(-ny[t], sz[t], ux[t]));
-            uy[t] = __builtin_fma(nx[t], sz[t], __builtin_fma(-nz[t], sx[t], uy[t]));
-            uz[t] = __builtin_fma(ny[t], sx[t], __builtin_fma(-nx[t], sy[t], uz[t]));
+            for (int t = 0; t < T; ++t) {
+                ux[t] = __builtin_fma(nz[t], sy[t], __builtin_fma(-ny[t], sz[t], ux[t]));
+                uy[t] = __builtin_fma(nx[t], sz[t], __builtin_fma(-nz[t], sx[t], uy[t]));
+                uz[t] = __builtin_fma(ny[t], sx[t], __builtin_fma(-nx[t], sy[t], uz[t]));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                ux[t] += sx[t];
+                uy[t] += sy[t];
+                uz[t] += sz[t];
+            }
         }
     }
 
@@ -1224,6 +1234,11 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
     if (i < st.n_targets) {
         double Sx, Sy, Sz;
         reduce_partials(st.partial, sp.plan, stride, i, Sx, Sy, Sz);
+        if (sp.sums_out && i < st.n_active) {   // kept for the adjoint's reverse sweep
+            sp.sums_out[i] = Sx;
+            sp.sums_out[sp.sums_stride + i] = Sy;
+            sp.sums_out[2 * (size_t)sp.sums_stride + i] = Sz;
+        }
         const double x = st.xs[i], y = st.xs[stride + i], z = st.xs[2 * stride + i];
         // u = sum_j w_j (x_i x x_j)/d_ij / (-4 PI)    (PanelVelocityParallel.f90:1332)
         double ux, uy, uz;

@@ -57,8 +57,8 @@ struct __align__(16) AdjRec {
 };
 
 // Partial rows per (segment, target tile) of the adjoint's pair pass:
-// S (3), V (3), N (3), g (1).
-constexpr int kAdjRows = 10;
+// V (3), N (3).
+constexpr int kAdjRows = 6;
 
 // One reverse RK4 stage of the adjoint on n active panels.  Row arrays have
 // `stride` entries per row (rows x, y, z, zeta); source and record arrays
@@ -74,6 +74,7 @@ struct AdjStage {
     double *m;             // 3 rows
     AdjRec *rec;
     const Source *ys;      // this stage's input as source records {x, zeta A}
+    const double *sums;    // forward sums S of this stage's input, 3 rows
     const double *area;
     PairPlan plan;         // segmented plan of the pair pass over n_pad records
     double *partial;       // pair pass: kAdjRows rows per (segment + tile)
@@ -123,6 +124,10 @@ struct StageParams {
     double divisor;   // 6, 3, 3, 6
     int renormalize;
     double t_next;    // forced step: time of the next stage input (of the result at stage 3)
+    // if set, the factored sums S of active targets go here (3 rows of
+    // sums_stride; the plan's rows must hold S, not crossed sums)
+    double *sums_out;
+    int sums_stride;
 };
 
 bool plan_supported(int targets_per_lane, int tile);

@@ -78,10 +78,16 @@ def parse():
                    help="targets_per_lane,tile,workgroups,recip for the pair kernel")
     p.add_argument("--adjoint-level", type=int, default=6,
                    help="cubed-sphere level of the adjoint workload (0 = skip)")
+    p.add_argument("--adjoint-fd-panels", type=int, default=32,
+                   help="panels of the adjoint's central-difference check (0 = skip it)")
     p.add_argument("--ensemble-members", type=int, default=64,
                    help="members of the ensemble workload (0 = skip)")
     p.add_argument("--large-level", type=int, default=8,
                    help="cubed-sphere level of the two-vortex large solve (0 = skip)")
+    p.add_argument("--large-modes", type=str, default="fp64,fp64_fast,mixed,mixed_kahan,mixed_bounded",
+                   help="arithmetic modes of the large solve")
+    p.add_argument("--ensemble-modes", type=str, default="mixed,fp64_fast",
+                   help="arithmetic modes of the ensemble workload")
     return p.parse_args()
 
 
@@ -221,6 +227,21 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
     t_adj_stored = time.perf_counter() - t0
     stored_diff = float(np.abs(gz1 - gz).max() / np.abs(gz).max())
 
+    out = dict(workload=f"rh4_cubed_sphere_L{level}_{steps}steps_dt{dt:g}_checkpoint{every}",
+               mode=mode, n_active=g.n_active, J=J, gradient_ms=t_adj * 1e3,
+               forward_ms_active_only=t_fwd_active * 1e3,
+               forward_ms_active_and_vertices=t_fwd_vertices * 1e3,
+               forward_ms_all_classes=t_fwd * 1e3,
+               adjoint_fwd_ratio_same_map=t_adj / t_fwd_active,
+               gradient_ms_all_stages_stored=t_adj_stored * 1e3,
+               adjoint_fwd_ratio_same_map_all_stages_stored=t_adj_stored / t_fwd_active,
+               all_stages_stored_vs_checkpointed_max_rel_diff=stored_diff,
+               adjoint_fwd_ratio_active_and_vertices=t_adj / t_fwd_vertices,
+               adjoint_fwd_ratio_all_classes=t_adj / t_fwd,
+               fd_panels=int(n_fd))
+    if n_fd <= 0:
+        ctx.close()
+        return out
     idx = np.random.default_rng(0).choice(g.n_active, n_fd, replace=False)
 
     def Jof(zz):
@@ -249,18 +270,8 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
                         Jof(z - eps * (np.arange(g.n_active) == j))) / (2 * eps) for j in idx])
         fd_report[f"eps_{eps:g}_plain"] = report(fd)
     ctx.close()
-    return dict(workload=f"rh4_cubed_sphere_L{level}_{steps}steps_dt{dt:g}_checkpoint{every}",
-                mode=mode, n_active=g.n_active, J=J, gradient_ms=t_adj * 1e3,
-                forward_ms_active_only=t_fwd_active * 1e3,
-                forward_ms_active_and_vertices=t_fwd_vertices * 1e3,
-                forward_ms_all_classes=t_fwd * 1e3,
-                adjoint_fwd_ratio_same_map=t_adj / t_fwd_active,
-                gradient_ms_all_stages_stored=t_adj_stored * 1e3,
-                adjoint_fwd_ratio_same_map_all_stages_stored=t_adj_stored / t_fwd_active,
-                all_stages_stored_vs_checkpointed_max_rel_diff=stored_diff,
-                adjoint_fwd_ratio_active_and_vertices=t_adj / t_fwd_vertices,
-                adjoint_fwd_ratio_all_classes=t_adj / t_fwd,
-                fd_panels=int(n_fd), fd=fd_report)
+    out["fd"] = fd_report
+    return out
 
 
 def ensemble_measurement(n_members=64, levels=(4, 5, 6), steps=10, dt=0.01,
@@ -268,21 +279,26 @@ def ensemble_measurement(n_members=64, levels=(4, 5, 6), steps=10, dt=0.01,
     """Ensemble workload: 64 members cycling through cubed-sphere levels 4, 5
     and 6 (RH4 plus a seeded smooth perturbation of RMS 0.05 max|zeta|,
     correlation 0.3 rad), 10 RK4 steps at dt = 0.01 in one HIP graph, launch
-    plan per level from the autotuner.  Aggregate pair rate over all
-    members (all three target classes counted)."""
+    targets per lane tuned on the ensemble itself, segment lengths derived
+    from its whole work.  Aggregate pair rate over all members (all three
+    target classes counted)."""
     from thesiscode_amd import ensemble as ensmod
     members = ensmod.make_members(n_members, levels)
     out = {}
     for mode in modes:
         prec = {"mixed": 1, "fp64": 0, "fp64_fast": 2}[mode]
-        counts = {L: sum(1 for m in members if m["level"] == L) for L in set(levels)}
-        table = ensmod.autotune(sorted(set(levels)), precision=prec, weights=counts)
+        table = ensmod.autotune_members(members, precision=prec)
         el, pairs, ens = ensmod.run(members, dt, steps, prec, table=table, use_graph=True)
+        plans = {}
+        for m, d in enumerate(members):
+            plans.setdefault(str(d["level"]), list(ens.member_plan(m)))
         ens.close()
         out[mode] = dict(pair_rate_all_classes=pairs * steps / el, seconds=el,
                          ms_per_step=el * 1e3 / steps,
-                         plans={str(L): [t["targets_per_lane"], t["chunk"]]
-                                for L, t in table.items()})
+                         plans_targets_per_lane_chunk_work_items=plans,
+                         items_per_workgroup=next(iter(table.values()))["items_per_workgroup"],
+                         tuning_ms_per_stage={f"T{T}_items{k}": ms for T, k, ms in
+                                              next(iter(table.values()))["tried"]})
     return dict(workload=f"{n_members} members, levels {list(levels)}, {steps} RK4 steps dt={dt}, "
                          f"one HIP graph", modes=out)
 
@@ -594,18 +610,20 @@ def main():
         },
     }
     if args.adjoint_level > 0:
-        adj = adjoint_measurement(args.adjoint_level)
+        adj = adjoint_measurement(args.adjoint_level, n_fd=args.adjoint_fd_panels)
         print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
         # the gradient against a forward of the map it differentiates (the
         # active panels alone, the same 20 steps and arithmetic)
         out["adjoint_fwd_ratio"] = adj["adjoint_fwd_ratio_same_map"]
         out["config"]["adjoint"] = adj
     if args.ensemble_members > 0:
-        ens = ensemble_measurement(args.ensemble_members)
+        ens = ensemble_measurement(args.ensemble_members,
+                                   modes=[m for m in args.ensemble_modes.split(",") if m])
         print(f"# ensemble: {ens}", file=sys.stderr, flush=True)
         out["config"]["ensemble"] = ens
     if args.large_level > 0:
-        big = large_solve_measurement(args.large_level)
+        big = large_solve_measurement(args.large_level,
+                                      modes=[m for m in args.large_modes.split(",") if m])
         out["config"]["large_solve"] = big
     if args.cpu_seconds > 0:
         port = cpu_baseline(g, za, args.cpu_seconds)

@@ -296,11 +296,22 @@ int bve_ensemble_load_member(bve_ensemble *ens, int member, int n_active, const 
                              const double *relvort_active, const double *area_active,
                              int n_passive, const double *x_passive, int n_particles,
                              const double *x_particles, const double *absvort);
-/* Launch geometry of one member: targets per lane (1, 2, 4) and sources per
- * work item (a multiple of 256); 0 keeps the built-in choice.  Set by the
- * autotuner per problem size.  Members with the same targets per lane share
- * one persistent pair launch (the autotuner gives all members one value). */
+/* Launch geometry of one member: targets per lane (1, 2, 4; 0 = built-in
+ * choice) and sources per work item (a multiple of 256).  chunk = 0 (the
+ * default) gives every member of a targets-per-lane value one segment
+ * length from the whole ensemble's work (see
+ * bve_ensemble_set_items_per_workgroup).  Members with the same targets per
+ * lane share one persistent pair launch (the autotuner gives all members
+ * one value). */
 int bve_ensemble_set_member_plan(bve_ensemble *ens, int member, int targets_per_lane, int chunk);
+/* The member's launch geometry as the next advance runs it: targets per
+ * lane, sources per work item and work items (any pointer may be NULL). */
+int bve_ensemble_get_member_plan(bve_ensemble *ens, int member, int *targets_per_lane, int *chunk,
+                                 int *work_items);
+/* Members with chunk 0: work items per resident workgroup of their shared
+ * pair launch (0 = 16); more items balance the launch's end better, fewer
+ * write fewer partial rows. */
+int bve_ensemble_set_items_per_workgroup(bve_ensemble *ens, int items);
 /* Workgroups of each persistent pair launch (0 = as many as the kernel's
  * occupancy keeps resident). */
 int bve_ensemble_set_grid(bve_ensemble *ens, int workgroups);

@@ -287,3 +287,32 @@ def test_ensemble_workload_64_members():
             err = np.abs(first[m][k] - r[k]).max() / disp
             print(f"level {d['level']} {k}: {err:.2e}")
             assert err < 10 * MIXED_BAR
+
+
+def test_ensemble_derived_segments_64_members():
+    """64 members with chunk 0: every member of the shared launch gets the
+    segment length derived from the whole ensemble (level 6: far longer
+    than one source tile, so far fewer partial rows), and each member still
+    equals its unbatched run with that geometry bit for bit (fp64_fast,
+    2 RK4 steps)."""
+    from thesiscode_amd.ensemble import load_members, make_members
+    members = make_members(64, levels=(4, 5, 6))
+    dt, n = 0.01, 2
+    prec = _lib.BVE_PRECISION_FP64_FAST
+    with _lib.Ensemble(len(members)) as ens:
+        ens.set_params(OMEGA, 0.05, prec, 0.02)
+        load_members(ens, members)
+        for m in range(len(members)):
+            ens.set_member_plan(m, 2, 0)
+        plans = [ens.member_plan(m) for m in range(len(members))]
+        ens.advance(dt, n, use_graph=True)
+        got = [ens.store_member(m) for m in range(len(members))]
+    chunks = {plans[m][1] for m, d in enumerate(members) if d["level"] == 6}
+    print("level-6 members (T, chunk, work items):", {plans[m] for m, d in enumerate(members)
+                                                      if d["level"] == 6})
+    assert len(chunks) == 1 and chunks.pop() >= 8 * 256
+    for m, d in enumerate(members):
+        T, chunk, _ = plans[m]
+        ref = alone_run(d, prec, dt, n, (T, chunk))
+        for k in ("active_x", "active_relvort", "passive_x", "particles_x"):
+            assert np.array_equal(got[m][k], ref[k]), (m, k)

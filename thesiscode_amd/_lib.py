@@ -60,6 +60,8 @@ SIGNATURES = {
     "bve_ensemble_set_params": (_i, [_p, _d, _d, _i, _d]),
     "bve_ensemble_load_member": (_i, [_p, _i, _i, _p, _p, _p, _i, _p, _i, _p, _p]),
     "bve_ensemble_set_member_plan": (_i, [_p, _i, _i, _i]),
+    "bve_ensemble_get_member_plan": (_i, [_p, _i, _p, _p, _p]),
+    "bve_ensemble_set_items_per_workgroup": (_i, [_p, _i]),
     "bve_ensemble_set_grid": (_i, [_p, _i]),
     "bve_ensemble_advance": (_i, [_p, _d, _i, _i]),
     "bve_ensemble_synchronize": (_i, [_p]),
@@ -456,6 +458,19 @@ class Ensemble:
     def set_member_plan(self, m, targets_per_lane=0, chunk=0):
         self._check(self._L.bve_ensemble_set_member_plan(self._h, m, targets_per_lane, chunk),
                     "bve_ensemble_set_member_plan")
+
+    def member_plan(self, m):
+        """(targets per lane, sources per work item, work items) of member m
+        as the next advance runs it."""
+        t, c, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.bve_ensemble_get_member_plan(self._h, m, ctypes.byref(t),
+                                                         ctypes.byref(c), ctypes.byref(w)),
+                    "bve_ensemble_get_member_plan")
+        return t.value, c.value, w.value
+
+    def set_items_per_workgroup(self, items):
+        self._check(self._L.bve_ensemble_set_items_per_workgroup(self._h, items),
+                    "bve_ensemble_set_items_per_workgroup")
 
     def set_grid(self, workgroups):
         self._check(self._L.bve_ensemble_set_grid(self._h, workgroups), "bve_ensemble_set_grid")

@@ -1973,6 +1973,7 @@ struct bve_ensemble {
     std::string error;
     std::vector<bve_ctx *> members;
     int grid_wg = 0;   // workgroups per persistent pair launch, 0 = occupancy
+    int items_per_wg = 16;   // derived segment lengths: work items per resident workgroup
     int n_cu = 256;
     int precision = BVE_PRECISION_FP64;
 
@@ -2064,6 +2065,14 @@ int ens_build(bve_ensemble *e)
     // (targets-per-lane group, -cost): groups in order, largest items first
     std::vector<std::pair<std::pair<int, long long>, EnsItem>> costed;
     std::vector<int2> stage_map, src_map, tgt_map;
+    static const int kT[kEnsGroups] = {1, 2, 4};
+    const bool kahan = e->precision == BVE_PRECISION_MIXED_KAHAN;
+    const int recip = e->precision == BVE_PRECISION_FP64_FAST ? 4 : 1;
+    auto group_of = [](int T) { return T == 1 ? 0 : (T == 2 ? 1 : 2); };
+    auto group_slots = [&](int g) {
+        const int per_cu = ensemble_pair_blocks_per_cu(kT[g], mixed, kahan, recip);
+        return e->grid_wg ? e->grid_wg : (per_cu > 0 ? per_cu : 2) * e->n_cu;
+    };
     for (int m = 0; m < (int)e->members.size(); ++m) {
         bve_ctx *c = e->members[m];
         if (!c->panels_loaded || !c->x0)
@@ -2074,6 +2083,35 @@ int ens_build(bve_ensemble *e)
             int rc = update_plan(c);
             if (rc) return e->fail(rc, "member %d: %s", m, c->error.c_str());
         }
+    }
+    // Members without an explicit chunk share one segment length per
+    // targets-per-lane group: items_per_wg segments per resident workgroup
+    // of the group's batched launch over all its members' (target tile,
+    // source tile) items, which cost the same at every level.  The queue
+    // then balances the launch as the single-context plan does, and a
+    // member's partial rows number its segments + tiles (level 6 in a
+    // 64-member ensemble: a few hundred rows instead of ~2,800 with
+    // 1,024-source chunks).
+    long long group_items[kEnsGroups] = {0, 0, 0};
+    for (bve_ctx *c : e->members)
+        if (c->plan_chunk == 0)
+            group_items[group_of(c->plan.targets_per_lane)] +=
+                (long long)c->plan.n_tiles * c->plan.n_src_tiles;
+    for (int m = 0; m < (int)e->members.size(); ++m) {
+        bve_ctx *c = e->members[m];
+        if (c->plan_chunk != 0) continue;
+        const int g = group_of(c->plan.targets_per_lane);
+        const long long len =
+            std::max(1LL, group_items[g] / ((long long)e->items_per_wg * std::max(1, group_slots(g))));
+        c->plan_chunk = (int)std::min<long long>(len, c->plan.n_src_tiles * (long long)c->plan.n_tiles) * 256;
+        const PairPlan p = plan_for(c, c->n_targets, c->n_src_pad);
+        c->plan_chunk = 0;
+        c->plan = p;
+        const int rc = ensure_partial(c, partial_doubles(p, c->t_stride));
+        if (rc) return e->fail(rc, "member %d: %s", m, c->error.c_str());
+    }
+    for (int m = 0; m < (int)e->members.size(); ++m) {
+        bve_ctx *c = e->members[m];
         if (mixed) {
             int rc = ensure_mixed(c, c->n_src_pad, c->t_stride);
             if (rc) return e->fail(rc, "member %d: %s", m, c->error.c_str());
@@ -2086,7 +2124,7 @@ int ens_build(bve_ensemble *e)
         states.back().member = (unsigned)m;
         plans.push_back(c->plan);
         const int T = c->plan.targets_per_lane;
-        const int group = T == 1 ? 0 : (T == 2 ? 1 : 2);
+        const int group = group_of(T);
         const long long cost = (long long)kBlock * T * c->plan.chunk;
         for (int sg = 0; sg < c->plan.segments; ++sg)
             costed.push_back({{group, -cost}, EnsItem{m, sg}});
@@ -2128,16 +2166,13 @@ int ens_build(bve_ensemble *e)
     L.pair_members = e->d_pair;
     L.mixed_members = e->d_mixed;
     L.mixed = mixed;
-    L.kahan = e->precision == BVE_PRECISION_MIXED_KAHAN;
-    L.recip = e->precision == BVE_PRECISION_FP64_FAST ? 4 : 1;
-    static const int kT[kEnsGroups] = {1, 2, 4};
+    L.kahan = kahan;
+    L.recip = recip;
     for (int g = 0; g < kEnsGroups; ++g) {
         L.group_begin[g] = group_begin[g];
         L.group_count[g] = group_count[g];
         // default grid: every workgroup slot the kernel's occupancy allows
-        const int per_cu = ensemble_pair_blocks_per_cu(kT[g], L.mixed, L.kahan, L.recip);
-        const int slots = e->grid_wg ? e->grid_wg : (per_cu > 0 ? per_cu : 2) * e->n_cu;
-        L.group_grid[g] = std::max(1, std::min(group_count[g], slots));
+        L.group_grid[g] = std::max(1, std::min(group_count[g], group_slots(g)));
     }
     L.states = e->d_states;
     L.plans = e->d_plans;
@@ -2330,6 +2365,31 @@ int bve_ensemble_set_member_plan(bve_ensemble *e, int member, int targets_per_la
         const int rc = update_plan(c);
         if (rc) return e->fail(rc, "member %d: %s", member, c->error.c_str());
     }
+    return BVE_OK;
+}
+
+int bve_ensemble_get_member_plan(bve_ensemble *e, int member, int *targets_per_lane, int *chunk,
+                                 int *work_items)
+{
+    if (!e) return BVE_ERR_ARGUMENT;
+    if (member < 0 || member >= (int)e->members.size())
+        return e->fail(BVE_ERR_ARGUMENT, "bve_ensemble_get_member_plan: member %d out of range", member);
+    HIP_TRY(e, hipSetDevice(e->device));
+    const int rc = ens_build(e);
+    if (rc) return rc;
+    const PairPlan &p = e->members[member]->plan;
+    if (targets_per_lane) *targets_per_lane = p.targets_per_lane;
+    if (chunk) *chunk = p.chunk;
+    if (work_items) *work_items = p.segments;
+    return BVE_OK;
+}
+
+int bve_ensemble_set_items_per_workgroup(bve_ensemble *e, int items)
+{
+    if (!e || items < 0) return BVE_ERR_ARGUMENT;
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    ens_free_tables(e);
+    e->items_per_wg = items ? items : 16;
     return BVE_OK;
 }
 

@@ -7,8 +7,10 @@
 // host lists the work items in decreasing cost, and the persistent grid
 // takes them from a queue, so each workgroup gets a similar share of the
 // work whatever the items cost (the mixed kernel's careful tiles cost more
-// than its far-field ones); every member keeps its own chunk length, chosen
-// per size by the autotuner.  The kernels are compiled per targets-per-lane
+// than its far-field ones).  By default the members of one launch share a
+// segment length derived from the launch's whole work (a tuned number of
+// segments per resident workgroup); an explicit chunk keeps a member's
+// own.  The kernels are compiled per targets-per-lane
 // count T, so each holds only its own registers (the mixed loop needs four
 // waves per SIMD): one launch per T present among the members, one in all
 // when the autotuner's common T is used.

@@ -61,10 +61,47 @@ def load_members(ens, members):
 
 
 def candidate_plans(level):
-    """(targets per lane, chunk) pairs the autotuner tries for a level."""
+    """(targets per lane, chunk) pairs the autotuner tries for a level; chunk
+    0 lets the ensemble derive the segment length from its whole work."""
     n_src = gridmod.counts(level)[0]
     chunks = [c for c in (256, 512, 1024, 2048, 4096, 8192, 16384) if c <= max(512, n_src)]
-    return [(T, c) for T in (1, 2, 4) for c in chunks]
+    return [(T, c) for T in (1, 2, 4) for c in [0] + chunks]
+
+
+ITEMS_PER_WORKGROUP = (16, 32, 64, 128)
+
+
+def autotune_members(members, precision=_lib.BVE_PRECISION_FP64, omega=ic.OMEGA_EARTH, dt=0.01,
+                     device=0, log=None, steps=2):
+    """Tune the ensemble itself: every member gets one targets-per-lane value
+    (one shared pair launch) and chunk 0, the segment length the ensemble
+    derives from its whole work and the work items per resident workgroup;
+    each (T, items) in (1, 2, 4) x ITEMS_PER_WORKGROUP is timed over `steps`
+    RK4 steps of all members from the initial state.  Returns {level:
+    {"targets_per_lane", "chunk", "items_per_workgroup", "ms_per_stage",
+    "tried"}} for the levels present (tried: (T, items, ms per stage))."""
+    tried = []
+    with _lib.Ensemble(len(members), device) as ens:
+        ens.set_params(omega, 0.05, precision, 0.02)
+        for T in (1, 2, 4):
+            for items in ITEMS_PER_WORKGROUP:
+                load_members(ens, members)
+                ens.set_items_per_workgroup(items)
+                for m in range(len(members)):
+                    ens.set_member_plan(m, T, 0)
+                ens.advance(dt, 1, use_graph=False)
+                ens.synchronize()
+                t0 = time.perf_counter()
+                ens.advance(dt, steps, use_graph=False)
+                ens.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / (4 * steps)
+                tried.append((T, items, ms))
+                if log:
+                    log(f"autotune ensemble: T={T} items/workgroup={items} {ms:.3f} ms/stage")
+    T, items, ms = min(tried, key=lambda r: r[2])
+    return {L: dict(targets_per_lane=T, chunk=0, items_per_workgroup=items, ms_per_stage=ms,
+                    tried=tried)
+            for L in sorted({d["level"] for d in members})}
 
 
 def autotune(levels, precision=_lib.BVE_PRECISION_FP64, members_per_level=2, omega=ic.OMEGA_EARTH,
@@ -151,6 +188,8 @@ def run(members, dt, n_steps, precision, table=None, use_graph=True, omega=ic.OM
     for m, d in enumerate(members):
         if table and d["level"] in table:
             t = table[d["level"]]
+            if "items_per_workgroup" in t:
+                ens.set_items_per_workgroup(t["items_per_workgroup"])
             ens.set_member_plan(m, t["targets_per_lane"], t["chunk"])
     if warmup:
         ens.advance(dt, n_steps, use_graph)

@@ -351,6 +351,30 @@ class ErrorSample:
                                ("particles", slice(k2, None)))}
         return float(du.max() / un.max()), float((du[keep] / un[keep]).max()), per
 
+    @staticmethod
+    def positions(ctx):
+        ax, _, px = ctx.store_panels()
+        qx, _ = ctx.store_particles()
+        return np.concatenate([ax, px, qx])
+
+    def set_step_reference(self, ctx, reload, dt):
+        """One float64 RK4 step of every target from the initial state (the
+        context must be in the float64 mode): the reference of step_error,
+        itself within ~1e-12 of the oracle's BVERK4."""
+        reload()
+        self.x0 = self.positions(ctx)
+        ctx.advance(dt, 1)
+        self.x1 = self.positions(ctx)
+        self.disp = float(np.abs(self.x1 - self.x0).max())
+
+    def step_error(self, ctx, reload, dt):
+        """One RK4 step of the context's mode from the initial state (the
+        advance path a timed run takes, all four stages) against the float64
+        step: max position difference over every target / max displacement."""
+        reload()
+        ctx.advance(dt, 1)
+        return float(np.abs(self.positions(ctx) - self.x1).max() / self.disp)
+
 
 def peak_of(mode):
     """Vector peak of the pipe a mode's pair loop runs on (TFLOP/s)."""
@@ -385,6 +409,7 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
     # from the same plan and kernel as the timed launches
     reload()
     err, err_point, err_class = sample.errors(ctx)
+    step_err = sample.step_error(ctx, reload, args.dt)
     ctx.set_precision(_lib.BVE_PRECISION_FP64)
     avg_ms = kernel_ms / max(launches, 1)
     achieved = pairs_all / 4.0 * FLOPS_PER_PAIR / (avg_ms * 1e-3) / 1e12
@@ -392,6 +417,7 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
                 pair_rate_all_classes=pairs_all * args.steps / el,
                 ms_per_step=el * 1e3 / max(args.steps, 1), rel_err_vs_fp64_oracle=err,
                 max_pointwise_rel_err=err_point, rel_err_per_class=err_class,
+                step_rel_err_vs_fp64_step=step_err,
                 error_sample=f"{len(sample)} targets (active, passive and particle classes), "
                              f"velocities of the resident state from the timed plan",
                 launches=launches, avg_launch_ms=avg_ms, achieved_tflops=achieved,
@@ -432,6 +458,7 @@ def pareto_curve(ctx, reload, sample, dt, steps, base_launch_ms, base_step_ms):
         ctx.set_profiling(False)
         reload()
         err, err_point, _ = sample.errors(ctx)
+        step_err = sample.step_error(ctx, reload, dt)
         launch_ms = kernel_ms / max(launches, 1)
         step_ms = el * 1e3 / steps
         out.append(dict(mode=mode, **({"error_target": param} if mode == "mixed_bounded"
@@ -439,11 +466,12 @@ def pareto_curve(ctx, reload, sample, dt, steps, base_launch_ms, base_step_ms):
                         ms_per_launch=launch_ms, ms_per_step=step_ms,
                         speedup_vs_fp64_launch=base_launch_ms / launch_ms,
                         speedup_vs_fp64_step=base_step_ms / step_ms,
-                        rel_err=err, max_pointwise_rel_err=err_point))
+                        rel_err=err, max_pointwise_rel_err=err_point,
+                        step_rel_err=step_err))
         print(f"# pareto {out[-1]}", file=sys.stderr, flush=True)
     ctx.set_error_target(1e-10)
     ctx.set_precision(_lib.BVE_PRECISION_FP64)
-    ok = [p for p in out if p["rel_err"] <= ERR_BAR]
+    ok = [p for p in out if max(p["rel_err"], p["step_rel_err"]) <= ERR_BAR]
     best = max(ok, key=lambda p: p["speedup_vs_fp64_step"]) if ok else None
     return {"points": out, "fastest_within_1e-10": best}
 
@@ -502,6 +530,8 @@ def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed
         ctx.load_particles(g.particles_x, absv)
 
     args = types.SimpleNamespace(dt=dt, warmup=1, steps=1)
+    ctx.set_precision(MODES["fp64"])
+    sample.set_step_reference(ctx, reload, dt)
     out = {}
     for mode in modes:
         out[mode] = measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample)
@@ -549,6 +579,8 @@ def main():
           flush=True)
 
     sample = ErrorSample(g, za, [ic.latlon_to_xyz(np.pi / 4, 0.0)])
+    ctx.set_precision(MODES["fp64"])
+    sample.set_step_reference(ctx, reload, args.dt)
     results = {}
     for mode in [m for m in args.modes.split(",") if m]:
         results[mode] = measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample)
@@ -563,7 +595,8 @@ def main():
     print(f"# host round trip: {roundtrip}", file=sys.stderr, flush=True)
     ctx.close()
 
-    ok = [m for m in results if results[m]["rel_err_vs_fp64_oracle"] <= ERR_BAR]
+    ok = [m for m in results if max(results[m]["rel_err_vs_fp64_oracle"],
+                                    results[m]["step_rel_err_vs_fp64_step"]) <= ERR_BAR]
     best = max(ok, key=lambda m: results[m]["pair_rate"]) if ok else None
     r = results[best] if best else {}
     out = {

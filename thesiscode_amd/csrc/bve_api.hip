@@ -162,6 +162,9 @@ struct bve_ctx {
     double4 *tile_box = nullptr, *group_box = nullptr;
     double *tile_mom = nullptr;   // bounded mode: kTileMoments per 256-source tile
     double *scale = nullptr;      // bounded mode: velocity scale of the current pass
+    // set between a bounded stage update, which folded max |u| into scale,
+    // and the next pair pass of the same run, which then needs no probe
+    bool scale_from_stage = false;
     int mixed_src_cap = 0, mixed_tgt_cap = 0, mixed_src_cap_mom = 0;
     size_t staging_cap = 0;      // doubles
     bool particles_have_absvort = false;
@@ -643,8 +646,9 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
         int rc = ensure_mixed(c, a.n_src_pad, a.t_stride);
         if (rc) return rc;
         ma = mixed_args_of(c, a, n_targets, n_sources);
-        HIP_TRY(c, is_bounded(c) ? launch_bounded_prep(ma, c->stream)
+        HIP_TRY(c, is_bounded(c) ? launch_bounded_prep(ma, c->stream, !c->scale_from_stage)
                                  : launch_mixed_prep(ma, c->stream));
+        c->scale_from_stage = false;
     }
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (c->profiling) {
@@ -680,6 +684,16 @@ int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPl
     if (sum_store && (plan.crossed || plan.exact)) return c->fail(BVE_ERR_STATE, "run_steps_on: sums of a crossed plan");
     StepState st = step_state(c);
     st.n_targets = n_targets;
+    // bounded mode: the run's first pass probes the velocity scale; each
+    // stage update then folds max |u| of its stage into the scale word for
+    // the next pass (the field moves by O(dt) between stages, well inside
+    // the routing's margin)
+    const bool fold_scale = is_bounded(c) && c->scale;
+    c->scale_from_stage = false;
+    struct ScaleFlagClear {   // no later pass outside this run reuses the scale
+        bve_ctx *c;
+        ~ScaleFlagClear() { c->scale_from_stage = false; }
+    } clear_on_exit{c};
     if (forced) {
         st.panel_absvort = c->panel_absvort;
         st.t_step = t;
@@ -713,7 +727,11 @@ int run_steps_on(bve_ctx *c, double dt, int n_steps, int n_targets, const PairPl
                 sp.sums_out = sum_store + (size_t)(4 * n + s) * 3 * stage_stride;
                 sp.sums_stride = (int)stage_stride;
             }
+            const bool fold = fold_scale && !(n == n_steps - 1 && s == 3);
+            st.vel_max = fold ? c->scale : nullptr;
+            if (fold) HIP_TRY(c, launch_scale_reset(c->scale, c->stream));
             HIP_TRY(c, launch_stage_update(st, sp, c->stream));
+            c->scale_from_stage = fold;
         }
     }
     return BVE_OK;

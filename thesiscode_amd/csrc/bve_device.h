@@ -671,9 +671,11 @@ __device__ __forceinline__ void pair_mixed_segments(const MixedArgs &a, MixedLds
 // holds a target's own index, the group's sphere reaches the tile's, t may
 // exceed kBoundedTmax, or the rounding bound 8 eps32 t^2 sum|w| / rho_min of
 // the remainder exceeds tau = 0.3 err_target S, S being the velocity scale
-// max |x_i x S_i| the probe pass finds (a lower bound of the true maximum,
-// so tau errs on the safe side): vortex-core tiles and near tiles run in
-// float64, the background far field in float32.  Four pairs share one
+// max |x_i x S_i| (4 PI |u|): over the previous RK4 stage's targets inside
+// an advance (the field moves by O(dt) between stages), else over the probe
+// pass's targets (a lower bound of the true maximum, so tau errs on the
+// safe side): vortex-core tiles and near tiles run in float64, the
+// background far field in float32.  Four pairs share one
 // v_rcp_f32 of the product of their denominators (all >= (1 - kBoundedTmax) E,
 // so the product stays far from float32 underflow).
 
@@ -1230,7 +1232,7 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
 #pragma clang fp contract(off)
     const int i = blk * 256 + threadIdx.x;
     const int stride = st.t_stride;
-    double ke_local = 0.0;
+    double ke_local = 0.0, u_abs = 0.0;
     if (i < st.n_targets) {
         double Sx, Sy, Sz;
         reduce_partials(st.partial, sp.plan, stride, i, Sx, Sy, Sz);
@@ -1245,7 +1247,10 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
         velocity_from_sums(sp.plan.exact | sp.plan.crossed, x, y, z, Sx, Sy, Sz, ux, uy, uz);
         // a coincident source and target (1 - x.y = 0) makes the sums NaN or
         // Inf, as the reference's 0/0 does: reported, not silently carried
-        if (!finite3(ux, uy, uz)) flag_nonfinite(st.status, st.member, i);
+        if (!finite3(ux, uy, uz))
+            flag_nonfinite(st.status, st.member, i);
+        else if (st.vel_max)   // in the pair sums' units: |x_i x S_i| = 4 PI |u|
+            u_abs = 4.0 * kPi * sqrt((ux * ux + uy * uy) + uz * uz);
         const double kx = sp.dt * ux, ky = sp.dt * uy, kz = sp.dt * uz;
         const double *base = (sp.stage == 0) ? st.x0 : st.acc;
         // x + k1/6 + k2/3 + k3/3 + k4/6, left to right (:637-644)
@@ -1318,6 +1323,11 @@ __device__ __forceinline__ void stage_update_body(const StepState &st, const Sta
     if (sp.stage == 0) {
         const double b = block_sum(ke_local, sh);
         if (threadIdx.x == 0) st.ke_part[blk] = b;
+    }
+    if (st.vel_max) {
+        const double m = block_max(u_abs, sh);
+        if (threadIdx.x == 0 && m > 0.0 && m <= 1.7976931348623157e308)
+            atomicMax((unsigned long long *)st.vel_max, (unsigned long long)__double_as_longlong(m));
     }
 }
 

@@ -114,6 +114,10 @@ struct StepState {
     // numeric status word (may be null) and the member index its keys carry
     unsigned long long *status;
     unsigned member;
+    // if set (bounded mode), the stage update folds max 4 PI |u| = max
+    // |x_i x S_i| over its targets into this non-negative double (atomicMax
+    // on its bits): the velocity scale of the next pair pass
+    double *vel_max;
 };
 
 struct StageParams {
@@ -142,7 +146,9 @@ hipError_t launch_mixed_prep(const MixedArgs &a, hipStream_t s);
 hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s);
 // Bounded mixed mode: preparation (records, tile moments, group spheres,
 // velocity-scale probe) and the pair pass.
-hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s);
+hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe = true);
+// Clears the velocity-scale word (before a stage update folds max |u| into it).
+hipError_t launch_scale_reset(double *scale, hipStream_t s);
 hipError_t launch_pair_sum_bounded(const MixedArgs &a, hipStream_t s);
 int pair_bounded_blocks_per_cu(int targets_per_lane);
 // Reference-order pass (a.plan.exact): bitwise the reference's sums.

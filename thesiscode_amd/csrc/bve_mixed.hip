@@ -160,7 +160,16 @@ int pair_bounded_blocks_per_cu(int T)
     }
 }
 
-hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s)
+hipError_t launch_scale_reset(double *scale, hipStream_t s)
+{
+    if (!scale) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bounded_scale_reset_kernel, dim3(1), dim3(64), 0, s, scale);
+    return hipGetLastError();
+}
+
+// probe = false: the scale word already holds the velocity scale (max |u|
+// of the previous RK4 stage, folded in by its stage update).
+hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
 {
     if (!a.tile_mom || !a.scale) return hipErrorInvalidValue;
     hipError_t e = launch_mixed_prep(a, s);
@@ -168,9 +177,8 @@ hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s)
     hipLaunchKernelGGL(bounded_moment_prep_kernel, dim3(a.base.n_src_pad / kMixedTile),
                        dim3(kMixedTile), 0, s, a.base.src, a.n_sources, a.tile_box, a.tile_mom);
     e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bounded_scale_reset_kernel, dim3(1), dim3(64), 0, s, a.scale);
-    e = hipGetLastError();
+    if (e != hipSuccess || !probe) return e;
+    e = launch_scale_reset(a.scale, s);
     if (e != hipSuccess || a.n_targets <= 0) return e;
     // the first target of every group_stride-th group of 64: at most ~1024
     // probes, a target every ~220 at level 7 (the groups are spatially

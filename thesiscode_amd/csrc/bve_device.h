@@ -657,20 +657,26 @@ __device__ __forceinline__ void pair_mixed_segments(const MixedArgs &a, MixedLds
 //
 //     1/D = 1/E + delta/E^2 + delta^2 / (E^2 D),
 //
-// so the tile's sum in difference form (x_i x x_i = 0) is
+// so the tile's factored sum (x_j = c + o_j; only x_i x S is used) is
 //
-//     sum_j w_j (o_j - a) / D_ij          (a = x_i - c)
-//       = (m - a W0) / E + [(P x_i + M) - a (m.x_i + Q)] / E^2
+//     S = sum_j w_j x_j / D_ij
+//       = (W0 c + m) / E + (c (m.x_i) + P x_i + R) / E^2
 //
 // with the float64 tile moments W0 = sum w, m = sum w o, P = sum w o o^T
-// and the float32 remainder sums M = sum (delta^2/D) w o, Q = sum (delta^2/D) w.
+// and the float32 remainder sum R = sum_j (delta_j^2/D_ij) w_j x_j: per pair
+// three products for delta, the difference D, a quarter of a shared
+// reciprocal, delta^2 / D and three FMAs into R.
 // The remainder is t^2 of the plain term (t = |delta| / E, at most
 // 2R/rho + 2R^2/rho^2 for a tile of radius R whose centre lies rho from
-// the target), so its float32 rounding is t^2 of a float32 pair's.  A
+// the target), so its float32 rounding is t^2 of a float32 pair's: about
+// 8 eps32 t^2 |w| / d for the rounding of delta^2 / D (x_i x x_j has length
+// about d = |x_i - x_j|) and 8 eps32 t^2 |w| / d^2 for that of the records
+// w x_j and of the sums (the whole vector, radial part included).  A
 // (target group, tile) pair runs the float64 pass instead when the tile
 // holds a target's own index, the group's sphere reaches the tile's, t may
-// exceed kBoundedTmax, or the rounding bound 8 eps32 t^2 sum|w| / rho_min of
-// the remainder exceeds tau = 0.3 err_target S, S being the velocity scale
+// exceed kBoundedTmax, or the rounding bound 8 eps32 t^2 sum|w| /
+// (d_min min(d_min, 1)) of the remainder exceeds tau = 0.3 err_target S,
+// d_min the least distance between the two spheres and S the velocity scale
 // max |x_i x S_i| (4 PI |u|): over the previous RK4 stage's targets inside
 // an advance (the field moves by O(dt) between stages), else over the probe
 // pass's targets (a lower bound of the true maximum, so tau errs on the
@@ -684,15 +690,15 @@ constexpr double kBoundedEps32 = 5.9604644775390625e-08;   // 2^-24
 
 struct BoundedLds {
     float4 pos[kMixedTile];         // o_j (.w unused)
-    float4 wgt[kMixedTile];         // w_j o_j, w_j
+    float4 wgt[kMixedTile];         // w_j x_j (.w unused)
     StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
 };
 
-// Remainder sums of the source tile in LDS for N targets.
+// Remainder sums R of the source tile in LDS for N targets.
 template <int N>
 __device__ __forceinline__ void remainder_tile(const BoundedLds &lds, const float *xf,
                                                const float *yf, const float *zf, const float *Ef,
-                                               float *Mx, float *My, float *Mz, float *Q)
+                                               float *Rx, float *Ry, float *Rz)
 {
 #pragma unroll 2
     for (int k = 0; k < kMixedTile; k += 4) {
@@ -720,34 +726,34 @@ __device__ __forceinline__ void remainder_tile(const BoundedLds &lds, const floa
             f[3] = (dl[3] * dl[3]) * (D[2] * r23);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                Mx[t] = __builtin_fmaf(f[r], v[r].x, Mx[t]);
-                My[t] = __builtin_fmaf(f[r], v[r].y, My[t]);
-                Mz[t] = __builtin_fmaf(f[r], v[r].z, Mz[t]);
-                Q[t] = __builtin_fmaf(f[r], v[r].w, Q[t]);
+                Rx[t] = __builtin_fmaf(f[r], v[r].x, Rx[t]);
+                Ry[t] = __builtin_fmaf(f[r], v[r].y, Ry[t]);
+                Rz[t] = __builtin_fmaf(f[r], v[r].z, Rz[t]);
             }
         }
     }
 }
 
-// u += x_i x (tile sum) of a remainder tile: its float64 expansion plus the
-// float32 remainder sums.  mom = {W0, m (3), P (xx yy zz xy xz yz), W}.
+// u += x_i x S of a remainder tile, S = k c + e m + e^2 (P x_i + R) with
+// e = 1/E and k = (W0 + e (m.x_i)) e: the tile's float64 expansion plus its
+// float32 remainder sum.  mom = {W0, m (3), P (xx yy zz xy xz yz), W}.
 __device__ __forceinline__ void bounded_combine(const double *__restrict__ mom, const double4 &sb,
                                                 double xi, double yi, double zi, double E,
-                                                float Mx, float My, float Mz, float Q,
+                                                float Rx, float Ry, float Rz,
                                                 double &ux, double &uy, double &uz)
 {
-    const double ax = xi - sb.x, ay = yi - sb.y, az = zi - sb.z;
     const double W0 = mom[0], m0 = mom[1], m1 = mom[2], m2 = mom[3];
     const double Pxx = mom[4], Pyy = mom[5], Pzz = mom[6], Pxy = mom[7], Pxz = mom[8],
                  Pyz = mom[9];
-    const double inv = 1.0 / E;
-    const double h = (m0 * xi + m1 * yi + m2 * zi) + (double)Q;
-    const double Bx = ((Pxx * xi + Pxy * yi + Pxz * zi) + (double)Mx) - ax * h;
-    const double By = ((Pxy * xi + Pyy * yi + Pyz * zi) + (double)My) - ay * h;
-    const double Bz = ((Pxz * xi + Pyz * yi + Pzz * zi) + (double)Mz) - az * h;
-    const double Sx = ((m0 - ax * W0) + Bx * inv) * inv;
-    const double Sy = ((m1 - ay * W0) + By * inv) * inv;
-    const double Sz = ((m2 - az * W0) + Bz * inv) * inv;
+    const double e = recip_f64(E);
+    const double k = (W0 + (m0 * xi + m1 * yi + m2 * zi) * e) * e;
+    const double e2 = e * e;
+    const double Bx = (Pxx * xi + Pxy * yi + Pxz * zi) + (double)Rx;
+    const double By = (Pxy * xi + Pyy * yi + Pyz * zi) + (double)Ry;
+    const double Bz = (Pxz * xi + Pyz * yi + Pzz * zi) + (double)Rz;
+    const double Sx = k * sb.x + (e * m0 + e2 * Bx);
+    const double Sy = k * sb.y + (e * m1 + e2 * By);
+    const double Sz = k * sb.z + (e * m2 + e2 * Bz);
     ux += yi * Sz - zi * Sy;
     uy += zi * Sx - xi * Sz;
     uz += xi * Sy - yi * Sx;
@@ -811,8 +817,9 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             if (!use64) {
                 const double q = R / rho_c;
                 const double tb = 2.0 * q * (1.0 + q);
+                const double dmin = rho_c - R;
                 use64 = tb >= (double)kBoundedTmax ||
-                        8.0 * kBoundedEps32 * tb * tb * W > tau * (rho_c - R);
+                        8.0 * kBoundedEps32 * tb * tb * W > tau * dmin * fmin(dmin, 1.0);
             }
             f64[t] = use64;
             any |= use64;
@@ -823,18 +830,18 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             __syncthreads();
         }
         if (!any) {
-            float Mx[T], My[T], Mz[T], Q[T], Ef[T];
+            float Rx[T], Ry[T], Rz[T], Ef[T];
             double E[T];
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 E[t] = c0_of(t) - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
                 Ef[t] = (float)E[t];
-                Mx[t] = My[t] = Mz[t] = Q[t] = 0.f;
+                Rx[t] = Ry[t] = Rz[t] = 0.f;
             }
-            remainder_tile<T>(lds, xf, yf, zf, Ef, Mx, My, Mz, Q);
+            remainder_tile<T>(lds, xf, yf, zf, Ef, Rx, Ry, Rz);
 #pragma unroll
             for (int t = 0; t < T; ++t)
-                bounded_combine(mom, sb, xi[t], yi[t], zi[t], E[t], Mx[t], My[t], Mz[t], Q[t],
+                bounded_combine(mom, sb, xi[t], yi[t], zi[t], E[t], Rx[t], Ry[t], Rz[t],
                                 ux[t], uy[t], uz[t]);
         } else if (all) {
             // every target of the lane in float64: one pass over the records
@@ -879,10 +886,9 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                     uz[t] += xi[t] * sy[0] - yi[t] * sx[0];
                 } else {
                     const double E = c0_of(t) - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
-                    float Ef[1] = {(float)E}, Mx[1] = {0.f}, My[1] = {0.f}, Mz[1] = {0.f},
-                          Q[1] = {0.f};
-                    remainder_tile<1>(lds, &xf[t], &yf[t], &zf[t], Ef, Mx, My, Mz, Q);
-                    bounded_combine(mom, sb, xi[t], yi[t], zi[t], E, Mx[0], My[0], Mz[0], Q[0],
+                    float Ef[1] = {(float)E}, Rx[1] = {0.f}, Ry[1] = {0.f}, Rz[1] = {0.f};
+                    remainder_tile<1>(lds, &xf[t], &yf[t], &zf[t], Ef, Rx, Ry, Rz);
+                    bounded_combine(mom, sb, xi[t], yi[t], zi[t], E, Rx[0], Ry[0], Rz[0],
                                     ux[t], uy[t], uz[t]);
                 }
             }
@@ -1020,27 +1026,63 @@ __device__ __forceinline__ void mixed_target_prep_body(const PairArgs &b, int n_
     __syncthreads();
 }
 
-// Tile moments of the bounded pass (kMixedTile threads per tile, after
-// mixed_source_prep_body has written the tile centres): W0 = sum w,
-// m = sum w o, P = sum w o o^T, W = sum |w| with o = x - c in float64.
-__device__ __forceinline__ void bounded_moment_prep_body(const Source *__restrict__ src,
-                                                         int n_sources,
-                                                         const double4 *__restrict__ tile_box,
+// Sums over a workgroup of 256 threads (4 waves) of N values per thread:
+// a butterfly within each wave (both lanes of a step add the same two
+// values, so every lane holds the same wave sum), then the wave sums in
+// wave order.  Every thread gets the N totals; fixed order, deterministic.
+// sh: at least 4 N doubles.
+template <int N>
+__device__ __forceinline__ void block_sums(double (&v)[N], double *sh)
+{
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) v[q] += __shfl_xor(v[q], m, 64);
+    const int wave = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) sh[wave * N + q] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = ((sh[q] + sh[N + q]) + sh[2 * N + q]) + sh[3 * N + q];
+    __syncthreads();
+}
+
+// Source tile `tile` of a bounded pass (kMixedTile threads): the centre c
+// (mean of the valid sources) and radius -> tile_box; the float32 records
+// pos = {fl32(x - c), 0} and wgt = {fl32(w x), 0}; and the float64 moments
+// {W0 = sum w, m = sum w o, P = sum w o o^T (xx yy zz xy xz yz),
+// W = sum |w|, 0} of o = x - c -> mom[tile * kTileMoments ...].
+__device__ __forceinline__ void bounded_source_prep_body(const Source *__restrict__ src,
+                                                         int n_sources, float4 *__restrict__ src32,
+                                                         float4 *__restrict__ srcw,
+                                                         double4 *__restrict__ tile_box,
                                                          double *__restrict__ mom, int tile,
                                                          double *sh)
 {
     const int j = tile * kMixedTile + threadIdx.x;
     const Source s = src[j];
-    const double w = j < n_sources ? s.w : 0.0;
-    const double4 c = tile_box[tile];
-    const double ox = s.x - c.x, oy = s.y - c.y, oz = s.z - c.z;
-    const double v[11] = {w, w * ox, w * oy, w * oz, w * ox * ox, w * oy * oy, w * oz * oz,
-                          w * ox * oy, w * ox * oz, w * oy * oz, fabs(w)};
-    for (int q = 0; q < 11; ++q) {
-        const double b = block_sum(v[q], sh);
-        if (threadIdx.x == 0) mom[(size_t)tile * kTileMoments + q] = b;
+    const bool valid = j < n_sources;
+    const double w = valid ? s.w : 0.0;
+    double cs[4] = {valid ? 1.0 : 0.0, valid ? s.x : 0.0, valid ? s.y : 0.0, valid ? s.z : 0.0};
+    block_sums<4>(cs, sh);
+    const double inv = cs[0] > 0 ? 1.0 / cs[0] : 0.0;
+    const double cx = cs[1] * inv, cy = cs[2] * inv, cz = cs[3] * inv;
+    const double ox = s.x - cx, oy = s.y - cy, oz = s.z - cz;
+    const double r = block_max(valid ? sqrt(ox * ox + oy * oy + oz * oz) : -1.0, sh);
+    src32[j] = make_float4((float)ox, (float)oy, (float)oz, 0.f);
+    srcw[j] = make_float4((float)(w * s.x), (float)(w * s.y), (float)(w * s.z), 0.f);
+    double v[11] = {w, w * ox, w * oy, w * oz, w * ox * ox, w * oy * oy, w * oz * oz,
+                    w * ox * oy, w * ox * oz, w * oy * oz, fabs(w)};
+    block_sums<11>(v, sh);
+    if (threadIdx.x == 0) {
+        tile_box[tile] = make_double4(cx, cy, cz, r);
+        double *m = mom + (size_t)tile * kTileMoments;
+#pragma unroll
+        for (int q = 0; q < 11; ++q) m[q] = v[q];
+        m[11] = 0.0;
     }
-    if (threadIdx.x == 0) mom[(size_t)tile * kTileMoments + 11] = 0.0;
 }
 
 // Velocity-scale probe of the bounded pass: for target row `row` (the

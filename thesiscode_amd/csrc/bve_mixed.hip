@@ -96,12 +96,12 @@ hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s)
 // Bounded mixed mode
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kMixedTile) void bounded_moment_prep_kernel(
-    const Source *__restrict__ src, int n_sources, const double4 *__restrict__ tile_box,
-    double *__restrict__ mom)
+__global__ __launch_bounds__(kMixedTile) void bounded_source_prep_kernel(
+    const Source *__restrict__ src, int n_sources, float4 *__restrict__ src32,
+    float4 *__restrict__ srcw, double4 *__restrict__ tile_box, double *__restrict__ mom)
 {
     __shared__ double sh[kMixedTile];
-    bounded_moment_prep_body(src, n_sources, tile_box, mom, blockIdx.x, sh);
+    bounded_source_prep_body(src, n_sources, src32, srcw, tile_box, mom, blockIdx.x, sh);
 }
 
 // The probe's maximum starts from zero every pass (a kernel, not a memset,
@@ -172,10 +172,13 @@ hipError_t launch_scale_reset(double *scale, hipStream_t s)
 hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
 {
     if (!a.tile_mom || !a.scale) return hipErrorInvalidValue;
-    hipError_t e = launch_mixed_prep(a, s);
+    hipLaunchKernelGGL(bounded_source_prep_kernel, dim3(a.base.n_src_pad / kMixedTile),
+                       dim3(kMixedTile), 0, s, a.base.src, a.n_sources, a.src32, a.srcw,
+                       a.tile_box, a.tile_mom);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bounded_moment_prep_kernel, dim3(a.base.n_src_pad / kMixedTile),
-                       dim3(kMixedTile), 0, s, a.base.src, a.n_sources, a.tile_box, a.tile_mom);
+    hipLaunchKernelGGL(mixed_target_prep_kernel, dim3(a.base.t_stride / 256), dim3(256), 0, s,
+                       a.base, a.n_targets, a.group_box);
     e = hipGetLastError();
     if (e != hipSuccess || !probe) return e;
     e = launch_scale_reset(a.scale, s);

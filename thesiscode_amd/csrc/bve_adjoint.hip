@@ -59,15 +59,18 @@ __device__ __forceinline__ void recips(const double (&d)[R], double (&r)[R])
 // (P_kl = x_l . n_k, P_lk = x_k . n_l, d = 1 - x_k . x_l).  The two other
 // sums of the gradient come without a pass of their own: S_k is the forward
 // sum of the same stage input, kept by the forward sweep, and
-// g_k = sum_l P_lk / d = x_k . N_k.  A pair costs 22 float64 operations
-// with four pairs sharing a reciprocal.
+// g_k = sum_l P_lk / d = x_k . N_k.  With the records carrying w_l x_l and
+// each lane w_k x_k, w_l P_kl + w_k P_lk = (w_l x_l).n_k + (w_k x_k).n_l is one
+// chain of six FMAs, and a pair costs about 20 float64 operations with four
+// pairs sharing a reciprocal.
 
 // Source tile at jt (in LDS) against the T targets of this lane; MASK: the
 // tile holds the own index of some lane's target (self pairs are dropped).
 template <int T, int R, bool MASK>
 __device__ __forceinline__ void vjp_tile(const AdjRec *lds, int jt, const int (&self)[T],
                                          const double (&xk)[T], const double (&yk)[T],
-                                         const double (&zk)[T], const double (&wk)[T],
+                                         const double (&zk)[T], const double (&ax)[T],
+                                         const double (&ay)[T], const double (&az)[T],
                                          const double (&ex)[T], const double (&ey)[T],
                                          const double (&ez)[T], double (&Vx)[T], double (&Vy)[T],
                                          double (&Vz)[T], double (&Nx)[T], double (&Ny)[T],
@@ -75,33 +78,51 @@ __device__ __forceinline__ void vjp_tile(const AdjRec *lds, int jt, const int (&
 {
     constexpr int TILE = 256;
     for (int k = 0; k < TILE; k += R) {
-        AdjRec a[R];
+        // positions first (denominators and reciprocals of every target),
+        // the rest of each record where its pairs are summed
+        double px[R], py[R], pz[R], rc[T][R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = lds[k + r];
+        for (int r = 0; r < R; ++r) {
+            px[r] = lds[k + r].x;
+            py[r] = lds[k + r].y;
+            pz[r] = lds[k + r].z;
+        }
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-            double d[R], rc[R];
+            double d[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                d[r] = __builtin_fma(-xk[t], a[r].x, 1.0);
-                d[r] = __builtin_fma(-yk[t], a[r].y, d[r]);
-                d[r] = __builtin_fma(-zk[t], a[r].z, d[r]);
+                d[r] = __builtin_fma(-xk[t], px[r], 1.0);
+                d[r] = __builtin_fma(-yk[t], py[r], d[r]);
+                d[r] = __builtin_fma(-zk[t], pz[r], d[r]);
                 if (MASK && jt + k + r == self[t]) d[r] = 1.0;
             }
-            recips<R>(d, rc);
+            recips<R>(d, rc[t]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                double rr = rc[r];
-                if (MASK && jt + k + r == self[t]) rr = 0.0;
-                const double pt = __builtin_fma(a[r].z, ez[t], __builtin_fma(a[r].y, ey[t], a[r].x * ex[t]));
-                const double ps = __builtin_fma(zk[t], a[r].nz, __builtin_fma(yk[t], a[r].ny, xk[t] * a[r].nx));
-                const double gam = (rr * rr) * __builtin_fma(a[r].w, pt, wk[t] * ps);
-                Vx[t] = __builtin_fma(gam, a[r].x, Vx[t]);
-                Vy[t] = __builtin_fma(gam, a[r].y, Vy[t]);
-                Vz[t] = __builtin_fma(gam, a[r].z, Vz[t]);
-                Nx[t] = __builtin_fma(rr, a[r].nx, Nx[t]);
-                Ny[t] = __builtin_fma(rr, a[r].ny, Ny[t]);
-                Nz[t] = __builtin_fma(rr, a[r].nz, Nz[t]);
+            for (int r = 0; r < R; ++r)
+                if (MASK && jt + k + r == self[t]) rc[t][r] = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const AdjRec &a = lds[k + r];
+            const double wx = a.wx, wy = a.wy, wz = a.wz, nx = a.nx, ny = a.ny, nz = a.nz;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const double rr = rc[t][r];
+                // (w_l x_l).n_k + (w_k x_k).n_l
+                double sp = az[t] * nz;
+                sp = __builtin_fma(ay[t], ny, sp);
+                sp = __builtin_fma(ax[t], nx, sp);
+                sp = __builtin_fma(wz, ez[t], sp);
+                sp = __builtin_fma(wy, ey[t], sp);
+                sp = __builtin_fma(wx, ex[t], sp);
+                const double gam = (rr * rr) * sp;
+                Vx[t] = __builtin_fma(gam, px[r], Vx[t]);
+                Vy[t] = __builtin_fma(gam, py[r], Vy[t]);
+                Vz[t] = __builtin_fma(gam, pz[r], Vz[t]);
+                Nx[t] = __builtin_fma(rr, nx, Nx[t]);
+                Ny[t] = __builtin_fma(rr, ny, Ny[t]);
+                Nz[t] = __builtin_fma(rr, nz, Nz[t]);
             }
         }
     }
@@ -117,7 +138,7 @@ __device__ __forceinline__ void vjp_range(const AdjRec *__restrict__ rec, int se
     constexpr int TILE = 256;
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-    double xk[T], yk[T], zk[T], wk[T], ex[T], ey[T], ez[T];
+    double xk[T], yk[T], zk[T], ax[T], ay[T], az[T], ex[T], ey[T], ez[T];
     double Vx[T], Vy[T], Vz[T], Nx[T], Ny[T], Nz[T];
     int self[T];
 #pragma unroll
@@ -127,7 +148,9 @@ __device__ __forceinline__ void vjp_range(const AdjRec *__restrict__ rec, int se
         xk[t] = a.x;
         yk[t] = a.y;
         zk[t] = a.z;
-        wk[t] = a.w;
+        ax[t] = a.wx;
+        ay[t] = a.wy;
+        az[t] = a.wz;
         ex[t] = a.nx;
         ey[t] = a.ny;
         ez[t] = a.nz;
@@ -143,9 +166,11 @@ __device__ __forceinline__ void vjp_range(const AdjRec *__restrict__ rec, int se
         for (int t = 0; t < T; ++t)
             diag |= diag_tile(tile_base + t * kBlock + wave * 64, self_end, jt, TILE);
         if (diag)
-            vjp_tile<T, R, true>(lds, jt, self, xk, yk, zk, wk, ex, ey, ez, Vx, Vy, Vz, Nx, Ny, Nz);
+            vjp_tile<T, R, true>(lds, jt, self, xk, yk, zk, ax, ay, az, ex, ey, ez, Vx, Vy, Vz, Nx, Ny,
+                                 Nz);
         else
-            vjp_tile<T, R, false>(lds, jt, self, xk, yk, zk, wk, ex, ey, ez, Vx, Vy, Vz, Nx, Ny, Nz);
+            vjp_tile<T, R, false>(lds, jt, self, xk, yk, zk, ax, ay, az, ex, ey, ez, Vx, Vy, Vz, Nx,
+                                  Ny, Nz);
     }
 #pragma unroll
     for (int t = 0; t < T; ++t) {
@@ -200,7 +225,7 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(AdjStage a)
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= a.n_pad) return;
     if (i >= a.n) {
-        a.rec[i] = AdjRec{0, 0, 0, 0, 0, 0, 0, 0};
+        a.rec[i] = AdjRec{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         return;
     }
     const int st = a.stride;
@@ -217,8 +242,8 @@ __global__ __launch_bounds__(256) void adj_prep_kernel(AdjStage a)
     a.m[st + i] = my;
     a.m[2 * st + i] = mz;
     // n = m x x
-    a.rec[i] = AdjRec{y.x, y.y, y.z, y.w, my * y.z - mz * y.y, mz * y.x - mx * y.z,
-                      mx * y.y - my * y.x, 0.0};
+    a.rec[i] = AdjRec{y.x, y.y, y.z, 0.0, y.w * y.x, y.w * y.y, y.w * y.z, my * y.z - mz * y.y,
+                      mz * y.x - mx * y.z, mx * y.y - my * y.x};
 }
 
 // Ybar of stage s from the pair pass's segment rows and the forward sums S

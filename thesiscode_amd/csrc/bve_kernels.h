@@ -51,10 +51,11 @@ struct MixedArgs {
 
 constexpr int kTileMoments = 12;
 
-// Record of one active panel in the adjoint's pair pass: position, weight
-// w = zeta A and n = m x x (m its velocity cotangent); 64 B.
+// Record of one active panel in the adjoint's pair pass: position x, the
+// weighted position w x (w = zeta A) and n = m x x (m its velocity
+// cotangent); 80 B, the position in the first 32.
 struct __align__(16) AdjRec {
-    double x, y, z, w, nx, ny, nz, pad;
+    double x, y, z, pad, wx, wy, wz, nx, ny, nz;
 };
 
 // Partial rows per (segment, target tile) of the adjoint's pair pass:

@@ -48,6 +48,14 @@ ERR_BAR = 1e-10
 METRIC = "Biot-Savart pair-interactions/s per RK4 step at <=1e-10 rel err; adjoint/fwd ratio"
 MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4,
          "mixed_bounded": 5}
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE (x 2.00) + WRITE_SIZE
+# (x 1.00), the factors a plain copy of known size gives (thesiscode_amd/
+# copy_probe), for the default plans on the level-7 Gaussian vortex
+# (profiles/pair_sum_bounded_L7_counters.md, pair_sum_f64_fast_L7_traffic_pmc.md)
+MEASURED_TRAFFIC = {
+    ("pair_sum_bounded_seg_kernel<2>", 7): 1.331e9,
+    ("pair_sum_f64_seg_kernel<4,512,4>", 7): 1.99e8,
+}
 
 
 def kernel_name(mode, T, tile, recip, segments):
@@ -399,12 +407,14 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
     ctx.set_profiling(False)
     T, tile, chunk, n_chunks, recip = ctx.plan()
     launch = ctx.launch()
-    # HBM bytes per launch: the partial-sum rows written (24 B per target per
-    # row), the targets each row's work item reads (the same count) and the
-    # source records (32 B each); for the float64 segmented launch at level 7
-    # this is 206 MB against 199 MB from rocprofv3 FETCH_SIZE + WRITE_SIZE
-    # (copy-calibrated)
-    traffic = 2.0 * launch["partial_bytes"] + 32.0 * ctx.n_active
+    # HBM bytes per launch, modelled: the partial-sum rows written (24 B per
+    # target per row), the targets each row's work item reads (the same
+    # count) and the source records (32 B each); for the float64 segmented
+    # launch at level 7 this is 206 MB against 199 MB measured.  The mixed
+    # and bounded passes also fetch their source records again from HBM
+    # (6.3 MB of them per pass exceed an XCD's L2), which the model leaves out.
+    traffic_model = 2.0 * launch["partial_bytes"] + 32.0 * ctx.n_active
+    kernel = kernel_name(mode, T, tile, recip, launch["segments"])
     # accuracy of this mode vs the oracle: the resident state's velocities
     # from the same plan and kernel as the timed launches
     reload()
@@ -424,8 +434,9 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
                 peak_tflops=peak_of(mode), roofline_frac=achieved / peak_of(mode),
                 plan=dict(targets_per_lane=T, tile=tile, chunk=chunk, n_chunks=n_chunks,
                           recip=recip, **launch),
-                kernel=kernel_name(mode, T, tile, recip, launch["segments"]),
-                traffic_bytes_per_launch=traffic)
+                kernel=kernel,
+                traffic_bytes_per_launch=MEASURED_TRAFFIC.get((kernel, args.level)),
+                traffic_model_bytes_per_launch=traffic_model)
 
 
 def pareto_curve(ctx, reload, sample, dt, steps, base_launch_ms, base_step_ms):
@@ -529,7 +540,7 @@ def large_solve_measurement(level=8, modes=("fp64", "fp64_fast", "mixed", "mixed
         ctx.load_panels(g.active_x, za, g.active_area, g.passive_x)
         ctx.load_particles(g.particles_x, absv)
 
-    args = types.SimpleNamespace(dt=dt, warmup=1, steps=1)
+    args = types.SimpleNamespace(dt=dt, warmup=1, steps=1, level=level)
     ctx.set_precision(MODES["fp64"])
     sample.set_step_reference(ctx, reload, dt)
     out = {}
@@ -633,9 +644,11 @@ def main():
             "unit": "TFLOP/s",
             "frac": r.get("roofline_frac"),
             "traffic": r.get("traffic_bytes_per_launch"),
-            "traffic_source": "launch-plan model (partial-sum rows written + the targets their "
-                              "work items read + sources per XCD), checked against rocprofv3 "
-                              "FETCH_SIZE/WRITE_SIZE",
+            "traffic_model": r.get("traffic_model_bytes_per_launch"),
+            "traffic_source": "rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per launch of this "
+                              "kernel on this workload (copy-calibrated; null when not measured); "
+                              "traffic_model: partial-sum rows written + the targets their "
+                              "segments read + the source records",
             "kernel": r.get("kernel"),
             "launches": r.get("launches"),
             "avg_launch_ms": r.get("avg_launch_ms"),

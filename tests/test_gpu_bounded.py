@@ -87,6 +87,9 @@ def test_level7_gauss_within_target(chunk):
     with bounded_context(g, za, absv) as c:
         c.set_chunk(chunk)
         assert c.launch()["segments"] > 0
+        # two targets per lane: pair_sum_bounded_seg_kernel<2>, the kernel
+        # bench.py times
+        assert c.plan()[0] == 2
         if chunk:
             assert c.plan()[2] == chunk
         u = c.resident_velocity()

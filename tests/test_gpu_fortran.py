@@ -88,6 +88,18 @@ def test_reference_driver_runs_on_the_dropin(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(DROPIN),
                     reason="the drop-in driver is built only where the reference sources are")
+def test_reference_driver_runs_on_the_dropin_bounded(tmp_path):
+    """The same driver with BVE_PRECISION=mixed_bounded (float64 tile
+    expansions, float32 remainders, error target 1e-10): its table agrees
+    with the reference's to 1e-9 relative over the 12 steps."""
+    got = run_dropin(tmp_path, "rh4_driver_L4_noremesh.namelist", precision="mixed_bounded")
+    ref = read_dat(os.path.join(GOLDEN, "rh4_driver_L4_noremesh.dat"))
+    assert got.shape == ref.shape
+    assert np.allclose(got, ref, rtol=1e-9, atol=1e-15), np.abs(got - ref).max(axis=0)
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN),
+                    reason="the drop-in driver is built only where the reference sources are")
 def test_reference_driver_remesh_on_the_dropin(tmp_path):
     """The same with a remesh every 5 steps: the driver calls ResetRK4 and
     the module re-gathers the new grid onto the device.  Rows before the

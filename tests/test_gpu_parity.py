@@ -139,6 +139,7 @@ def test_reruns_are_bitwise_reproducible(ctx):
     (_lib.BVE_PRECISION_FP64, (4, 256, 4096, 0)), (_lib.BVE_PRECISION_FP64, (4, 512, 0, 0)),
     (_lib.BVE_PRECISION_FP64_FAST, (1, 256, 0, 2)), (_lib.BVE_PRECISION_FP64_FAST, (2, 512, 64, 4)),
     (_lib.BVE_PRECISION_FP64_FAST, (4, 256, 4096, 4)), (_lib.BVE_PRECISION_FP64_FAST, (4, 512, 0, 2)),
+    (_lib.BVE_PRECISION_FP64_FAST, (4, 512, 0, 4)),
     # segments longer than a target tile's row of source tiles: one segment
     # for everything, segments cutting tiles unevenly, segment and tile
     # boundaries coinciding, and a short last segment
@@ -165,6 +166,42 @@ def test_launch_plans_agree(ctx, precision, plan):
     assert normwise_rel(u[:na], ref_a) < TOL
     assert normwise_rel(u[na:na + npp], ref_s) < TOL
     assert normwise_rel(u[na + npp:], ref_p) < TOL
+
+
+@pytest.mark.parametrize("precision,plan,bar", [
+    (_lib.BVE_PRECISION_MIXED, (1, 256, 0, 0), 1e-6), (_lib.BVE_PRECISION_MIXED, (2, 256, 0, 0), 1e-6),
+    (_lib.BVE_PRECISION_MIXED, (4, 256, 0, 0), 1e-6),
+    (_lib.BVE_PRECISION_MIXED_KAHAN, (2, 256, 0, 0), 1e-6),
+    (_lib.BVE_PRECISION_MIXED_BOUNDED, (1, 256, 0, 0), 1e-10),
+    (_lib.BVE_PRECISION_MIXED_BOUNDED, (2, 256, 0, 0), 1e-10),
+    (_lib.BVE_PRECISION_MIXED_BOUNDED, (4, 256, 0, 0), 1e-10),
+    (_lib.BVE_PRECISION_MIXED_BOUNDED, (2, 256, 5, 0), 1e-10)])
+def test_reduced_precision_plans_agree(ctx, precision, plan, bar):
+    """The reduced-precision passes on every targets-per-lane value they are
+    compiled for (the level-7 defaults are 2 for mixed and bounded), and on
+    an uneven segment count, against the oracle for all three target
+    classes at their own bars: 1e-6 for float32 far-field terms (measured
+    3e-7 to 5e-7 at levels 4 to 8), the error target 1e-10 for the bounded
+    mode."""
+    g = gridmod.cubed_sphere(5)
+    z = ic.rh4_wave(g.active_x)
+    na, npp = g.n_active, g.n_passive
+    ref_a, _ = oracle.active_rhs(g.active_x, z, g.active_area, 0.0)
+    ref_s = oracle.smooth_velocity(g.passive_x, g.active_x, z, g.active_area, 0.05)
+    ref_p = oracle.passive_rhs(g.particles_x, g.active_x, z, g.active_area)
+    t = np.concatenate([g.active_x, g.passive_x, g.particles_x])
+    ctx.set_smooth(0.05)
+    ctx.set_precision(precision)
+    ctx.set_plan(*plan)
+    try:
+        u = ctx.velocity(t, g.active_x, z, g.active_area, self_end=na, smooth_range=(na, na + npp))
+    finally:
+        ctx.set_plan(0, 0, 0, 0)
+        ctx.set_precision(_lib.BVE_PRECISION_FP64)
+    errs = (normwise_rel(u[:na], ref_a), normwise_rel(u[na:na + npp], ref_s),
+            normwise_rel(u[na + npp:], ref_p))
+    print(f"precision {precision} plan {plan}: " + " ".join(f"{e:.2e}" for e in errs))
+    assert max(errs) < bar
 
 
 def test_fast_fp64_step_matches_reference(ctx):

@@ -7,7 +7,8 @@ module BVEInterfaceModule
 	implicit none
 	private
 	public :: BVE_OK, BVE_PRECISION_FP64, BVE_PRECISION_MIXED, BVE_PRECISION_FP64_FAST
-	public :: BVE_PRECISION_MIXED_KAHAN, BVE_PRECISION_REFERENCE, bve_precision_from_env
+	public :: BVE_PRECISION_MIXED_KAHAN, BVE_PRECISION_REFERENCE, BVE_PRECISION_MIXED_BOUNDED
+	public :: bve_precision_from_env, bve_set_error_target
 	public :: bve_create, bve_destroy, bve_set_omega, bve_set_smooth, bve_set_precision
 	public :: bve_set_renormalize, bve_load_panels, bve_rk4_step, bve_store_panels, bve_reset
 	public :: bve_load_panel_absvort, bve_stratosphere_rk4_step, bve_advection_rk4_step
@@ -17,7 +18,7 @@ module BVEInterfaceModule
 	integer(c_int), parameter :: BVE_OK = 0
 	integer(c_int), parameter :: BVE_PRECISION_FP64 = 0, BVE_PRECISION_MIXED = 1, &
 	                             BVE_PRECISION_FP64_FAST = 2, BVE_PRECISION_MIXED_KAHAN = 3, &
-	                             BVE_PRECISION_REFERENCE = 4
+	                             BVE_PRECISION_REFERENCE = 4, BVE_PRECISION_MIXED_BOUNDED = 5
 
 	interface
 		integer(c_int) function bve_create(ctx, device) bind(C, name="bve_create")
@@ -54,6 +55,13 @@ module BVEInterfaceModule
 			type(c_ptr), value :: ctx
 			integer(c_int), value :: precision
 			real(c_double), value :: nearfield
+		end function
+
+		integer(c_int) function bve_set_error_target(ctx, rel_err) &
+				bind(C, name="bve_set_error_target")
+			import :: c_int, c_ptr, c_double
+			type(c_ptr), value :: ctx
+			real(c_double), value :: rel_err
 		end function
 
 		integer(c_int) function bve_set_renormalize(ctx, on) bind(C, name="bve_set_renormalize")
@@ -192,8 +200,8 @@ contains
 	end function
 
 	! Arithmetic the drop-in modules run: the environment variable
-	! BVE_PRECISION (fp64, fp64_fast, mixed, mixed_kahan or reference) when it
-	! is set, else `default`.  An unknown value is an error (-1).
+	! BVE_PRECISION (fp64, fp64_fast, mixed, mixed_kahan, mixed_bounded or
+	! reference) when it is set, else `default`.  An unknown value is an error (-1).
 	function bve_precision_from_env(default) result(precision)
 		integer(c_int), intent(in) :: default
 		integer(c_int) :: precision
@@ -213,6 +221,8 @@ contains
 			precision = BVE_PRECISION_MIXED
 		case ('mixed_kahan')
 			precision = BVE_PRECISION_MIXED_KAHAN
+		case ('mixed_bounded')
+			precision = BVE_PRECISION_MIXED_BOUNDED
 		case ('reference')
 			precision = BVE_PRECISION_REFERENCE
 		case default

@@ -344,9 +344,9 @@ class ErrorSample:
         """Velocities of the resident state evaluated by the context's own
         pass (the plan and kernel a step times, over all targets at once)
         against the oracle at the sample.  Returns (normwise, max pointwise,
-        per class): normwise max_i |du_i| / max_i |u_i| over the whole
+        per class, rms): normwise max_i |du_i| / max_i |u_i| over the whole
         sample, pointwise max_i |du_i| / |u_i| over targets with |u_i| >
-        1e-3 max |u|."""
+        1e-3 max |u|, rms sqrt(mean |du_i|^2 / mean |u_i|^2)."""
         ua, up, uq = ctx.resident_velocity()
         got = np.concatenate([ua[self.ia], up[self.ip], uq[self.iq]])
         ref = np.concatenate([self.ua, self.up, self.uq])
@@ -357,7 +357,8 @@ class ErrorSample:
         per = {cls: float(du[sl].max() / un.max())
                for cls, sl in (("active", slice(0, k1)), ("passive", slice(k1, k2)),
                                ("particles", slice(k2, None)))}
-        return float(du.max() / un.max()), float((du[keep] / un[keep]).max()), per
+        rms = float(np.sqrt(np.mean(du ** 2) / np.mean(un ** 2)))
+        return float(du.max() / un.max()), float((du[keep] / un[keep]).max()), per, rms
 
     @staticmethod
     def positions(ctx):
@@ -418,7 +419,7 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
     # accuracy of this mode vs the oracle: the resident state's velocities
     # from the same plan and kernel as the timed launches
     reload()
-    err, err_point, err_class = sample.errors(ctx)
+    err, err_point, err_class, err_rms = sample.errors(ctx)
     step_err = sample.step_error(ctx, reload, args.dt)
     ctx.set_precision(_lib.BVE_PRECISION_FP64)
     avg_ms = kernel_ms / max(launches, 1)
@@ -426,7 +427,7 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
     return dict(pair_rate=pairs_primary * args.steps / el,
                 pair_rate_all_classes=pairs_all * args.steps / el,
                 ms_per_step=el * 1e3 / max(args.steps, 1), rel_err_vs_fp64_oracle=err,
-                max_pointwise_rel_err=err_point, rel_err_per_class=err_class,
+                max_pointwise_rel_err=err_point, rms_rel_err=err_rms, rel_err_per_class=err_class,
                 step_rel_err_vs_fp64_step=step_err,
                 error_sample=f"{len(sample)} targets (active, passive and particle classes), "
                              f"velocities of the resident state from the timed plan",
@@ -468,7 +469,7 @@ def pareto_curve(ctx, reload, sample, dt, steps, base_launch_ms, base_step_ms):
         launches, kernel_ms = ctx.profile()
         ctx.set_profiling(False)
         reload()
-        err, err_point, _ = sample.errors(ctx)
+        err, err_point, _, err_rms = sample.errors(ctx)
         step_err = sample.step_error(ctx, reload, dt)
         launch_ms = kernel_ms / max(launches, 1)
         step_ms = el * 1e3 / steps
@@ -477,7 +478,7 @@ def pareto_curve(ctx, reload, sample, dt, steps, base_launch_ms, base_step_ms):
                         ms_per_launch=launch_ms, ms_per_step=step_ms,
                         speedup_vs_fp64_launch=base_launch_ms / launch_ms,
                         speedup_vs_fp64_step=base_step_ms / step_ms,
-                        rel_err=err, max_pointwise_rel_err=err_point,
+                        rel_err=err, max_pointwise_rel_err=err_point, rms_rel_err=err_rms,
                         step_rel_err=step_err))
         print(f"# pareto {out[-1]}", file=sys.stderr, flush=True)
     ctx.set_error_target(1e-10)

@@ -662,6 +662,10 @@ def main():
         # the gradient against a forward of the map it differentiates (the
         # active panels alone, the same 20 steps and arithmetic)
         out["adjoint_fwd_ratio"] = adj["adjoint_fwd_ratio_same_map"]
+        # the same in the shared-reciprocal float64 mode (its forward passes
+        # and recomputation on pair_sum_f64_seg_kernel<..,4>)
+        adj["fp64_fast"] = adjoint_measurement(args.adjoint_level, n_fd=0, mode="fp64_fast")
+        print(f"# adjoint fp64_fast: {adj['fp64_fast']}", file=sys.stderr, flush=True)
         out["config"]["adjoint"] = adj
     if args.ensemble_members > 0:
         ens = ensemble_measurement(args.ensemble_members,

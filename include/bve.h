@@ -219,7 +219,9 @@ int bve_velocity(bve_ctx *ctx, int n_targets, const double *x_targets, int self_
  * until the process ends and shared by every call on that GPU, so calls
  * that overlap on one GPU race.  At most 262144 targets and 262144
  * sources; float64, float64-fast and reference-order modes (the others
- * return BVE_ERR_ARGUMENT).  Segmented plans are capped at 1024 segments. */
+ * return BVE_ERR_ARGUMENT).  Segmented plans are capped at 1024 segments.
+ * Its pass is not timed by bve_set_profiling (timing events would be
+ * created on demand). */
 int bve_velocity_noalloc(bve_ctx *ctx, int n_targets, const double *x_targets, int self_end,
                          int smooth_begin, int smooth_end, int n_sources, const double *x_sources,
                          const double *vort, const double *area, double *u, double *dvort);

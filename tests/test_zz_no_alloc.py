@@ -78,3 +78,17 @@ def test_limits_and_modes_are_checked():
         c.set_precision(_lib.BVE_PRECISION_MIXED)
         with pytest.raises(_lib.BVEError, match="float64 and reference-order modes"):
             c.velocity_noalloc(10, 1, 10, 1, 1, 1, 1)
+
+
+def test_profiling_does_not_time_the_noalloc_pass():
+    """With profiling on, the allocation-free pass records no timing events
+    (they would be created on demand); the result is unchanged."""
+    g, z, t, na, nb = problem(3)
+    with _lib.Context(0) as c:
+        ref, _ = noalloc(c, t, g.active_x, z, g.active_area, na, (na, nb))
+        c.set_profiling(True)
+        got, _ = noalloc(c, t, g.active_x, z, g.active_area, na, (na, nb))
+        launches, _ = c.profile()
+        c.set_profiling(False)
+    assert launches == 0
+    assert np.array_equal(got, ref)

@@ -636,7 +636,10 @@ PairArgs pair_args_of(const bve_ctx *c, const PairPlan &plan)
 
 // One pair-sum pass over n_targets targets and n_sources sources in the
 // context's arithmetic (the mixed pass first builds its per-stage records).
-int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
+// profile = false: no timing events even when profiling is on (they may be
+// created on demand, which the allocation-free entry point must not do).
+int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources,
+              bool profile = true)
 {
     PairArgs a = a_in;
     a.n_targets = n_targets;
@@ -651,7 +654,8 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
         c->scale_from_stage = false;
     }
     std::pair<hipEvent_t, hipEvent_t> ev{};
-    if (c->profiling) {
+    const bool timed = profile && c->profiling;
+    if (timed) {
         ev = take_events(c);
         HIP_TRY(c, hipEventRecord(ev.first, c->stream));
     }
@@ -663,7 +667,7 @@ int pair_pass(bve_ctx *c, const PairArgs &a_in, int n_targets, int n_sources)
         HIP_TRY(c, launch_pair_sum_exact(a, c->stream));
     else
         HIP_TRY(c, launch_pair_sum_f64(a, c->stream));
-    if (c->profiling) {
+    if (timed) {
         HIP_TRY(c, hipEventRecord(ev.second, c->stream));
         c->pending.push_back(ev);
     }
@@ -1644,7 +1648,7 @@ int bve_velocity_noalloc(bve_ctx *c, int n_targets, const double *x_targets, int
     pa.partial = partial;
     pa.queue = queue;
     pa.area = area;
-    const int rc = pair_pass(c, pa, n_targets, n_sources);
+    const int rc = pair_pass(c, pa, n_targets, n_sources, false);
     if (rc) return rc;
     HIP_TRY(c, launch_finalize_velocity(partial, plan, t, stride, n_targets, c->omega, u,
                                         dvort, dvort ? self_end : 0, c->status, c->stream));

@@ -179,3 +179,33 @@ def test_ragged_inputs(na, npp, npart):
             err = np.abs(got - r[key]).max() / disp
             print(f"{key}: {err:.2e} (bar {bound:.1e})")
             assert err < bound, key
+
+
+def test_rh4_ten_steps_track_float64():
+    """A smooth field (RH4 at level 6, all three target classes) over 10
+    RK4 steps at dt = 0.01: the bounded mode's positions and vorticity stay
+    within 10 x 1e-10 of the float64 run's, relative to their change over
+    the run (the per-evaluation target, accumulated over the steps)."""
+    g = gridmod.cubed_sphere(6)
+    z = ic.rh4_wave(g.active_x)
+    omega = ic.OMEGA_EARTH
+    absv = ic.rh4_wave(g.particles_x) + 2 * omega * g.particles_x[:, 2]
+    out = {}
+    for precision in (_lib.BVE_PRECISION_FP64, BOUNDED):
+        print("RH4 level 6, 10 steps, precision", precision, flush=True)
+        with _lib.Context(0) as c:
+            c.set_omega(omega)
+            c.set_precision(precision)
+            c.load_panels(g.active_x, z, g.active_area, g.passive_x)
+            c.load_particles(g.particles_x, absv)
+            c.advance(0.01, 10)
+            ax, av, px = c.store_panels()
+            qx, _ = c.store_particles()
+        out[precision] = (np.concatenate([ax, px, qx]), av)
+    (x64, z64), (xb, zb) = out[_lib.BVE_PRECISION_FP64], out[BOUNDED]
+    disp = np.abs(x64 - np.concatenate([g.active_x, g.passive_x, g.particles_x])).max()
+    pos = np.abs(xb - x64).max() / disp
+    vort = np.abs(zb - z64).max() / np.abs(z64 - z).max()
+    print(f"bounded RH4 L6, 10 steps: positions {pos:.2e}, vorticity {vort:.2e}")
+    assert pos < 1e-9
+    assert vort < 1e-9

@@ -622,7 +622,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        # the arithmetic of the headline mode: float64, or float32 pair terms
+        # with float64 expansions, near field and sums
+        "dtype": "f32+f64" if (best or "").startswith("mixed") else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"gauss_vortex_cubed_sphere_L{args.level}_rk4_step",

@@ -12,7 +12,8 @@ from thesiscode_amd import _lib
 
 pytestmark = pytest.mark.gpu
 MODES = [_lib.BVE_PRECISION_FP64, _lib.BVE_PRECISION_FP64_FAST, _lib.BVE_PRECISION_MIXED,
-         _lib.BVE_PRECISION_MIXED_KAHAN, _lib.BVE_PRECISION_REFERENCE]
+         _lib.BVE_PRECISION_MIXED_KAHAN, _lib.BVE_PRECISION_REFERENCE,
+         _lib.BVE_PRECISION_MIXED_BOUNDED]
 
 
 def panels(n=600, seed=5):

@@ -351,7 +351,8 @@ def test_voronoi_step_matches_oracle(ctx):
     assert tot == pytest.approx(r["total_ke"], rel=2 * TOL)
 
 
-@pytest.mark.parametrize("precision", [0, 2, _lib.BVE_PRECISION_REFERENCE])
+@pytest.mark.parametrize("precision", [0, 2, _lib.BVE_PRECISION_REFERENCE,
+                                       _lib.BVE_PRECISION_MIXED_BOUNDED])
 @pytest.mark.parametrize("name", sorted(voronoi_manifest()))
 def test_voronoi_steps_match_reference(ctx, name, precision):
     """bve_voronoi_rk4_step on icosahedral Voronoi grids against the compiled
@@ -359,7 +360,9 @@ def test_voronoi_steps_match_reference(ctx, name, precision):
     the reference's state before it (its STRIPACK areas, the saved initial
     areas under UseSavedArea).  Positions within 1e-12 of the step's
     displacement, relVort within 1e-12 of its change, KE per generator and
-    totalKE (1/2 sum KE * current area, as the drop-in forms it) 1e-12."""
+    totalKE (1/2 sum KE * current area, as the drop-in forms it) 1e-12; the
+    bounded mode to its error target, 1e-10."""
+    bar = 1e-10 if precision == _lib.BVE_PRECISION_MIXED_BOUNDED else TOL
     c = voronoi_manifest()[name]
     g = load_golden(name)
     print("case", name, "precision", precision, flush=True)
@@ -373,12 +376,12 @@ def test_voronoi_steps_match_reference(ctx, name, precision):
             got = np.stack([gx, gy, gz], 1)
             ref = np.stack([g[f"s{s}_x"], g[f"s{s}_y"], g[f"s{s}_z"]], 1)
             disp = np.abs(ref - np.stack([x, y, z], 1)).max()
-            assert np.abs(got - ref).max() < TOL * disp, (name, s)
+            assert np.abs(got - ref).max() < bar * disp, (name, s)
             dz = np.abs(g[f"s{s}_relvort"] - rv).max()
-            assert np.abs(grv - g[f"s{s}_relvort"]).max() < TOL * dz, (name, s)
-            assert normwise_rel(ke, g[f"ke_step{s}"]) < TOL, (name, s)
+            assert np.abs(grv - g[f"s{s}_relvort"]).max() < bar * dz, (name, s)
+            assert normwise_rel(ke, g[f"ke_step{s}"]) < bar, (name, s)
             tot = 0.5 * np.sum(ke * ca)
-            assert tot == pytest.approx(g[f"total_ke_step{s}"][0], rel=TOL), (name, s)
+            assert tot == pytest.approx(g[f"total_ke_step{s}"][0], rel=bar), (name, s)
             if precision == _lib.BVE_PRECISION_REFERENCE:
                 # the reference's order and rounding: bit for bit
                 for key, v in (("x", gx), ("y", gy), ("z", gz), ("relvort", grv)):

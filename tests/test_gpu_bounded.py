@@ -112,6 +112,30 @@ def test_level8_two_vortices_within_target():
     assert err < TARGET
 
 
+def test_level8_step_within_target():
+    """config[4]: one RK4 step of the level-8 two-vortex state at dt = 0.002
+    in the bounded mode against the float64 step (itself held to the oracle
+    on sampled targets in test_gpu_workloads): every target's position
+    within 1e-10 of the step's displacement."""
+    g = gridmod.cubed_sphere(8)
+    za, zp, _ = ic.two_vortices(g)
+    absv = zp + 2 * OMEGA * g.particles_x[:, 2]
+    x0 = np.concatenate([g.active_x, g.passive_x, g.particles_x])
+    out = {}
+    for precision in (_lib.BVE_PRECISION_FP64, BOUNDED):
+        print("level 8 step, precision", precision, flush=True)
+        with bounded_context(g, za, absv) as c:
+            c.set_precision(precision)
+            c.advance(0.002, 1)
+            ax, _, px = c.store_panels()
+            qx, _ = c.store_particles()
+        out[precision] = np.concatenate([ax, px, qx])
+    disp = np.abs(out[_lib.BVE_PRECISION_FP64] - x0).max()
+    err = np.abs(out[BOUNDED] - out[_lib.BVE_PRECISION_FP64]).max() / disp
+    print(f"level 8 bounded step: {err:.3e}")
+    assert err < TARGET
+
+
 def test_level7_step_within_target():
     """One RK4 step at level 7 in the bounded mode against the oracle's
     BVERK4 (every active panel, sampled parents and vertices)."""

@@ -234,6 +234,23 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
     J1, gz1, _ = gradient(ctx, F, g.active_area, dt, steps, steps)
     t_adj_stored = time.perf_counter() - t0
     stored_diff = float(np.abs(gz1 - gz).max() / np.abs(gz).max())
+    # the mixed reverse sweep (float64 mode only): float32 pair terms for
+    # well-separated tiles, recomputation with shared float64 reciprocals;
+    # the forward sweep, hence J and the checkpoints, stays float64
+    mixed = None
+    if mode == "fp64":
+        ratio = 0.7
+        ctx.set_adjoint_precision(_lib.BVE_PRECISION_MIXED, ratio)
+        gradient(ctx, F, g.active_area, dt, 2, 1)
+        t0 = time.perf_counter()
+        Jm, gzm, gxm = gradient(ctx, F, g.active_area, dt, steps, every)
+        t_mixed = time.perf_counter() - t0
+        ctx.set_adjoint_precision(_lib.BVE_PRECISION_FP64)
+        mixed = dict(ratio=ratio, gradient_ms=t_mixed * 1e3,
+                     adjoint_fwd_ratio_same_map=t_mixed / t_fwd_active,
+                     J_equals_float64=bool(Jm == J),
+                     grad_zeta_max_rel_diff_vs_float64=float(np.abs(gzm - gz).max() / np.abs(gz).max()),
+                     grad_x_max_rel_diff_vs_float64=float(np.abs(gxm - gx).max() / np.abs(gx).max()))
 
     out = dict(workload=f"rh4_cubed_sphere_L{level}_{steps}steps_dt{dt:g}_checkpoint{every}",
                mode=mode, n_active=g.n_active, J=J, gradient_ms=t_adj * 1e3,
@@ -247,6 +264,8 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
                adjoint_fwd_ratio_active_and_vertices=t_adj / t_fwd_vertices,
                adjoint_fwd_ratio_all_classes=t_adj / t_fwd,
                fd_panels=int(n_fd))
+    if mixed is not None:
+        out["mixed_reverse_sweep"] = mixed
     if n_fd <= 0:
         ctx.close()
         return out
@@ -257,10 +276,10 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
         xT, zT = ctx.adjoint_forward(dt, steps, steps)
         return math.fsum((g.active_area * zT * F.weight(xT)).tolist())
 
-    def report(fd):
-        rel = np.abs(gz[idx] - fd) / np.abs(fd)
+    def report(fd, grad=gz):
+        rel = np.abs(grad[idx] - fd) / np.abs(fd)
         return dict(max_rel_err=float(rel.max()), median_rel_err=float(np.median(rel)),
-                    normwise_rel_err=float(np.abs(gz[idx] - fd).max() / np.abs(fd).max()))
+                    normwise_rel_err=float(np.abs(grad[idx] - fd).max() / np.abs(fd).max()))
 
     # central differences at eps = 1e-6 with each J difference propagated in
     # difference form (bve_difference_forward): no cancellation
@@ -271,6 +290,8 @@ def adjoint_measurement(level, steps=20, every=5, dt=0.01, n_fd=32, mode="fp64")
     t_exact = time.perf_counter() - t0
     fd_report = {"eps_1e-06_difference_form": dict(report(fd_exact),
                                                    seconds=t_exact)}
+    if mixed is not None:
+        mixed["fd_eps_1e-06_difference_form"] = report(fd_exact, gzm)
     # plain quotients of two forward runs, for comparison: at eps = 1e-6 the
     # rounding noise of J (~1e-14 relative) dominates them
     for eps in (1e-6, 1e-3):
@@ -664,6 +685,9 @@ def main():
         # the gradient against a forward of the map it differentiates (the
         # active panels alone, the same 20 steps and arithmetic)
         out["adjoint_fwd_ratio"] = adj["adjoint_fwd_ratio_same_map"]
+        # the mixed reverse sweep's gradient (float64 forward sweep) against
+        # the same float64 forward
+        out["adjoint_fwd_ratio_mixed_reverse_sweep"] = adj["mixed_reverse_sweep"]["adjoint_fwd_ratio_same_map"]
         # the same in the shared-reciprocal float64 mode (its forward passes
         # and recomputation on pair_sum_f64_seg_kernel<..,4>)
         adj["fp64_fast"] = adjoint_measurement(args.adjoint_level, n_fd=0, mode="fp64_fast")

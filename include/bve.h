@@ -263,6 +263,15 @@ int bve_adjoint_forward(bve_ctx *ctx, double dt, int n_steps, int checkpoint_eve
  * NULL) and dJ/dzeta_0 (n_active). */
 int bve_adjoint_backward(bve_ctx *ctx, const double *lambda_x, const double *lambda_zeta,
                          double *grad_x0, double *grad_zeta0);
+/* No reference counterpart.  Arithmetic of the reverse sweep's pair pass
+ * (the forward sweep and its recomputation keep the context's float64
+ * mode): BVE_PRECISION_FP64 (the default) or BVE_PRECISION_MIXED, float32
+ * pair terms wherever the bounding spheres of a 64-panel target group and a
+ * 256-panel source tile are far apart (radii sum < ratio x centre
+ * distance, 0 < ratio < 1; 0.7 is the tested value), float64 elsewhere and
+ * in every carry.  Takes effect at the next bve_adjoint_forward or
+ * bve_adjoint_backward call. */
+int bve_set_adjoint_precision(bve_ctx *ctx, int precision, double ratio);
 
 /* No reference counterpart.  Exact finite differences of the same active-
  * panel RK4 map: from the resident state (x0, zeta0) and a vorticity

@@ -229,6 +229,33 @@ def test_level6_adjoint_workload():
     assert abs(pred - directional) < 1e-9 * abs(directional)
 
 
+def test_level6_mixed_adjoint_workload():
+    """The same workload with the mixed reverse sweep (bve_set_adjoint_precision
+    MIXED, ratio 0.7): dJ/dzeta_0 against exact central differences on the
+    same 8 panels to 1e-6, and against the float64 gradient to 1e-7 of its
+    largest entry (measured 6e-9)."""
+    from thesiscode_amd.adjoint import GaussianWeightFunctional, central_difference, gradient
+    g = gridmod.cubed_sphere(6)
+    z = ic.rh4_wave(g.active_x)
+    F = GaussianWeightFunctional()
+    dt, steps, every = 0.01, 20, 5
+    with _lib.Context(0) as c:
+        c.set_omega(OMEGA)
+        c.load_panels(g.active_x, z, g.active_area, g.passive_x)
+        J64, gz64, _ = gradient(c, F, g.active_area, dt, steps, every)
+        c.set_adjoint_precision(_lib.BVE_PRECISION_MIXED, 0.7)
+        J, gz, _ = gradient(c, F, g.active_area, dt, steps, every)
+        c.set_adjoint_precision(_lib.BVE_PRECISION_FP64)
+        idx = np.random.default_rng(0).choice(g.n_active, 32, replace=False)[:8]
+        fd = np.array([central_difference(c, F, g.active_area, dt, steps, j, 1e-6) for j in idx])
+    rel = np.abs(gz[idx] - fd) / np.abs(fd)
+    e64 = np.abs(gz - gz64).max() / np.abs(gz64).max()
+    print(f"mixed adjoint vs central differences {rel.max():.2e}, vs float64 gradient {e64:.2e}")
+    assert J == J64
+    assert rel.max() < 1e-6
+    assert e64 < 1e-7
+
+
 # the autotuner's mixed-mode choice for these levels (targets per lane,
 # sources per work item), as bench.py measured it
 ENSEMBLE_PLANS = {4: (1, 256), 5: (1, 256), 6: (1, 512)}

@@ -54,6 +54,7 @@ SIGNATURES = {
     "bve_difference_forward": (_i, [_p, _d, _i, _p, _p, _p, _p, _p]),
     "bve_adjoint_forward": (_i, [_p, _d, _i, _i, _p, _p]),
     "bve_adjoint_backward": (_i, [_p, _p, _p, _p, _p]),
+    "bve_set_adjoint_precision": (_i, [_p, _i, _d]),
     "bve_ensemble_create": (_i, [ctypes.POINTER(_p), _i, _i]),
     "bve_ensemble_destroy": (_i, [_p]),
     "bve_ensemble_last_error": (ctypes.c_char_p, [_p]),
@@ -351,6 +352,12 @@ class Context:
         return xT, zT, dxT, dzT
 
     # adjoint
+    def set_adjoint_precision(self, precision, ratio=0.7):
+        """Arithmetic of the adjoint's pair pass: BVE_PRECISION_FP64 or
+        BVE_PRECISION_MIXED (float32 for well-separated group/tile pairs)."""
+        self._check(self._L.bve_set_adjoint_precision(self._h, precision, ratio),
+                    "bve_set_adjoint_precision")
+
     def adjoint_forward(self, dt, n_steps, checkpoint_every):
         """Active-panel forward sweep with checkpoints; returns (x_T, zeta_T)."""
         xT = np.zeros((self.n_active, 3))

@@ -66,7 +66,7 @@ __device__ __forceinline__ void recips(const double (&d)[R], double (&r)[R])
 
 // Source tile at jt (in LDS) against the T targets of this lane; MASK: the
 // tile holds the own index of some lane's target (self pairs are dropped).
-template <int T, int R, bool MASK>
+template <int T, int R, bool MASK, int TILE = 256>
 __device__ __forceinline__ void vjp_tile(const AdjRec *lds, int jt, const int (&self)[T],
                                          const double (&xk)[T], const double (&yk)[T],
                                          const double (&zk)[T], const double (&ax)[T],
@@ -76,7 +76,6 @@ __device__ __forceinline__ void vjp_tile(const AdjRec *lds, int jt, const int (&
                                          double (&Vz)[T], double (&Nx)[T], double (&Ny)[T],
                                          double (&Nz)[T])
 {
-    constexpr int TILE = 256;
     for (int k = 0; k < TILE; k += R) {
         // positions first (denominators and reciprocals of every target),
         // the rest of each record where its pairs are summed
@@ -199,6 +198,211 @@ __global__ __launch_bounds__(kBlock, T == 4 ? 2 : 3) void vjp_kernel(
     });
 }
 
+// ---------------------------------------------------------------------------
+// Mixed pair pass: float32 far field, float64 near field
+// ---------------------------------------------------------------------------
+//
+// A gradient needs far less than float64 accuracy of its pair sums, and the
+// forward trajectory it differentiates stays float64.  A (target tile,
+// 256-source tile) pair runs the float64 tile above when the source tile
+// holds one of the targets' own indices or when its bounding sphere comes
+// close to that of a 256-record part of the target tile (r_g + r_s >=
+// ratio * |c_g - c_s|); every other pair runs in float32 with the
+// tile-relative denominator of the mixed forward pass,
+//
+//     d = (A_k + B_l) - a_k . o_l,   a_k = x_k - c_s,  o_l = x_l - c_s,
+//     A_k = |a_k|^2 / 2 + c_k,       B_l = |o_l|^2 / 2 + c_l,  c = (1 - |x|^2) / 2,
+//
+// which is 1 - x_k . x_l exactly and, for separated spheres, rounds to
+// within about ((1 + ratio) / (1 - ratio))^2 float32 roundings of d
+// (|a| + |o| <= (1 + ratio) |c_g - c_s| against |a - o| >= (1 - ratio)
+// |c_g - c_s|).  Per pair: an add and three FMAs for d, one v_rcp_f32, six
+// FMAs for w_l P_kl + w_k P_lk, two products for gamma and six FMAs into V
+// and N; the float32 tile sums are carried into float64 after each tile.
+
+// The float64 tiles of the mixed pass share one reciprocal among four pairs,
+// as the float64 pass does.
+constexpr int kMixedR = 4;
+
+union AdjMixedLds {
+    AdjRec32 r32[256];
+    AdjRec r64[256];
+};
+
+// Float32 state of one target against one source tile: -a_k, A_k, w_k x_k, n_k.
+struct AdjTgt32 {
+    float nax, nay, naz, A, wx, wy, wz, nx, ny, nz;
+};
+
+struct AdjAcc32 {
+    float Vx, Vy, Vz, Nx, Ny, Nz;
+};
+
+template <int N, int TILE>
+__device__ __forceinline__ void vjp_tile32(const AdjRec32 *lds, const AdjTgt32 (&g)[N],
+                                           AdjAcc32 (&acc)[N])
+{
+#pragma unroll 2
+    for (int k = 0; k < TILE; ++k) {
+        const float4 o = lds[k].o, p = lds[k].x, w = lds[k].w, n = lds[k].n;
+#pragma unroll
+        for (int t = 0; t < N; ++t) {
+            const float d = __builtin_fmaf(
+                g[t].nax, o.x, __builtin_fmaf(g[t].nay, o.y, __builtin_fmaf(g[t].naz, o.z, g[t].A + o.w)));
+            const float r = __builtin_amdgcn_rcpf(d);
+            // (w_l x_l).n_k + (w_k x_k).n_l
+            float sp = g[t].wz * n.z;
+            sp = __builtin_fmaf(g[t].wy, n.y, sp);
+            sp = __builtin_fmaf(g[t].wx, n.x, sp);
+            sp = __builtin_fmaf(w.z, g[t].nz, sp);
+            sp = __builtin_fmaf(w.y, g[t].ny, sp);
+            sp = __builtin_fmaf(w.x, g[t].nx, sp);
+            const float gam = (r * r) * sp;
+            acc[t].Vx = __builtin_fmaf(gam, p.x, acc[t].Vx);
+            acc[t].Vy = __builtin_fmaf(gam, p.y, acc[t].Vy);
+            acc[t].Vz = __builtin_fmaf(gam, p.z, acc[t].Vz);
+            acc[t].Nx = __builtin_fmaf(r, n.x, acc[t].Nx);
+            acc[t].Ny = __builtin_fmaf(r, n.y, acc[t].Ny);
+            acc[t].Nz = __builtin_fmaf(r, n.z, acc[t].Nz);
+        }
+    }
+}
+
+// Float32 state of a target at x (float64) with float32 w x and n against
+// the tile with centre sb.
+__device__ __forceinline__ AdjTgt32 adj_tgt32(double x, double y, double z, const float (&wn)[6],
+                                              const double4 &sb)
+{
+    const float ax = (float)(x - sb.x), ay = (float)(y - sb.y), az = (float)(z - sb.z);
+    const double ck = 0.5 * (1.0 - (x * x + y * y + z * z));
+    const double a2 = (double)ax * ax + (double)ay * ay + (double)az * az;
+    return AdjTgt32{-ax, -ay, -az, (float)(0.5 * a2 + ck), wn[0], wn[1], wn[2], wn[3], wn[4], wn[5]};
+}
+
+// Target tile `ttile` against source tiles [j_begin, j_end) of a mixed pass
+// (rows as vjp_range).  The route is taken per workgroup, from the spheres
+// of the T record tiles its targets fill: a source tile runs in float64
+// when it holds one of the targets' own indices or lies close to one of
+// those spheres, so that the four waves stay in step between the tile
+// barriers.  Kept across tiles: the float64 positions and sums and the
+// float32 w x and n of each target; a float64 tile re-reads the targets'
+// float64 records.
+template <int T>
+__device__ __forceinline__ void vjp_mixed_range(const AdjStage &a,
+                                                const AdjRec *__restrict__ rec, int ttile,
+                                                int j_begin, int j_end, double *out,
+                                                int out_stride, int out_shift, AdjMixedLds &lds)
+{
+    constexpr int TILE = 256;
+    const int tile_base = ttile * (kBlock * T);
+    double xk[T], yk[T], zk[T];
+    double Vx[T], Vy[T], Vz[T], Nx[T], Ny[T], Nz[T];
+    float wn[T][6];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const AdjRec r = rec[tile_base + t * kBlock + threadIdx.x];
+        xk[t] = r.x;
+        yk[t] = r.y;
+        zk[t] = r.z;
+        wn[t][0] = (float)r.wx;
+        wn[t][1] = (float)r.wy;
+        wn[t][2] = (float)r.wz;
+        wn[t][3] = (float)r.nx;
+        wn[t][4] = (float)r.ny;
+        wn[t][5] = (float)r.nz;
+        Vx[t] = Vy[t] = Vz[t] = Nx[t] = Ny[t] = Nz[t] = 0.0;
+    }
+    const int self_hi = min(tile_base + kBlock * T, a.n);
+    for (int jt = j_begin; jt < j_end; jt += TILE) {
+        const double4 sb = a.tile_box[jt / TILE];
+        // workgroup-uniform
+        const bool diag = tile_base < self_hi && tile_base < jt + TILE && jt < self_hi;
+        bool use64 = diag;
+#pragma unroll
+        for (int u = 0; u < T; ++u) {
+            const double4 gb = a.tile_box[tile_base / TILE + u];
+            const double dx = gb.x - sb.x, dy = gb.y - sb.y, dz = gb.z - sb.z;
+            use64 |= !(gb.w + sb.w < a.ratio * sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        __syncthreads();
+        if (use64)
+            lds.r64[threadIdx.x] = rec[jt + threadIdx.x];
+        else
+            lds.r32[threadIdx.x] = a.rec32[jt + threadIdx.x];
+        __syncthreads();
+        if (!use64) {
+            AdjTgt32 g[T];
+            AdjAcc32 acc[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                g[t] = adj_tgt32(xk[t], yk[t], zk[t], wn[t], sb);
+                acc[t] = AdjAcc32{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            }
+            vjp_tile32<T, TILE>(lds.r32, g, acc);
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                Vx[t] += (double)acc[t].Vx;
+                Vy[t] += (double)acc[t].Vy;
+                Vz[t] += (double)acc[t].Vz;
+                Nx[t] += (double)acc[t].Nx;
+                Ny[t] += (double)acc[t].Ny;
+                Nz[t] += (double)acc[t].Nz;
+            }
+            continue;
+        }
+        double ax[T], ay[T], az[T], ex[T], ey[T], ez[T];
+        int self[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int i = tile_base + t * kBlock + threadIdx.x;
+            const AdjRec r = rec[i];
+            ax[t] = r.wx;
+            ay[t] = r.wy;
+            az[t] = r.wz;
+            ex[t] = r.nx;
+            ey[t] = r.ny;
+            ez[t] = r.nz;
+            self[t] = i < a.n ? i : -1;
+        }
+        if (diag)
+            vjp_tile<T, kMixedR, true>(lds.r64, jt, self, xk, yk, zk, ax, ay, az, ex, ey, ez, Vx,
+                                       Vy, Vz, Nx, Ny, Nz);
+        else
+            vjp_tile<T, kMixedR, false>(lds.r64, jt, self, xk, yk, zk, ax, ay, az, ex, ey, ez, Vx,
+                                        Vy, Vz, Nx, Ny, Nz);
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int i = tile_base + t * kBlock + threadIdx.x - out_shift;
+        const double v[kAdjRows] = {Vx[t], Vy[t], Vz[t], Nx[t], Ny[t], Nz[t]};
+#pragma unroll
+        for (int q = 0; q < kAdjRows; ++q) out[(size_t)q * out_stride + i] = v[q];
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock, T == 4 ? 2 : 3) void vjp_mixed_kernel(
+    AdjStage a, const AdjRec *__restrict__ rec, double *__restrict__ partial)
+{
+    constexpr int TT = kBlock * T;
+    __shared__ AdjMixedLds lds;
+    __shared__ int seg;
+    run_segments<256>(a.plan, a.queue, &seg, [&](int t, int j0, int j1, int s) {
+        vjp_mixed_range<T>(a, rec, t, j0, j1, partial + (size_t)(s + t) * kAdjRows * TT, TT,
+                           t * TT, lds);
+    });
+}
+
+template <int T>
+static int vjp_mixed_blocks()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vjp_mixed_kernel<T>, kBlock, 0) !=
+        hipSuccess)
+        return 0;
+    return n;
+}
+
 template <int T, int R>
 static int vjp_blocks()
 {
@@ -208,8 +412,9 @@ static int vjp_blocks()
     return n;
 }
 
-int vjp_blocks_per_cu(int T, int recip)
+int vjp_blocks_per_cu(int T, int recip, bool mixed)
 {
+    if (mixed) return T == 1 ? vjp_mixed_blocks<1>() : (T == 2 ? vjp_mixed_blocks<2>() : vjp_mixed_blocks<4>());
     if (recip == 4) return T == 1 ? vjp_blocks<1, 4>() : (T == 2 ? vjp_blocks<2, 4>() : vjp_blocks<4, 4>());
     return T == 1 ? vjp_blocks<1, 1>() : (T == 2 ? vjp_blocks<2, 1>() : vjp_blocks<4, 1>());
 }
@@ -219,31 +424,50 @@ int vjp_blocks_per_cu(int T, int recip)
 // ---------------------------------------------------------------------------
 
 // Cotangent records of stage s: kbar = b lam + c Ybar_{s+1}, mu = h kbar,
-// m = mu_x - 2 Omega mu_zeta e_z, n = m x x, from the stage's source records.
+// m = mu_x - 2 Omega mu_zeta e_z, n = m x x, from the stage's source records
+// (zero beyond n).  For a mixed pair pass also each 256-record tile's sphere
+// (centre the mean of its finite records below n) and the float32 records
+// relative to the tile centre.
 __global__ __launch_bounds__(256) void adj_prep_kernel(AdjStage a)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.n_pad) return;
-    if (i >= a.n) {
-        a.rec[i] = AdjRec{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        return;
+    __shared__ double sh[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;   // n_pad is a multiple of 256
+    AdjRec r{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < a.n) {
+        const int st = a.stride;
+        const Source y = a.ys[i];
+        // the last stage has no successor: its Ybar input is not read
+        const bool next = a.c != 0.0;
+        const double kx = a.b * a.lam[i] + (next ? a.c * a.ybar[i] : 0.0);
+        const double ky = a.b * a.lam[st + i] + (next ? a.c * a.ybar[st + i] : 0.0);
+        const double kz = a.b * a.lam[2 * st + i] + (next ? a.c * a.ybar[2 * st + i] : 0.0);
+        const double kzeta = a.b * a.lam[3 * st + i] + (next ? a.c * a.ybar[3 * st + i] : 0.0);
+        const double mx = a.h * kx, my = a.h * ky;
+        const double mz = a.h * kz - 2.0 * a.omega * (a.h * kzeta);
+        a.m[i] = mx;
+        a.m[st + i] = my;
+        a.m[2 * st + i] = mz;
+        // n = m x x
+        r = AdjRec{y.x, y.y, y.z, 0.0, y.w * y.x, y.w * y.y, y.w * y.z, my * y.z - mz * y.y,
+                   mz * y.x - mx * y.z, mx * y.y - my * y.x};
     }
-    const int st = a.stride;
-    const Source y = a.ys[i];
-    // the last stage has no successor: its Ybar input is not read
-    const bool next = a.c != 0.0;
-    const double kx = a.b * a.lam[i] + (next ? a.c * a.ybar[i] : 0.0);
-    const double ky = a.b * a.lam[st + i] + (next ? a.c * a.ybar[st + i] : 0.0);
-    const double kz = a.b * a.lam[2 * st + i] + (next ? a.c * a.ybar[2 * st + i] : 0.0);
-    const double kzeta = a.b * a.lam[3 * st + i] + (next ? a.c * a.ybar[3 * st + i] : 0.0);
-    const double mx = a.h * kx, my = a.h * ky;
-    const double mz = a.h * kz - 2.0 * a.omega * (a.h * kzeta);
-    a.m[i] = mx;
-    a.m[st + i] = my;
-    a.m[2 * st + i] = mz;
-    // n = m x x
-    a.rec[i] = AdjRec{y.x, y.y, y.z, 0.0, y.w * y.x, y.w * y.y, y.w * y.z, my * y.z - mz * y.y,
-                      mz * y.x - mx * y.z, mx * y.y - my * y.x};
+    a.rec[i] = r;
+    if (!a.rec32) return;   // uniform over the launch
+    const bool valid = i < a.n && finite3(r.x, r.y, r.z);
+    double c[4] = {valid ? 1.0 : 0.0, valid ? r.x : 0.0, valid ? r.y : 0.0, valid ? r.z : 0.0};
+    block_sums<4>(c, sh);
+    const double inv = c[0] > 0 ? 1.0 / c[0] : 0.0;
+    const double cx = c[1] * inv, cy = c[2] * inv, cz = c[3] * inv;
+    const double ox = r.x - cx, oy = r.y - cy, oz = r.z - cz;
+    const double rad = block_max(valid ? sqrt(ox * ox + oy * oy + oz * oz) : -1.0, sh);
+    if (threadIdx.x == 0) a.tile_box[blockIdx.x] = make_double4(cx, cy, cz, rad);
+    const float fx = (float)ox, fy = (float)oy, fz = (float)oz;
+    const double cl = 0.5 * (1.0 - (r.x * r.x + r.y * r.y + r.z * r.z));
+    const double o2 = (double)fx * fx + (double)fy * fy + (double)fz * fz;
+    a.rec32[i] = AdjRec32{make_float4(fx, fy, fz, (float)(0.5 * o2 + cl)),
+                          make_float4((float)r.x, (float)r.y, (float)r.z, 0.f),
+                          make_float4((float)r.wx, (float)r.wy, (float)r.wz, 0.f),
+                          make_float4((float)r.nx, (float)r.ny, (float)r.nz, 0.f)};
 }
 
 // Ybar of stage s from the pair pass's segment rows and the forward sums S
@@ -299,6 +523,11 @@ static hipError_t launch_vjp2(const AdjStage &a, hipStream_t s)
 template <int T>
 static hipError_t launch_vjp_t(const AdjStage &a, hipStream_t s)
 {
+    if (a.rec32) {
+        hipLaunchKernelGGL((vjp_mixed_kernel<T>), dim3(a.plan.grid), dim3(kBlock), 0, s, a,
+                           (const AdjRec *)a.rec, a.partial);
+        return hipGetLastError();
+    }
     if (a.plan.recip == 4) return launch_vjp2<T, 4>(a, s);
     return launch_vjp2<T, 1>(a, s);
 }
@@ -307,6 +536,8 @@ hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s)
 {
     if (a.plan.tile != 256 || a.n_pad % 256 != 0 || a.stride < a.n_pad || a.plan.segments <= 0 ||
         !a.queue || !a.sums || a.plan.n_tiles * kBlock * a.plan.targets_per_lane > a.n_pad)
+        return hipErrorInvalidValue;
+    if (a.rec32 && (!a.tile_box || !(a.ratio > 0.0 && a.ratio < 1.0)))
         return hipErrorInvalidValue;
     const int blocks = ceil_div(a.n_pad, 256);
     hipLaunchKernelGGL(adj_prep_kernel, dim3(blocks), dim3(256), 0, s, a);

@@ -182,6 +182,12 @@ struct bve_ctx {
     AdjRec *adj_rec = nullptr;
     double *adj_pt = nullptr;   // partial rows of the adjoint's pair pass
     size_t adj_pt_cap = 0, adj_ckpt_cap = 0, adj_stage_cap = 0, adj_sums_cap = 0;
+    // mixed pair pass of the adjoint (bve_set_adjoint_precision)
+    bool adj_mixed = false;
+    double adj_ratio = 0.7;
+    AdjRec32 *adj_rec32 = nullptr;
+    double4 *adj_tbox = nullptr;
+    size_t adj_rec32_cap = 0, adj_tbox_cap = 0;
 
     // profiling
     bool profiling = false;
@@ -759,7 +765,10 @@ void free_adjoint(bve_ctx *c)
     dev_free(c->adj_m);
     dev_free(c->adj_rec);
     dev_free(c->adj_pt);
+    dev_free(c->adj_rec32);
+    dev_free(c->adj_tbox);
     c->adj_pt_cap = c->adj_ckpt_cap = c->adj_stage_cap = c->adj_sums_cap = 0;
+    c->adj_rec32_cap = c->adj_tbox_cap = 0;
     c->adj_ready = false;
     c->adj_last_stored = false;
     c->adj_n = c->adj_pad = 0;
@@ -795,25 +804,51 @@ int copy_active_state(bve_ctx *c, double *rows, bool save)
     return BVE_OK;
 }
 
-// Plans of the adjoint: the active-only forward pass and the VJP pass.
-int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp)
+// Plans of the adjoint: the active-only forward pass, the reverse sweep's
+// recomputation pass and the VJP pass.  With the mixed adjoint the
+// recomputation shares one float64 reciprocal among four pairs (the
+// fp64_fast pass, ~1e-13 relative): it only rebuilds the stage inputs and
+// sums the reverse sweep linearises about, while the forward sweep (the
+// functional and the checkpoints) keeps the context's arithmetic.
+int adjoint_plans(bve_ctx *c, PairPlan &fwd, PairPlan &vjp, PairPlan *recompute = nullptr)
 {
     fwd = plan_for(c, c->n_active, c->n_src_pad);
     fwd.crossed = 0;   // the rows hold S, which the reverse sweep reuses
     int rc = ensure_partial(c, partial_doubles(fwd, c->t_stride));
     if (rc) return rc;
+    if (recompute) {
+        *recompute = fwd;
+        if (c->adj_mixed && c->precision == BVE_PRECISION_FP64) {
+            c->precision = BVE_PRECISION_FP64_FAST;
+            *recompute = plan_for(c, c->n_active, c->n_src_pad);
+            c->precision = BVE_PRECISION_FP64;
+            recompute->crossed = 0;
+            rc = ensure_partial(c, partial_doubles(*recompute, c->t_stride));
+            if (rc) return rc;
+        }
+    }
     const int T = c->plan_t ? c->plan_t : choose_targets_per_lane(c->n_active);
     // The adjoint's pair pass shares one float64 reciprocal (refined once)
     // among four pairs in both float64 modes: ~1e-13 relative, far inside
     // what a gradient needs, and 4-6% off the gradient at level 6.  It runs
     // the forward pass's segment queue with 2 segments per resident
     // workgroup: its items are few and alike (level 6: 4,608 of them for
-    // 768 slots), and every segment writes 10 partial rows per tile.
-    const int slots = vjp_blocks_per_cu(T, 4) * c->n_cu;
-    const int segments = c->plan_wg ? c->plan_wg : 2 * (slots > 0 ? slots : 2 * c->n_cu);
+    // 768 slots), and every segment writes 6 partial rows per target.  The
+    // mixed pass takes 4: its float64 tiles (near the diagonal) cost about
+    // twice a float32 one, and shorter segments spread them (level 6
+    // gradient: 71.5 ms at 2, 73.2 at 1, 69.0-69.2 at 4, 6 or 8).
+    const int slots = vjp_blocks_per_cu(T, 4, c->adj_mixed) * c->n_cu;
+    const int segments = c->plan_wg ? c->plan_wg
+                                    : (c->adj_mixed ? 4 : 2) * (slots > 0 ? slots : 2 * c->n_cu);
     vjp = make_segmented_plan(c->n_active, c->adj_pad, T, 256, segments, 4, slots);
-    return grow(c, c->adj_pt, c->adj_pt_cap,
-                (size_t)(vjp.segments + vjp.n_tiles) * kAdjRows * kBlock * T, false);
+    rc = grow(c, c->adj_pt, c->adj_pt_cap,
+              (size_t)(vjp.segments + vjp.n_tiles) * kAdjRows * kBlock * T, false);
+    if (!rc && c->adj_mixed) {
+        const size_t P = (size_t)c->adj_pad;
+        rc = grow(c, c->adj_rec32, c->adj_rec32_cap, P, true);
+        if (!rc) rc = grow(c, c->adj_tbox, c->adj_tbox_cap, P / 256, true);
+    }
+    return rc;
 }
 
 // One reverse RK4 step: lam <- lam + sum_s Ybar_s with the stage inputs in
@@ -846,6 +881,11 @@ int adjoint_reverse_step(bve_ctx *c, const Source *stages, const double *sums, c
         a.plan = vjp;
         a.partial = c->adj_pt;
         a.queue = c->queue;
+        if (c->adj_mixed) {
+            a.rec32 = c->adj_rec32;
+            a.tile_box = c->adj_tbox;
+            a.ratio = c->adj_ratio;
+        }
         HIP_TRY(c, launch_adjoint_stage(a, c->stream));
     }
     std::swap(c->adj_lam, c->adj_lam_acc);
@@ -1186,6 +1226,19 @@ int bve_set_error_target(bve_ctx *c, double rel_err)
     if (!(rel_err >= 0.0) || !(rel_err < 1.0))
         return c->fail(BVE_ERR_ARGUMENT, "bve_set_error_target: target must be in [0, 1)");
     c->err_target = rel_err;
+    return BVE_OK;
+}
+
+int bve_set_adjoint_precision(bve_ctx *c, int precision, double ratio)
+{
+    if (!c) return BVE_ERR_ARGUMENT;
+    if (precision != BVE_PRECISION_FP64 && precision != BVE_PRECISION_MIXED)
+        return c->fail(BVE_ERR_ARGUMENT,
+                       "bve_set_adjoint_precision: precision must be BVE_PRECISION_FP64 or BVE_PRECISION_MIXED");
+    if (precision == BVE_PRECISION_MIXED && !(ratio > 0.0 && ratio < 1.0))
+        return c->fail(BVE_ERR_ARGUMENT, "bve_set_adjoint_precision: ratio must be in (0, 1)");
+    c->adj_mixed = precision == BVE_PRECISION_MIXED;
+    if (c->adj_mixed) c->adj_ratio = ratio;
     return BVE_OK;
 }
 
@@ -1888,16 +1941,16 @@ int bve_adjoint_backward(bve_ctx *c, const double *lambda_x, const double *lambd
     HIP_TRY(c, launch_aos_to_soa(c->staging, n, c->adj_lam, c->adj_pad, 0, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->adj_lam + 3 * P, lambda_zeta, sizeof(double) * n,
                               hipMemcpyHostToDevice, c->stream));
-    PairPlan fwd, vjp;
-    rc = adjoint_plans(c, fwd, vjp);
+    PairPlan fwd, vjp, recompute;
+    rc = adjoint_plans(c, fwd, vjp, &recompute);
     if (rc) return rc;
     for (int seg = c->adj_ckpts - 1; seg >= 0; --seg) {
         const int n0 = seg * c->adj_every;
         const int len = std::min(c->adj_every, c->adj_steps - n0);
         if (!(seg == c->adj_ckpts - 1 && c->adj_last_stored)) {
             rc = copy_active_state(c, c->adj_ckpt + (size_t)seg * 4 * P, false);
-            if (!rc) rc = run_steps_on(c, c->adj_dt, len, c->n_active, fwd, c->adj_stages, P, false,
-                                       0.0, c->adj_sums);
+            if (!rc) rc = run_steps_on(c, c->adj_dt, len, c->n_active, recompute, c->adj_stages, P,
+                                       false, 0.0, c->adj_sums);
             if (rc) return rc;
         }
         c->adj_last_stored = false;   // the buffer now goes to earlier segments

@@ -62,6 +62,14 @@ struct __align__(16) AdjRec {
 // V (3), N (3).
 constexpr int kAdjRows = 6;
 
+// Float32 record of one active panel for the mixed adjoint pair pass, taken
+// relative to the centre c of its 256-record tile: o = {fl32(x - c), B} with
+// B = |o|^2 / 2 + (1 - |x|^2) / 2 (float64 from the rounded o, rounded
+// once), x = {fl32(x), 0}, w = {fl32(w x), 0}, n = {fl32(n), 0}; 64 B.
+struct __align__(16) AdjRec32 {
+    float4 o, x, w, n;
+};
+
 // One reverse RK4 stage of the adjoint on n active panels.  Row arrays have
 // `stride` entries per row (rows x, y, z, zeta); source and record arrays
 // have n_pad (a multiple of 1024) entries, zero beyond n.
@@ -81,11 +89,19 @@ struct AdjStage {
     PairPlan plan;         // segmented plan of the pair pass over n_pad records
     double *partial;       // pair pass: kAdjRows rows per (segment + tile)
     unsigned *queue;       // segment queue (two counters, zero between launches)
+    // Mixed pair pass when rec32 is set: its float32 records and the
+    // spheres of the n_pad / 256 record tiles.  A (target tile, source
+    // tile) pair runs in float32 when, for each 256-record part of the
+    // target tile, radii sum < ratio * centre distance.
+    AdjRec32 *rec32;
+    double4 *tile_box;
+    double ratio;
 };
 
 hipError_t launch_adjoint_stage(const AdjStage &a, hipStream_t s);
-// Workgroups of the adjoint pair kernel one CU holds at once.
-int vjp_blocks_per_cu(int targets_per_lane, int recip);
+// Workgroups of the adjoint pair kernel one CU holds at once (mixed: the
+// float32/float64 kernel).
+int vjp_blocks_per_cu(int targets_per_lane, int recip, bool mixed = false);
 
 // Device state of a panel RK4 step.  Targets are [active | passive | particles]
 // in SoA rows of length t_stride.

@@ -123,11 +123,12 @@ __global__ __launch_bounds__(256) void bounded_probe_kernel(const PairArgs b, in
 // 4 waves per SIMD (at most 128 VGPRs): the remainder loop needs the
 // latency hiding more than the registers
 template <int T>
-__global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(MixedArgs a)
+__global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(
+    MixedArgs a, const float4 *__restrict__ g32, const float4 *__restrict__ gw)
 {
     __shared__ BoundedLds lds;
     __shared__ int seg;
-    pair_bounded_segments<T>(a, lds, &seg);
+    pair_bounded_segments<T>(a, lds, &seg, g32, gw);
 }
 
 template <int T>
@@ -136,7 +137,7 @@ static hipError_t launch_bounded_t(const MixedArgs &a, hipStream_t s)
     if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T>), dim3(a.base.plan.grid), dim3(kBlock), 0,
-                       s, a);
+                       s, a, (const float4 *)a.src32, (const float4 *)a.srcw);
     return hipGetLastError();
 }
 

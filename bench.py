@@ -53,16 +53,24 @@ MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4
 # copy_probe), for the default plans on the level-7 Gaussian vortex
 # (profiles/pair_sum_bounded_L7_counters.md, pair_sum_f64_fast_L7_traffic_pmc.md)
 MEASURED_TRAFFIC = {
-    ("pair_sum_bounded_seg_kernel<2>", 7): 1.34e9,
+    ("pair_sum_bounded_seg_kernel<2,true>", 7): 1.34e9,
     ("pair_sum_f64_seg_kernel<4,512,4>", 7): 1.99e8,
 }
 
 
-def kernel_name(mode, T, tile, recip, segments):
+# Sources up to which the bounded pass reads its float32 records through
+# scalar loads (kBoundedScalarMaxSources), and the source padding.
+BOUNDED_SCALAR_MAX_SOURCES = 131072
+SOURCE_ALIGN = 512
+
+
+def kernel_name(mode, T, tile, recip, segments, n_sources=0):
     """Name of the pair kernel instantiation a mode's launch runs."""
     if mode == "mixed_bounded":
-        kind = "pair_sum_bounded_seg_kernel" if segments > 0 else "pair_sum_bounded_kernel"
-        return f"{kind}<{T}>"
+        if segments <= 0:
+            return f"pair_sum_bounded_kernel<{T}>"
+        n_pad = -(-n_sources // SOURCE_ALIGN) * SOURCE_ALIGN
+        return f"pair_sum_bounded_seg_kernel<{T},{str(n_pad <= BOUNDED_SCALAR_MAX_SOURCES).lower()}>"
     if mode.startswith("mixed"):
         kind = "pair_sum_mixed_seg_kernel" if segments > 0 else "pair_sum_mixed_kernel"
         return f"{kind}<{T},{str(mode == 'mixed_kahan').lower()}>"
@@ -436,7 +444,7 @@ def measure(ctx, reload, mode, args, pairs_primary, pairs_all, sample):
     # and bounded passes also fetch their source records again from HBM
     # (6.3 MB of them per pass exceed an XCD's L2), which the model leaves out.
     traffic_model = 2.0 * launch["partial_bytes"] + 32.0 * ctx.n_active
-    kernel = kernel_name(mode, T, tile, recip, launch["segments"])
+    kernel = kernel_name(mode, T, tile, recip, launch["segments"], ctx.n_active)
     # accuracy of this mode vs the oracle: the resident state's velocities
     # from the same plan and kernel as the timed launches
     reload()

@@ -87,7 +87,7 @@ def test_level7_gauss_within_target(chunk):
     with bounded_context(g, za, absv) as c:
         c.set_chunk(chunk)
         assert c.launch()["segments"] > 0
-        # two targets per lane: pair_sum_bounded_seg_kernel<2>, the kernel
+        # two targets per lane: pair_sum_bounded_seg_kernel<2,true>, the kernel
         # bench.py times
         assert c.plan()[0] == 2
         if chunk:

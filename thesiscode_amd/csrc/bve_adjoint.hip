@@ -66,7 +66,7 @@ __device__ __forceinline__ void recips(const double (&d)[R], double (&r)[R])
 
 // Source tile at jt (in LDS) against the T targets of this lane; MASK: the
 // tile holds the own index of some lane's target (self pairs are dropped).
-template <int T, int R, bool MASK, int TILE = 256>
+template <int T, int R, bool MASK>
 __device__ __forceinline__ void vjp_tile(const AdjRec *lds, int jt, const int (&self)[T],
                                          const double (&xk)[T], const double (&yk)[T],
                                          const double (&zk)[T], const double (&ax)[T],
@@ -76,6 +76,7 @@ __device__ __forceinline__ void vjp_tile(const AdjRec *lds, int jt, const int (&
                                          double (&Vz)[T], double (&Nx)[T], double (&Ny)[T],
                                          double (&Nz)[T])
 {
+    constexpr int TILE = 256;
     for (int k = 0; k < TILE; k += R) {
         // positions first (denominators and reciprocals of every target),
         // the rest of each record where its pairs are summed
@@ -238,12 +239,13 @@ struct AdjAcc32 {
     float Vx, Vy, Vz, Nx, Ny, Nz;
 };
 
-template <int N, int TILE>
+// Float32 sums of the 256-record tile in LDS for N targets.
+template <int N>
 __device__ __forceinline__ void vjp_tile32(const AdjRec32 *lds, const AdjTgt32 (&g)[N],
                                            AdjAcc32 (&acc)[N])
 {
 #pragma unroll 2
-    for (int k = 0; k < TILE; ++k) {
+    for (int k = 0; k < 256; ++k) {
         const float4 o = lds[k].o, p = lds[k].x, w = lds[k].w, n = lds[k].n;
 #pragma unroll
         for (int t = 0; t < N; ++t) {
@@ -286,7 +288,10 @@ __device__ __forceinline__ AdjTgt32 adj_tgt32(double x, double y, double z, cons
 // those spheres, so that the four waves stay in step between the tile
 // barriers.  Kept across tiles: the float64 positions and sums and the
 // float32 w x and n of each target; a float64 tile re-reads the targets'
-// float64 records.
+// float64 records.  (Routes per wave, with per-wave 64- or 128-target
+// spheres, sent fewer pairs to float64 but held three waves at each tile
+// barrier behind a fourth on a float64 tile: 400-425 against 336 us per
+// level-6 pass.)
 template <int T>
 __device__ __forceinline__ void vjp_mixed_range(const AdjStage &a,
                                                 const AdjRec *__restrict__ rec, int ttile,
@@ -338,7 +343,7 @@ __device__ __forceinline__ void vjp_mixed_range(const AdjStage &a,
                 g[t] = adj_tgt32(xk[t], yk[t], zk[t], wn[t], sb);
                 acc[t] = AdjAcc32{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             }
-            vjp_tile32<T, TILE>(lds.r32, g, acc);
+            vjp_tile32<T>(lds.r32, g, acc);
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 Vx[t] += (double)acc[t].Vx;

@@ -374,7 +374,7 @@ PairPlan plan_for(const bve_ctx *c, int n_targets, int n_src_pad)
         // error-bound granularity), stride-ordered segment queue, or
         // interleaved chunks when a chunk is set; rows hold x_i x S sums
         const int T = c->plan_t ? c->plan_t : (n_targets >= 131072 ? 2 : 1);
-        const int slots = pair_bounded_blocks_per_cu(T) * c->n_cu;
+        const int slots = pair_bounded_blocks_per_cu(T, n_src_pad) * c->n_cu;
         PairPlan p;
         if (c->plan_chunk > 0) {
             p = make_len_plan(n_targets, n_src_pad, T, 256, round_up(c->plan_chunk, 256) / 256, 1,

@@ -689,15 +689,22 @@ constexpr float kBoundedTmax = 0.9f;
 constexpr double kBoundedEps32 = 5.9604644775390625e-08;   // 2^-24
 
 struct BoundedLds {
+    float4 pos[kMixedTile];         // o_j (.w unused); passes without SCALAR
+    float4 wgt[kMixedTile];         // w_j x_j (.w unused); passes without SCALAR
     StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
 };
 
+// Sources up to which a bounded pass reads its float32 records through
+// scalar loads: each wave then fetches every tile itself, which pays while
+// the records (32 B per source) stay in an XCD's 4 MB L2.  Level 7 (98,304
+// sources): 5.46 against 5.66 ms per launch with the records staged in LDS;
+// level 8 (393,216): 87.6 against 84.7 ms.
+constexpr int kBoundedScalarMaxSources = 131072;
+
 // Remainder sums R of a source tile's float32 records (pos = {o, 0},
-// wgt = {w x, 0}) for N targets.  The records are read straight from
-// global memory at wave-uniform addresses, so they arrive through scalar
-// loads into SGPRs: no LDS staging, no workgroup barrier for a tile that no
-// target group routes to float64, and 3.8% off the level-7 launch against
-// LDS broadcasts (5.46 against 5.67 ms).
+// wgt = {w x, 0}) for N targets, from LDS or (SCALAR passes) straight from
+// global memory at wave-uniform addresses, where they arrive through scalar
+// loads into SGPRs.
 template <int N>
 __device__ __forceinline__ void remainder_tile(const float4 *__restrict__ pos,
                                                const float4 *__restrict__ wgt, const float *xf,
@@ -766,7 +773,7 @@ __device__ __forceinline__ void bounded_combine(const double *__restrict__ mom, 
 // Source tiles (first + k step) mod n_src_tiles, k in [k0, k1), for target
 // tile `ttile` of a bounded pass: the sums x_i x S_i of target i go to
 // out[i - out_off] and the rows out_stride, 2 out_stride after it.
-template <int T>
+template <int T, bool SCALAR>
 __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile, int first,
                                                    int step, int k0, int k1, double *out,
                                                    int out_stride, int out_off, BoundedLds &lds,
@@ -775,7 +782,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
 {
     // g32, gw: the float32 records a.src32, a.srcw as the kernel's
     // no-alias arguments (read-only here, so the compiler may use scalar
-    // loads for them)
+    // loads for them); read from LDS instead when not SCALAR
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const TargetClasses cls = a.base.cls;
@@ -805,6 +812,13 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
     for (int k = k0; k < k1; ++k) {
         const int tl = (int)(((long long)k * step + first) % nst);
         const int jt = tl * kMixedTile;
+        if (!SCALAR) {
+            __syncthreads();
+            lds.pos[threadIdx.x] = g32[jt + threadIdx.x];
+            lds.wgt[threadIdx.x] = gw[jt + threadIdx.x];
+        }
+        const float4 *pos = SCALAR ? g32 + jt : lds.pos;
+        const float4 *wgt = SCALAR ? gw + jt : lds.wgt;
         const double4 sb = a.tile_box[tl];
         const double *mom = a.tile_mom + (size_t)tl * kTileMoments;
         const double W = mom[10];
@@ -843,7 +857,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                 Ef[t] = (float)E[t];
                 Rx[t] = Ry[t] = Rz[t] = 0.f;
             }
-            remainder_tile<T>(g32 + jt, gw + jt, xf, yf, zf, Ef, Rx, Ry, Rz);
+            remainder_tile<T>(pos, wgt, xf, yf, zf, Ef, Rx, Ry, Rz);
 #pragma unroll
             for (int t = 0; t < T; ++t)
                 bounded_combine(mom, sb, xi[t], yi[t], zi[t], E[t], Rx[t], Ry[t], Rz[t],
@@ -892,7 +906,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                 } else {
                     const double E = c0_of(t) - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
                     float Ef[1] = {(float)E}, Rx[1] = {0.f}, Ry[1] = {0.f}, Rz[1] = {0.f};
-                    remainder_tile<1>(g32 + jt, gw + jt, &xf[t], &yf[t], &zf[t], Ef, Rx, Ry, Rz);
+                    remainder_tile<1>(pos, wgt, &xf[t], &yf[t], &zf[t], Ef, Rx, Ry, Rz);
                     bounded_combine(mom, sb, xi[t], yi[t], zi[t], E, Rx[0], Ry[0], Rz[0],
                                     ux[t], uy[t], uz[t]);
                 }
@@ -916,7 +930,7 @@ __device__ __forceinline__ double bounded_tau(const MixedArgs &a)
 }
 
 // Persistent body of a segmented bounded pass (source tiles in stride order).
-template <int T>
+template <int T, bool SCALAR>
 __device__ __forceinline__ void pair_bounded_segments(const MixedArgs &a, BoundedLds &lds,
                                                       int *seg_slot,
                                                       const float4 *__restrict__ g32,
@@ -926,7 +940,7 @@ __device__ __forceinline__ void pair_bounded_segments(const MixedArgs &a, Bounde
     const double tau = bounded_tau(a);
     run_segments<kMixedTile>(a.base.plan, a.base.queue, seg_slot,
                              [&](int t, int j0, int j1, int seg) {
-        pair_bounded_range<T>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile, j1 / kMixedTile,
+        pair_bounded_range<T, SCALAR>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile, j1 / kMixedTile,
                               a.base.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds, tau, g32, gw);
     });
 }

@@ -168,7 +168,9 @@ hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe = t
 // Clears the velocity-scale word (before a stage update folds max |u| into it).
 hipError_t launch_scale_reset(double *scale, hipStream_t s);
 hipError_t launch_pair_sum_bounded(const MixedArgs &a, hipStream_t s);
-int pair_bounded_blocks_per_cu(int targets_per_lane);
+// Workgroups per CU of the bounded pass over n_src_pad sources (its kernel
+// depends on the source count).
+int pair_bounded_blocks_per_cu(int targets_per_lane, int n_src_pad);
 // Reference-order pass (a.plan.exact): bitwise the reference's sums.
 hipError_t launch_pair_sum_exact(const PairArgs &a, hipStream_t s);
 hipError_t launch_stage_update(const StepState &st, const StageParams &sp, hipStream_t s);

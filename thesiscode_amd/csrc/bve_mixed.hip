@@ -122,41 +122,49 @@ __global__ __launch_bounds__(256) void bounded_probe_kernel(const PairArgs b, in
 
 // 4 waves per SIMD (at most 128 VGPRs): the remainder loop needs the
 // latency hiding more than the registers
-template <int T>
+template <int T, bool SCALAR>
 __global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(
     MixedArgs a, const float4 *__restrict__ g32, const float4 *__restrict__ gw)
 {
     __shared__ BoundedLds lds;
     __shared__ int seg;
-    pair_bounded_segments<T>(a, lds, &seg, g32, gw);
+    pair_bounded_segments<T, SCALAR>(a, lds, &seg, g32, gw);
+}
+
+template <int T, bool SCALAR>
+static hipError_t launch_bounded_ts(const MixedArgs &a, hipStream_t s)
+{
+    if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T, SCALAR>), dim3(a.base.plan.grid),
+                       dim3(kBlock), 0, s, a, (const float4 *)a.src32, (const float4 *)a.srcw);
+    return hipGetLastError();
 }
 
 template <int T>
 static hipError_t launch_bounded_t(const MixedArgs &a, hipStream_t s)
 {
-    if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
-        return hipErrorInvalidValue;
-    hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T>), dim3(a.base.plan.grid), dim3(kBlock), 0,
-                       s, a, (const float4 *)a.src32, (const float4 *)a.srcw);
-    return hipGetLastError();
+    return a.base.n_src_pad <= kBoundedScalarMaxSources ? launch_bounded_ts<T, true>(a, s)
+                                                        : launch_bounded_ts<T, false>(a, s);
 }
 
-template <int T>
+template <int T, bool SCALAR>
 static int bounded_seg_blocks()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_sum_bounded_seg_kernel<T>, kBlock,
-                                                     0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pair_sum_bounded_seg_kernel<T, SCALAR>,
+                                                     kBlock, 0) != hipSuccess)
         return 0;
     return n;
 }
 
-int pair_bounded_blocks_per_cu(int T)
+int pair_bounded_blocks_per_cu(int T, int n_src_pad)
 {
+    const bool scalar = n_src_pad <= kBoundedScalarMaxSources;
     switch (T) {
-    case 1: return bounded_seg_blocks<1>();
-    case 2: return bounded_seg_blocks<2>();
-    case 4: return bounded_seg_blocks<4>();
+    case 1: return scalar ? bounded_seg_blocks<1, true>() : bounded_seg_blocks<1, false>();
+    case 2: return scalar ? bounded_seg_blocks<2, true>() : bounded_seg_blocks<2, false>();
+    case 4: return scalar ? bounded_seg_blocks<4, true>() : bounded_seg_blocks<4, false>();
     default: return 0;
     }
 }

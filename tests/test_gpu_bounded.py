@@ -112,6 +112,27 @@ def test_level8_two_vortices_within_target():
     assert err < TARGET
 
 
+@pytest.mark.parametrize("T", [1, 4])
+def test_level8_staged_records_every_plan(T):
+    """Above 131,072 sources the bounded pass stages its float32 records in
+    LDS instead of reading them through scalar loads (a second kernel
+    instantiation): the level-8 two-vortex velocities on the 1- and
+    4-targets-per-lane plans (the default test covers 2) meet the target
+    against the oracle too."""
+    g = gridmod.cubed_sphere(8)
+    za, zp, _ = ic.two_vortices(g)
+    absv = zp + 2 * OMEGA * g.particles_x[:, 2]
+    lat, lon = np.deg2rad(20.0), np.deg2rad(10.0)
+    with bounded_context(g, za, absv) as c:
+        c.set_plan(T)
+        assert c.plan()[0] == T
+        u = c.resident_velocity()
+    err = sampled_errors(g, za, u, [ic.latlon_to_xyz(lat, lon), ic.latlon_to_xyz(lat, -lon)],
+                         n=32)
+    print(f"level 8 bounded, {T} targets per lane: {err:.3e}")
+    assert err < TARGET
+
+
 def test_level8_step_within_target():
     """config[4]: one RK4 step of the level-8 two-vortex state at dt = 0.002
     in the bounded mode against the float64 step (itself held to the oracle

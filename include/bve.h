@@ -269,8 +269,10 @@ int bve_adjoint_backward(bve_ctx *ctx, const double *lambda_x, const double *lam
  * pair terms wherever the bounding spheres of a 64-panel target group and a
  * 256-panel source tile are far apart (radii sum < ratio x centre
  * distance, 0 < ratio < 1; 0.7 is the tested value), float64 elsewhere and
- * in every carry.  Takes effect at the next bve_adjoint_forward or
- * bve_adjoint_backward call. */
+ * in every carry; the recomputation of checkpoint segments then shares one
+ * float64 reciprocal among four pairs.  The float32 terms bound the
+ * cotangents and sums to float32's range (~3e38).  Takes effect at the next
+ * bve_adjoint_forward or bve_adjoint_backward call. */
 int bve_set_adjoint_precision(bve_ctx *ctx, int precision, double ratio);
 
 /* No reference counterpart.  Exact finite differences of the same active-

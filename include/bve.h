@@ -271,7 +271,10 @@ int bve_adjoint_backward(bve_ctx *ctx, const double *lambda_x, const double *lam
  * distance, 0 < ratio < 1; 0.7 is the tested value), float64 elsewhere and
  * in every carry; the recomputation of checkpoint segments then shares one
  * float64 reciprocal among four pairs.  The float32 terms bound the
- * cotangents and sums to float32's range (~3e38).  Takes effect at the next
+ * cotangents and sums to float32's range (~3e38).  A repeated reverse sweep
+ * after one bve_adjoint_forward recomputes the last segment, which the
+ * first took from the forward sweep, so in this mode the two agree to
+ * rounding (~1e-13), not bit for bit.  Takes effect at the next
  * bve_adjoint_forward or bve_adjoint_backward call. */
 int bve_set_adjoint_precision(bve_ctx *ctx, int precision, double ratio);
 

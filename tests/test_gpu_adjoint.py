@@ -215,6 +215,27 @@ def test_mixed_adjoint_is_deterministic(ctx):
     assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
+def test_mixed_adjoint_repeated_reverse_sweep(ctx):
+    """A second mixed reverse sweep after the same forward sweep recomputes
+    the last checkpoint segment (the first used the forward sweep's own
+    float64 stage inputs) with the shared-reciprocal pass: it agrees with
+    the first to 1e-10 of the largest entry, not bit for bit."""
+    g, z = load(ctx, 5)
+    F = GaussianWeightFunctional()
+    ctx.set_adjoint_precision(_lib.BVE_PRECISION_MIXED)
+    try:
+        xT, zT = ctx.adjoint_forward(0.01, 6, 2)
+        lam_x, lam_zeta = F.cotangent(xT, zT, g.active_area)
+        gx1, gz1 = ctx.adjoint_backward(lam_x, lam_zeta)
+        gx2, gz2 = ctx.adjoint_backward(lam_x, lam_zeta)
+    finally:
+        ctx.set_adjoint_precision(_lib.BVE_PRECISION_FP64)
+    ez = np.abs(gz2 - gz1).max() / np.abs(gz1).max()
+    ex = np.abs(gx2 - gx1).max() / np.abs(gx1).max()
+    print(f"repeated mixed reverse sweep: zeta {ez:.2e}, x {ex:.2e}")
+    assert ez < 1e-10 and ex < 1e-10
+
+
 def test_adjoint_precision_arguments(ctx):
     with pytest.raises(_lib.BVEError, match="precision must be"):
         ctx.set_adjoint_precision(_lib.BVE_PRECISION_FP64_FAST)

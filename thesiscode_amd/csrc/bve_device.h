@@ -795,9 +795,12 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const int i = tile_base + t * kBlock + threadIdx.x;
-        xi[t] = a.base.tx[i];
-        yi[t] = a.base.ty[i];
-        zi[t] = a.base.tz[i];
+        // targets and partial rows stream (non-temporal): the L2 keeps the
+        // source records, which every workgroup re-reads (level 7: FETCH_SIZE
+        // 273 against 483 MiB per launch, 5.33 against 5.46 ms)
+        xi[t] = __builtin_nontemporal_load(&a.base.tx[i]);
+        yi[t] = __builtin_nontemporal_load(&a.base.ty[i]);
+        zi[t] = __builtin_nontemporal_load(&a.base.tz[i]);
         xf[t] = (float)xi[t];
         yf[t] = (float)yi[t];
         zf[t] = (float)zi[t];
@@ -917,9 +920,9 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         const int i = tile_base + t * kBlock + threadIdx.x - out_off;
-        out[i] = ux[t];
-        out[out_stride + i] = uy[t];
-        out[2 * (size_t)out_stride + i] = uz[t];
+        __builtin_nontemporal_store(ux[t], &out[i]);
+        __builtin_nontemporal_store(uy[t], &out[out_stride + i]);
+        __builtin_nontemporal_store(uz[t], &out[2 * (size_t)out_stride + i]);
     }
 }
 

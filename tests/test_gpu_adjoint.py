@@ -14,6 +14,7 @@ from thesiscode_amd.adjoint import GaussianWeightFunctional, central_difference,
 
 pytestmark = pytest.mark.gpu
 OMEGA = 2 * np.pi
+BAR_MIXED_TLM = 1e-10   # measured 3e-13 and 2.9e-12 at level 5
 
 
 @pytest.fixture(scope="module")
@@ -213,6 +214,35 @@ def test_mixed_adjoint_is_deterministic(ctx):
     a = mixed_gradient(ctx, F, g.active_area, 0.01, 6, 2)
     b = mixed_gradient(ctx, F, g.active_area, 0.01, 6, 2)
     assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def test_mixed_adjoint_matches_tangent_linear(ctx):
+    """The mixed reverse sweep at level 5 (where far tiles run in float32)
+    against the CPU float64 tangent-linear model, for a linear functional:
+    <grad, v> = dJ[v] for random directions, to 1e-10 of the sum of the
+    terms' magnitudes (the float64 sweep's bar)."""
+    g, z = load(ctx, 5)
+    rng = np.random.default_rng(5)
+    cx = rng.normal(size=(g.n_active, 3))
+    cz = rng.normal(size=g.n_active)
+    dt, nsteps = 0.01, 4
+    ctx.set_adjoint_precision(_lib.BVE_PRECISION_MIXED)
+    try:
+        ctx.adjoint_forward(dt, nsteps, 2)
+        gx, gz = ctx.adjoint_backward(cx, cz)
+    finally:
+        ctx.set_adjoint_precision(_lib.BVE_PRECISION_FP64)
+    worst = 0.0
+    for trial in range(2):
+        vx = rng.normal(size=(g.n_active, 3)) * 1e-2 * trial
+        vz = rng.normal(size=g.n_active)
+        _, _, dxT, dzT = oracle.tlm_rk4(g.active_x, z, g.active_area, vx, vz, dt, OMEGA, nsteps)
+        tlm = np.sum(cx * dxT) + np.sum(cz * dzT)
+        adj = np.sum(gx * vx) + np.sum(gz * vz)
+        scale = np.sum(np.abs(cx * dxT)) + np.sum(np.abs(cz * dzT))
+        worst = max(worst, abs(adj - tlm) / scale)
+        print(f"trial {trial}: tlm {tlm:.15e} mixed adjoint {adj:.15e} rel {abs(adj - tlm) / scale:.2e}")
+    assert worst < BAR_MIXED_TLM
 
 
 def test_mixed_adjoint_repeated_reverse_sweep(ctx):

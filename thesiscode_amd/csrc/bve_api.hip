@@ -161,6 +161,7 @@ struct bve_ctx {
     float *srcc = nullptr;
     double4 *tile_box = nullptr, *group_box = nullptr;
     double *tile_mom = nullptr;   // bounded mode: kTileMoments per 256-source tile
+    double *sub_mom = nullptr;    // bounded mode: kSubMoments per 64-source sub-tile
     double *scale = nullptr;      // bounded mode: velocity scale of the current pass
     // set between a bounded stage update, which folded max |u| into scale,
     // and the next pair pass of the same run, which then needs no probe
@@ -590,7 +591,9 @@ int ensure_mixed(bve_ctx *c, int n_src_pad, int t_stride)
     if (is_bounded(c)) {
         if (!c->tile_mom || n_src_pad > c->mixed_src_cap_mom) {
             dev_free(c->tile_mom);
+            dev_free(c->sub_mom);
             HIP_TRY(c, dev_alloc(&c->tile_mom, (size_t)(n_src_pad / 256) * kTileMoments));
+            HIP_TRY(c, dev_alloc(&c->sub_mom, (size_t)(n_src_pad / kSubTile) * kSubMoments));
             c->mixed_src_cap_mom = n_src_pad;
         }
         if (!c->scale) HIP_TRY(c, dev_alloc(&c->scale, 1));
@@ -613,6 +616,7 @@ MixedArgs mixed_args_of(const bve_ctx *c, const PairArgs &a, int n_targets, int 
     ma.rnf = c->nearfield;
     ma.kahan = c->precision == BVE_PRECISION_MIXED_KAHAN;
     ma.tile_mom = c->tile_mom;
+    ma.sub_mom = c->sub_mom;
     ma.scale = c->scale;
     ma.err_target = c->err_target;
     return ma;
@@ -1162,6 +1166,7 @@ int bve_destroy(bve_ctx *c)
     dev_free(c->tile_box);
     dev_free(c->group_box);
     dev_free(c->tile_mom);
+    dev_free(c->sub_mom);
     dev_free(c->scale);
     if (c->queue) (void)hipFree(c->queue);
     if (c->status) (void)hipFree(c->status);

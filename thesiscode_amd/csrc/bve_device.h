@@ -159,6 +159,14 @@ struct alignas(16) StagedSource {
     double x, y, z, wx, wy, wz;
 };
 
+// A source record as the float64 pair loop uses it: staged (as is) or
+// straight from its HBM record.
+__device__ __forceinline__ StagedSource staged_of(const StagedSource &s) { return s; }
+__device__ __forceinline__ StagedSource staged_of(const Source &s)
+{
+    return StagedSource{s.x, s.y, s.z, s.w * s.x, s.w * s.y, s.w * s.z};
+}
+
 // Stages `n` HBM records starting at src into LDS (all threads of the block).
 __device__ __forceinline__ void stage_sources(StagedSource *lds, const Source *__restrict__ src,
                                               int n)
@@ -180,8 +188,8 @@ __device__ __forceinline__ void stage_sources(StagedSource *lds, const Source *_
 //
 // MASK: the tile holds the own-index source of some target of this wave;
 // that pair's denominator (~0) is replaced by 1 and its 1/d by 0.
-template <int T, int TILE, int R, bool MASK>
-__device__ __forceinline__ void tile_f64(const StagedSource *__restrict__ lds, int jt,
+template <int T, int TILE, int R, bool MASK, class Rec>
+__device__ __forceinline__ void tile_f64(const Rec *__restrict__ lds, int jt,
                                          const double (&nx)[T], const double (&ny)[T],
                                          const double (&nz)[T], const double (&c0)[T],
                                          const int (&self)[T], double (&sx)[T],
@@ -192,7 +200,7 @@ __device__ __forceinline__ void tile_f64(const StagedSource *__restrict__ lds, i
     for (int k = 0; k < TILE; k += R) {
         StagedSource s[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[r] = lds[k + r];
+        for (int r = 0; r < R; ++r) s[r] = staged_of(lds[k + r]);
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             double d[R], q[R];
@@ -667,8 +675,11 @@ __device__ __forceinline__ void pair_mixed_segments(const MixedArgs &a, MixedLds
 // three products for delta, the difference D, a quarter of a shared
 // reciprocal, delta^2 / D and three FMAs into R.
 // The remainder is t^2 of the plain term (t = |delta| / E, at most
-// 2R/rho + 2R^2/rho^2 for a tile of radius R whose centre lies rho from
-// the target), so its float32 rounding is t^2 of a float32 pair's: about
+// 2R/rho + R^2/rho^2 for a tile of radius R whose centre lies rho from
+// the target: with unit vectors, E >= |x_i - c|^2 / 2 and
+// |x_i.o_j| <= rho R + |c.o_j|, |c.o_j| <= R^2 / 2 as
+// 1 - |c|^2 = mean_j |o_j|^2 <= R^2), so its float32 rounding is t^2 of a
+// float32 pair's: about
 // 8 eps32 t^2 |w| / d for the rounding of delta^2 / D (x_i x x_j has length
 // about d = |x_i - x_j|) and 8 eps32 t^2 |w| / d^2 for that of the records
 // w x_j and of the sums (the whole vector, radial part included).  A
@@ -681,7 +692,20 @@ __device__ __forceinline__ void pair_mixed_segments(const MixedArgs &a, MixedLds
 // an advance (the field moves by O(dt) between stages), else over the probe
 // pass's targets (a lower bound of the true maximum, so tau errs on the
 // safe side): vortex-core tiles and near tiles run in float64, the
-// background far field in float32.  Four pairs share one
+// background far field in float32.
+//
+// A (group, tile) pair that fails these tests is tried again per 64-source
+// sub-tile, each with its own centre c', radius and float64 moments: about
+// half the radius, so about a quarter of the rounding bound.  A CPU
+// emulation of this routing at level 7 (Gaussian vortex, target 1e-10)
+// puts 4.7% of the pairs in float64, against 15.7% at whole tiles.  A
+// sub-tile runs in float64 for every target of a lane when it does for one
+// of them.  The float32 records hold o' = x_j - c' (relative to the
+// sub-tile), so that the sub-tiles need no records of their own (a second
+// set would crowd the source records out of the L2); a whole-tile
+// expansion about c takes delta = x_i.o' + x_i.(c' - c), the second term
+// once per target and sub-tile, and its rounding then follows
+// |o'| + |c' - c| <= 2R in place of |o| <= R.  Four pairs share one
 // v_rcp_f32 of the product of their denominators (all >= (1 - kBoundedTmax) E,
 // so the product stays far from float32 underflow).
 
@@ -689,8 +713,8 @@ constexpr float kBoundedTmax = 0.9f;
 constexpr double kBoundedEps32 = 5.9604644775390625e-08;   // 2^-24
 
 struct BoundedLds {
-    float4 pos[kMixedTile];         // o_j (.w unused); passes without SCALAR
-    float4 wgt[kMixedTile];         // w_j x_j (.w unused); passes without SCALAR
+    float4 pos[kMixedTile];         // o'_j (.w unused); SCALAR passes: staged with f64
+    float4 wgt[kMixedTile];         // w_j x_j (.w unused); SCALAR passes: staged with f64
     StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
 };
 
@@ -701,18 +725,19 @@ struct BoundedLds {
 // level 8 (393,216): 87.6 against 84.7 ms.
 constexpr int kBoundedScalarMaxSources = 131072;
 
-// Remainder sums R of a source tile's float32 records (pos = {o, 0},
-// wgt = {w x, 0}) for N targets, from LDS or (SCALAR passes) straight from
-// global memory at wave-uniform addresses, where they arrive through scalar
-// loads into SGPRs.
-template <int N>
+// Remainder sums R over NS float32 records (pos = {o', 0}, wgt = {w x, 0})
+// for N targets, from LDS or (SCALAR passes) straight from global memory at
+// wave-uniform addresses, where they arrive through scalar loads into
+// SGPRs.  delta = x_i.o' + kf: kf = x_i.(c' - c) when the expansion is about
+// the tile centre c, 0 when it is about the sub-tile's own centre c'.
+template <int N, int NS>
 __device__ __forceinline__ void remainder_tile(const float4 *__restrict__ pos,
                                                const float4 *__restrict__ wgt, const float *xf,
                                                const float *yf, const float *zf, const float *Ef,
-                                               float *Rx, float *Ry, float *Rz)
+                                               const float *kf, float *Rx, float *Ry, float *Rz)
 {
 #pragma unroll 2
-    for (int k = 0; k < kMixedTile; k += 4) {
+    for (int k = 0; k < NS; k += 4) {
         float4 s[4], v[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -724,7 +749,9 @@ __device__ __forceinline__ void remainder_tile(const float4 *__restrict__ pos,
             float dl[4], D[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                dl[r] = __builtin_fmaf(xf[t], s[r].x, __builtin_fmaf(yf[t], s[r].y, zf[t] * s[r].z));
+                dl[r] = __builtin_fmaf(xf[t], s[r].x,
+                                       __builtin_fmaf(yf[t], s[r].y,
+                                                      __builtin_fmaf(zf[t], s[r].z, kf[t])));
                 D[r] = Ef[t] - dl[r];
             }
             const float p01 = D[0] * D[1], p23 = D[2] * D[3];
@@ -770,6 +797,100 @@ __device__ __forceinline__ void bounded_combine(const double *__restrict__ mom, 
     uz += xi * Sy - yi * Sx;
 }
 
+// Whether a (target group, source block) pair of a bounded pass runs in
+// float64: the block holds a target's own index (diag), the group sphere gb
+// reaches the block sphere sb, t may exceed kBoundedTmax, or the remainder's
+// rounding bound 8 eps32 t^2 W / (d_min min(d_min, 1)) exceeds tau
+// (W = sum |w| over the block).
+__device__ __forceinline__ bool bounded_route64(const double4 &gb, const double4 &sb, double W,
+                                                bool diag, double tau)
+{
+    const double dx = gb.x - sb.x, dy = gb.y - sb.y, dz = gb.z - sb.z;
+    const double rho_c = sqrt(dx * dx + dy * dy + dz * dz) - gb.w;
+    const double R = sb.w;
+    if (diag || !(rho_c > R)) return true;
+    const double q = R / rho_c;
+    const double tb = q * (2.0 + q);
+    const double dmin = rho_c - R;
+    return tb >= (double)kBoundedTmax ||
+           8.0 * kBoundedEps32 * tb * tb * W > tau * dmin * fmin(dmin, 1.0);
+}
+
+// u += x_i x S over NS float32 records (pos, wgt) and their float64
+// expansion (sphere sb, moments mom), for N targets.  TILE: the expansion
+// is about the tile centre and the records span the tile's sub-tiles
+// (offsets c' - c in mom); else the records are one sub-tile's and the
+// expansion is about its own centre.
+template <int N, int NS, bool TILE>
+__device__ __forceinline__ void remainder_block(const float4 *__restrict__ pos,
+                                                const float4 *__restrict__ wgt, const double4 &sb,
+                                                const double *__restrict__ mom, const double *c0,
+                                                const double *xi, const double *yi,
+                                                const double *zi, const float *xf,
+                                                const float *yf, const float *zf, double *ux,
+                                                double *uy, double *uz)
+{
+    float Rx[N], Ry[N], Rz[N], Ef[N], kf[N];
+    double E[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        E[t] = c0[t] - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
+        Ef[t] = (float)E[t];
+        Rx[t] = Ry[t] = Rz[t] = 0.f;
+        kf[t] = 0.f;
+    }
+    if constexpr (TILE) {
+        // unrolled, so that the offsets arrive with the tile's moments
+#pragma unroll
+        for (int s = 0; s < NS / kSubTile; ++s) {
+            const float4 dc = ((const float4 *)(mom + 12))[s];
+#pragma unroll
+            for (int t = 0; t < N; ++t)
+                kf[t] = __builtin_fmaf(xf[t], dc.x, __builtin_fmaf(yf[t], dc.y, zf[t] * dc.z));
+            remainder_tile<N, kSubTile>(pos + s * kSubTile, wgt + s * kSubTile, xf, yf, zf, Ef,
+                                        kf, Rx, Ry, Rz);
+        }
+    } else {
+        remainder_tile<N, NS>(pos, wgt, xf, yf, zf, Ef, kf, Rx, Ry, Rz);
+    }
+#pragma unroll
+    for (int t = 0; t < N; ++t)
+        bounded_combine(mom, sb, xi[t], yi[t], zi[t], E[t], Rx[t], Ry[t], Rz[t], ux[t], uy[t],
+                        uz[t]);
+}
+
+// u += x_i x S over NS float64 records (staged in LDS, or HBM records read
+// through scalar loads; the first is source jt), for N targets; diag: the
+// block holds the own index of some target.
+template <int N, int NS, class Rec>
+__device__ __forceinline__ void f64_block(const Rec *__restrict__ lds, int jt, bool diag,
+                                          const double *c0, const int *self, const double *xi,
+                                          const double *yi, const double *zi, double *ux,
+                                          double *uy, double *uz)
+{
+    double nx[N], ny[N], nz[N], cc[N], sx[N], sy[N], sz[N];
+    int sf[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        nx[t] = -xi[t];
+        ny[t] = -yi[t];
+        nz[t] = -zi[t];
+        cc[t] = c0[t];
+        sf[t] = self[t];
+        sx[t] = sy[t] = sz[t] = 0.0;
+    }
+    if (diag)
+        tile_f64<N, NS, 4, true>(lds, jt, nx, ny, nz, cc, sf, sx, sy, sz);
+    else
+        tile_f64<N, NS, 4, false>(lds, jt, nx, ny, nz, cc, sf, sx, sy, sz);
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        ux[t] += yi[t] * sz[t] - zi[t] * sy[t];
+        uy[t] += zi[t] * sx[t] - xi[t] * sz[t];
+        uz[t] += xi[t] * sy[t] - yi[t] * sx[t];
+    }
+}
+
 // Source tiles (first + k step) mod n_src_tiles, k in [k0, k1), for target
 // tile `ttile` of a bounded pass: the sums x_i x S_i of target i go to
 // out[i - out_off] and the rows out_stride, 2 out_stride after it.
@@ -778,18 +899,22 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                                                    int step, int k0, int k1, double *out,
                                                    int out_stride, int out_off, BoundedLds &lds,
                                                    double tau, const float4 *__restrict__ g32,
-                                                   const float4 *__restrict__ gw)
+                                                   const float4 *__restrict__ gw,
+                                                   const double *__restrict__ gmom,
+                                                   const double *__restrict__ gsub)
 {
-    // g32, gw: the float32 records a.src32, a.srcw as the kernel's
-    // no-alias arguments (read-only here, so the compiler may use scalar
-    // loads for them); read from LDS instead when not SCALAR
+    // g32, gw: the float32 records a.src32, a.srcw, and gmom, gsub: the
+    // moments a.tile_mom, a.sub_mom, as the kernel's no-alias arguments
+    // (read-only here, so the compiler may use scalar loads for them); the
+    // float32 records are read from LDS instead when not SCALAR
     const int tile_base = ttile * (kBlock * T);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    const int lane = threadIdx.x & 63;
     const TargetClasses cls = a.base.cls;
 
     // per-target state kept small (4 waves per SIMD): the group spheres are
-    // re-read per tile (wave-uniform, cached), c0 and the self index
-    // recomputed where needed
+    // re-read per tile (cached), c0 and the self index recomputed where
+    // needed
     double xi[T], yi[T], zi[T], ux[T], uy[T], uz[T];
     float xf[T], yf[T], zf[T];
 #pragma unroll
@@ -810,6 +935,12 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
         const int i = tile_base + t * kBlock + threadIdx.x;
         return (i >= cls.smooth_begin && i < cls.smooth_end) ? 1.0 + cls.eps2 : 1.0;
     };
+    auto self_of = [&](int t) {
+        const int i = tile_base + t * kBlock + threadIdx.x;
+        return i < cls.self_end ? i : -1;
+    };
+    // first target of this wave's group for t
+    auto group_of = [&](int t) { return tile_base + t * kBlock + wave * 64; };
 
     const int nst = a.base.n_src_pad / kMixedTile;
     for (int k = k0; k < k1; ++k) {
@@ -820,99 +951,75 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             lds.pos[threadIdx.x] = g32[jt + threadIdx.x];
             lds.wgt[threadIdx.x] = gw[jt + threadIdx.x];
         }
-        const float4 *pos = SCALAR ? g32 + jt : lds.pos;
-        const float4 *wgt = SCALAR ? gw + jt : lds.wgt;
         const double4 sb = a.tile_box[tl];
-        const double *mom = a.tile_mom + (size_t)tl * kTileMoments;
-        const double W = mom[10];
-        bool f64[T];
-        bool any = false, all = true;
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            // wave-uniform: one target group per wave and t
-            const double4 gb = a.group_box[(tile_base + t * kBlock) / kGroup + wave];
-            const double dx = gb.x - sb.x, dy = gb.y - sb.y, dz = gb.z - sb.z;
-            const double rho_c = sqrt(dx * dx + dy * dy + dz * dz) - gb.w;
-            const double R = sb.w;
-            bool use64 = diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt,
-                                   kMixedTile) || !(rho_c > R);
-            if (!use64) {
-                const double q = R / rho_c;
-                const double tb = 2.0 * q * (1.0 + q);
-                const double dmin = rho_c - R;
-                use64 = tb >= (double)kBoundedTmax ||
-                        8.0 * kBoundedEps32 * tb * tb * W > tau * dmin * fmin(dmin, 1.0);
-            }
-            f64[t] = use64;
-            any |= use64;
-            all &= use64;
+        const double *mom = gmom + (size_t)tl * kTileMoments;
+        const double *sub = gsub + (size_t)(jt / kSubTile) * kSubMoments;
+        // the routing tests run one per lane and meet in a ballot: lane t
+        // tests group t against the tile, lane 4 t + s (further down) group t
+        // against sub-tile s
+        bool r = false;
+        if (lane < T) {
+            const int g = group_of(lane);
+            r = bounded_route64(a.group_box[g / kGroup], sb, mom[10],
+                                diag_tile(g, cls.self_end, jt, kMixedTile), tau);
         }
+        const bool any = __ballot(r) != 0;
         if (__syncthreads_or(any ? 1 : 0)) {
             stage_sources(lds.f64, a.base.src + jt, kMixedTile);
+            if (SCALAR) {
+                // the sub-tiles' float32 loops read LDS too: short runs of
+                // scalar loads would each wait on the L2
+                lds.pos[threadIdx.x] = g32[jt + threadIdx.x];
+                lds.wgt[threadIdx.x] = gw[jt + threadIdx.x];
+            }
             __syncthreads();
         }
         if (!any) {
-            float Rx[T], Ry[T], Rz[T], Ef[T];
-            double E[T];
+            // every target of the lane: the whole tile in float32
+            double c0[T];
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-                E[t] = c0_of(t) - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
-                Ef[t] = (float)E[t];
-                Rx[t] = Ry[t] = Rz[t] = 0.f;
-            }
-            remainder_tile<T>(pos, wgt, xf, yf, zf, Ef, Rx, Ry, Rz);
+            for (int t = 0; t < T; ++t) c0[t] = c0_of(t);
+            remainder_block<T, kMixedTile, true>(SCALAR ? g32 + jt : lds.pos,
+                                                 SCALAR ? gw + jt : lds.wgt, sb, mom, c0, xi, yi,
+                                                 zi, xf, yf, zf, ux, uy, uz);
+            continue;
+        }
+        // the tile again per sub-tile, each in float32 or float64
+        r = false;
+        if (lane < 4 * T) {
+            const int g = group_of(lane >> 2), sl = lane & 3;
+            const double *sm = sub + sl * kSubMoments;
+            r = bounded_route64(a.group_box[g / kGroup], make_double4(sm[0], sm[1], sm[2], sm[3]),
+                                sm[14], diag_tile(g, cls.self_end, jt + sl * kSubTile, kSubTile),
+                                tau);
+        }
+        // a sub-tile runs in float64 for every target of the lane when it
+        // does for one of them (the lane's targets share its loops)
+        unsigned long long sub64 = __ballot(r);
 #pragma unroll
-            for (int t = 0; t < T; ++t)
-                bounded_combine(mom, sb, xi[t], yi[t], zi[t], E[t], Rx[t], Ry[t], Rz[t],
-                                ux[t], uy[t], uz[t]);
-        } else if (all) {
-            // every target of the lane in float64: one pass over the records
-            double nx[T], ny[T], nz[T], cc[T], sx[T], sy[T], sz[T];
-            int sf[T];
-            bool diag = false;
+        for (int t = 1; t < T; ++t) sub64 |= sub64 >> 4;
+#pragma unroll 1
+        for (int s = 0; s < kMixedTile / kSubTile; ++s) {
+            const int js = jt + s * kSubTile;
+            const double *sm = sub + s * kSubMoments;
+            double c0[T];
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-                const int i = tile_base + t * kBlock + threadIdx.x;
-                nx[t] = -xi[t];
-                ny[t] = -yi[t];
-                nz[t] = -zi[t];
-                cc[t] = c0_of(t);
-                sf[t] = i < cls.self_end ? i : -1;
-                sx[t] = sy[t] = sz[t] = 0.0;
-                diag |= diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, kMixedTile);
-            }
-            if (diag)
-                tile_f64<T, kMixedTile, 4, true>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
-            else
-                tile_f64<T, kMixedTile, 4, false>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
+            for (int t = 0; t < T; ++t) c0[t] = c0_of(t);
+            if ((sub64 >> s) & 1) {
+                int self[T];
+                bool diag = false;
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-                ux[t] += yi[t] * sz[t] - zi[t] * sy[t];
-                uy[t] += zi[t] * sx[t] - xi[t] * sz[t];
-                uz[t] += xi[t] * sy[t] - yi[t] * sx[t];
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                if (f64[t]) {
-                    const int i = tile_base + t * kBlock + threadIdx.x;
-                    double nx[1] = {-xi[t]}, ny[1] = {-yi[t]}, nz[1] = {-zi[t]};
-                    double cc[1] = {c0_of(t)}, sx[1] = {0.0}, sy[1] = {0.0}, sz[1] = {0.0};
-                    int sf[1] = {i < cls.self_end ? i : -1};
-                    if (diag_tile(tile_base + t * kBlock + wave * 64, cls.self_end, jt, kMixedTile))
-                        tile_f64<1, kMixedTile, 4, true>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
-                    else
-                        tile_f64<1, kMixedTile, 4, false>(lds.f64, jt, nx, ny, nz, cc, sf, sx, sy, sz);
-                    ux[t] += yi[t] * sz[0] - zi[t] * sy[0];
-                    uy[t] += zi[t] * sx[0] - xi[t] * sz[0];
-                    uz[t] += xi[t] * sy[0] - yi[t] * sx[0];
-                } else {
-                    const double E = c0_of(t) - ((xi[t] * sb.x + yi[t] * sb.y) + zi[t] * sb.z);
-                    float Ef[1] = {(float)E}, Rx[1] = {0.f}, Ry[1] = {0.f}, Rz[1] = {0.f};
-                    remainder_tile<1>(pos, wgt, &xf[t], &yf[t], &zf[t], Ef, Rx, Ry, Rz);
-                    bounded_combine(mom, sb, xi[t], yi[t], zi[t], E, Rx[0], Ry[0], Rz[0],
-                                    ux[t], uy[t], uz[t]);
+                for (int t = 0; t < T; ++t) {
+                    self[t] = self_of(t);
+                    diag |= diag_tile(group_of(t), cls.self_end, js, kSubTile);
                 }
+                f64_block<T, kSubTile>(lds.f64 + s * kSubTile, js, diag, c0, self, xi, yi, zi, ux,
+                                       uy, uz);
+            } else {
+                remainder_block<T, kSubTile, false>(
+                    lds.pos + s * kSubTile, lds.wgt + s * kSubTile,
+                    make_double4(sm[0], sm[1], sm[2], sm[3]), sm + 4, c0, xi, yi, zi, xf, yf, zf,
+                    ux, uy, uz);
             }
         }
     }
@@ -937,14 +1044,17 @@ template <int T, bool SCALAR>
 __device__ __forceinline__ void pair_bounded_segments(const MixedArgs &a, BoundedLds &lds,
                                                       int *seg_slot,
                                                       const float4 *__restrict__ g32,
-                                                      const float4 *__restrict__ gw)
+                                                      const float4 *__restrict__ gw,
+                                                      const double *__restrict__ gmom,
+                                                      const double *__restrict__ gsub)
 {
     constexpr int TT = kBlock * T;
     const double tau = bounded_tau(a);
     run_segments<kMixedTile>(a.base.plan, a.base.queue, seg_slot,
                              [&](int t, int j0, int j1, int seg) {
         pair_bounded_range<T, SCALAR>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile, j1 / kMixedTile,
-                              a.base.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds, tau, g32, gw);
+                              a.base.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds, tau, g32, gw,
+                              gmom, gsub);
     });
 }
 
@@ -1050,18 +1160,24 @@ __device__ __forceinline__ void mixed_target_prep_body(const PairArgs &b, int n_
     __syncthreads();
 }
 
-// Sums over a workgroup of 256 threads (4 waves) of N values per thread:
-// a butterfly within each wave (both lanes of a step add the same two
-// values, so every lane holds the same wave sum), then the wave sums in
-// wave order.  Every thread gets the N totals; fixed order, deterministic.
-// sh: at least 4 N doubles.
+// Sums over a wave of N values per lane by a butterfly: both lanes of a
+// step add the same two values, so every lane ends with the same sums.
 template <int N>
-__device__ __forceinline__ void block_sums(double (&v)[N], double *sh)
+__device__ __forceinline__ void wave_sums(double (&v)[N])
 {
 #pragma unroll
     for (int q = 0; q < N; ++q)
 #pragma unroll
         for (int m = 32; m > 0; m >>= 1) v[q] += __shfl_xor(v[q], m, 64);
+}
+
+// Sums over a workgroup of 256 threads (4 waves) of N values per thread:
+// the wave sums (wave_sums), then added in wave order.  Every thread gets
+// the N totals; fixed order, deterministic.  sh: at least 4 N doubles.
+template <int N>
+__device__ __forceinline__ void block_sums(double (&v)[N], double *sh)
+{
+    wave_sums<N>(v);
     const int wave = threadIdx.x / 64;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
@@ -1074,15 +1190,20 @@ __device__ __forceinline__ void block_sums(double (&v)[N], double *sh)
 }
 
 // Source tile `tile` of a bounded pass (kMixedTile threads): the centre c
-// (mean of the valid sources) and radius -> tile_box; the float32 records
-// pos = {fl32(x - c), 0} and wgt = {fl32(w x), 0}; and the float64 moments
+// (mean of the valid sources) and radius -> tile_box, the float64 moments
 // {W0 = sum w, m = sum w o, P = sum w o o^T (xx yy zz xy xz yz),
-// W = sum |w|, 0} of o = x - c -> mom[tile * kTileMoments ...].
+// W = sum |w|, 0} of o = x - c -> mom[tile * kTileMoments ...], and
+// wgt = {fl32(w x), 0}.  Each wave does the same for its 64-source
+// sub-tile: centre c' and radius and the moments of o' = x - c' ->
+// sub_mom[sub-tile * kSubMoments ...], the offset {fl32(c' - c), 0} ->
+// the tile's mom entry (doubles 12 + 2 wave, 13 + 2 wave), and the float32
+// records pos = {fl32(o'), 0}.
 __device__ __forceinline__ void bounded_source_prep_body(const Source *__restrict__ src,
                                                          int n_sources, float4 *__restrict__ src32,
                                                          float4 *__restrict__ srcw,
                                                          double4 *__restrict__ tile_box,
-                                                         double *__restrict__ mom, int tile,
+                                                         double *__restrict__ mom,
+                                                         double *__restrict__ sub_mom, int tile,
                                                          double *sh)
 {
     const int j = tile * kMixedTile + threadIdx.x;
@@ -1090,12 +1211,12 @@ __device__ __forceinline__ void bounded_source_prep_body(const Source *__restric
     const bool valid = j < n_sources;
     const double w = valid ? s.w : 0.0;
     double cs[4] = {valid ? 1.0 : 0.0, valid ? s.x : 0.0, valid ? s.y : 0.0, valid ? s.z : 0.0};
+    double ws[4] = {cs[0], cs[1], cs[2], cs[3]};
     block_sums<4>(cs, sh);
     const double inv = cs[0] > 0 ? 1.0 / cs[0] : 0.0;
     const double cx = cs[1] * inv, cy = cs[2] * inv, cz = cs[3] * inv;
     const double ox = s.x - cx, oy = s.y - cy, oz = s.z - cz;
     const double r = block_max(valid ? sqrt(ox * ox + oy * oy + oz * oz) : -1.0, sh);
-    src32[j] = make_float4((float)ox, (float)oy, (float)oz, 0.f);
     srcw[j] = make_float4((float)(w * s.x), (float)(w * s.y), (float)(w * s.z), 0.f);
     double v[11] = {w, w * ox, w * oy, w * oz, w * ox * ox, w * oy * oy, w * oz * oz,
                     w * ox * oy, w * ox * oz, w * oy * oz, fabs(w)};
@@ -1106,6 +1227,31 @@ __device__ __forceinline__ void bounded_source_prep_body(const Source *__restric
 #pragma unroll
         for (int q = 0; q < 11; ++q) m[q] = v[q];
         m[11] = 0.0;
+    }
+
+    // the wave's sub-tile (all lanes hold the same wave sums)
+    wave_sums<4>(ws);
+    const double winv = ws[0] > 0 ? 1.0 / ws[0] : 0.0;
+    const double sx = ws[1] * winv, sy = ws[2] * winv, sz = ws[3] * winv;
+    const double px = s.x - sx, py = s.y - sy, pz = s.z - sz;
+    double sr = valid ? sqrt(px * px + py * py + pz * pz) : -1.0;
+#pragma unroll
+    for (int k = 32; k > 0; k >>= 1) sr = fmax(sr, __shfl_xor(sr, k, 64));
+    src32[j] = make_float4((float)px, (float)py, (float)pz, 0.f);
+    double u[11] = {w, w * px, w * py, w * pz, w * px * px, w * py * py, w * pz * pz,
+                    w * px * py, w * px * pz, w * py * pz, fabs(w)};
+    wave_sums<11>(u);
+    if ((threadIdx.x & 63) == 0) {
+        double *m = sub_mom + (size_t)(j / kSubTile) * kSubMoments;
+        m[0] = sx;
+        m[1] = sy;
+        m[2] = sz;
+        m[3] = sr;
+#pragma unroll
+        for (int q = 0; q < 11; ++q) m[4 + q] = u[q];
+        m[15] = 0.0;
+        ((float4 *)(mom + (size_t)tile * kTileMoments + 12))[threadIdx.x / 64] =
+            make_float4((float)(sx - cx), (float)(sy - cy), (float)(sz - cz), 0.f);
     }
 }
 

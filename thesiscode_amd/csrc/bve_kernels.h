@@ -32,7 +32,7 @@ struct MixedArgs {
     int n_targets;        // real targets (padding is left out of the spheres)
     int n_sources;        // real sources
     float4 *src32;        // per source: o = x - tile centre (float32), |o|^2 / 2 + c
-                          // (bounded mode: o, 0)
+                          // (bounded mode: o' = x - sub-tile centre, 0)
     float4 *srcw;         // per source: w o, w (float32; bounded mode: w x, 0)
     float *srcc;          // per source: c = (1 - |x|^2) / 2
     double4 *tile_box;    // per 256-source tile: centre, radius
@@ -40,16 +40,26 @@ struct MixedArgs {
     double rnf;           // near-field radius (float64 below it)
     int kahan;            // 1: Kahan-compensated float32 sums
     // Bounded mode (BVE_PRECISION_MIXED_BOUNDED): per 256-source tile the
-    // float64 moments {W0, m (3), P (6), W, pad} of kTileMoments doubles
+    // float64 moments {W0, m (3), P (6), W, pad, sub-tile offsets (8)} of
+    // kTileMoments doubles
     // (W0 = sum w, m = sum w o, P = sum w o o^T as xx yy zz xy xz yz,
     // W = sum |w|, o = x - tile centre), the velocity scale max |x x S| the
     // probe pass finds, and the error target relative to it.
     double *tile_mom;
     double *scale;
     double err_target;
+    // Bounded mode, per 64-source sub-tile (kSubMoments doubles): its
+    // centre c' (3), radius, the float64 moments {W0, m (3), P (6), W} of
+    // o' = x - c', and a pad.  The bounded pass's float32 records src32 are
+    // o', relative to the sub-tile; each tile's entry in tile_mom ends with
+    // the float32 offsets {fl32(c' - c), 0} of its four sub-tiles from its
+    // own centre c (four float4 in the last eight doubles).
+    double *sub_mom;
 };
 
-constexpr int kTileMoments = 12;
+constexpr int kTileMoments = 20;
+constexpr int kSubTile = 64;
+constexpr int kSubMoments = 16;
 
 // Record of one active panel in the adjoint's pair pass: position x, the
 // weighted position w x (w = zeta A) and n = m x x (m its velocity

@@ -98,10 +98,11 @@ hipError_t launch_pair_sum_mixed(const MixedArgs &a, hipStream_t s)
 
 __global__ __launch_bounds__(kMixedTile) void bounded_source_prep_kernel(
     const Source *__restrict__ src, int n_sources, float4 *__restrict__ src32,
-    float4 *__restrict__ srcw, double4 *__restrict__ tile_box, double *__restrict__ mom)
+    float4 *__restrict__ srcw, double4 *__restrict__ tile_box, double *__restrict__ mom,
+    double *__restrict__ sub_mom)
 {
     __shared__ double sh[kMixedTile];
-    bounded_source_prep_body(src, n_sources, src32, srcw, tile_box, mom, blockIdx.x, sh);
+    bounded_source_prep_body(src, n_sources, src32, srcw, tile_box, mom, sub_mom, blockIdx.x, sh);
 }
 
 // The probe's maximum starts from zero every pass (a kernel, not a memset,
@@ -124,11 +125,12 @@ __global__ __launch_bounds__(256) void bounded_probe_kernel(const PairArgs b, in
 // latency hiding more than the registers
 template <int T, bool SCALAR>
 __global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(
-    MixedArgs a, const float4 *__restrict__ g32, const float4 *__restrict__ gw)
+    MixedArgs a, const float4 *__restrict__ g32, const float4 *__restrict__ gw,
+    const double *__restrict__ gmom, const double *__restrict__ gsub)
 {
     __shared__ BoundedLds lds;
     __shared__ int seg;
-    pair_bounded_segments<T, SCALAR>(a, lds, &seg, g32, gw);
+    pair_bounded_segments<T, SCALAR>(a, lds, &seg, g32, gw, gmom, gsub);
 }
 
 template <int T, bool SCALAR>
@@ -137,7 +139,8 @@ static hipError_t launch_bounded_ts(const MixedArgs &a, hipStream_t s)
     if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T, SCALAR>), dim3(a.base.plan.grid),
-                       dim3(kBlock), 0, s, a, (const float4 *)a.src32, (const float4 *)a.srcw);
+                       dim3(kBlock), 0, s, a, (const float4 *)a.src32, (const float4 *)a.srcw,
+                       (const double *)a.tile_mom, (const double *)a.sub_mom);
     return hipGetLastError();
 }
 
@@ -180,10 +183,10 @@ hipError_t launch_scale_reset(double *scale, hipStream_t s)
 // of the previous RK4 stage, folded in by its stage update).
 hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
 {
-    if (!a.tile_mom || !a.scale) return hipErrorInvalidValue;
+    if (!a.tile_mom || !a.scale || !a.sub_mom) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bounded_source_prep_kernel, dim3(a.base.n_src_pad / kMixedTile),
                        dim3(kMixedTile), 0, s, a.base.src, a.n_sources, a.src32, a.srcw,
-                       a.tile_box, a.tile_mom);
+                       a.tile_box, a.tile_mom, a.sub_mom);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mixed_target_prep_kernel, dim3(a.base.t_stride / 256), dim3(256), 0, s,
@@ -205,7 +208,7 @@ hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
 hipError_t launch_pair_sum_bounded(const MixedArgs &a, hipStream_t s)
 {
     if (a.base.plan.tile != kMixedTile || a.base.n_src_pad % kMixedTile != 0 ||
-        a.base.t_stride % 256 != 0 || !a.tile_mom || !a.scale)
+        a.base.t_stride % 256 != 0 || !a.tile_mom || !a.scale || !a.sub_mom)
         return hipErrorInvalidValue;
     switch (a.base.plan.targets_per_lane) {
     case 1: return launch_bounded_t<1>(a, s);

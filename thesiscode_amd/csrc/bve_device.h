@@ -716,6 +716,7 @@ struct BoundedLds {
     float4 pos[kMixedTile];         // o'_j (.w unused); SCALAR passes: staged with f64
     float4 wgt[kMixedTile];         // w_j x_j (.w unused); SCALAR passes: staged with f64
     StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
+    int4 need[2];                   // per wave: float64 records needed (alternate tiles)
 };
 
 // Sources up to which a bounded pass reads its float32 records through
@@ -941,6 +942,8 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
     };
     // first target of this wave's group for t
     auto group_of = [&](int t) { return tile_base + t * kBlock + wave * 64; };
+    // every wave has read the need slots of the previous range
+    __syncthreads();
 
     const int nst = a.base.n_src_pad / kMixedTile;
     for (int k = k0; k < k1; ++k) {
@@ -951,6 +954,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             lds.pos[threadIdx.x] = g32[jt + threadIdx.x];
             lds.wgt[threadIdx.x] = gw[jt + threadIdx.x];
         }
+        const int par = (k - k0) & 1;
         const double4 sb = a.tile_box[tl];
         const double *mom = gmom + (size_t)tl * kTileMoments;
         const double *sub = gsub + (size_t)(jt / kSubTile) * kSubMoments;
@@ -964,7 +968,13 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                                 diag_tile(g, cls.self_end, jt, kMixedTile), tau);
         }
         const bool any = __ballot(r) != 0;
-        if (__syncthreads_or(any ? 1 : 0)) {
+        // one barrier: each wave posts its need in this tile's slot, which
+        // is next written two tiles on, after another barrier (and ranges
+        // start with one)
+        if (lane == 0) (&lds.need[par].x)[wave] = any ? 1 : 0;
+        __syncthreads();
+        const int4 need = lds.need[par];
+        if (need.x | need.y | need.z | need.w) {
             stage_sources(lds.f64, a.base.src + jt, kMixedTile);
             if (SCALAR) {
                 // the sub-tiles' float32 loops read LDS too: short runs of

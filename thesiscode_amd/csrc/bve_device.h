@@ -717,6 +717,7 @@ struct BoundedLds {
     float4 wgt[kMixedTile];         // w_j x_j (.w unused); SCALAR passes: staged with f64
     StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
     int4 need[2];                   // per wave: float64 records needed (alternate tiles)
+    float4 wgt_alt[kMixedTile];     // SCALAR passes: w_j x_j of odd tiles (wgt: even)
 };
 
 // Sources up to which a bounded pass reads its float32 records through
@@ -949,12 +950,17 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
     for (int k = k0; k < k1; ++k) {
         const int tl = (int)(((long long)k * step + first) % nst);
         const int jt = tl * kMixedTile;
+        const int par = (k - k0) & 1;
+        // SCALAR passes read o' through scalar loads and w x from LDS (the
+        // scalar cache alone would carry both for 16 waves per CU), staged
+        // for every tile in two alternating buffers that the per-tile
+        // barrier below publishes
+        float4 *wgt_lds = (SCALAR && par) ? lds.wgt_alt : lds.wgt;
         if (!SCALAR) {
             __syncthreads();
             lds.pos[threadIdx.x] = g32[jt + threadIdx.x];
-            lds.wgt[threadIdx.x] = gw[jt + threadIdx.x];
         }
-        const int par = (k - k0) & 1;
+        wgt_lds[threadIdx.x] = gw[jt + threadIdx.x];
         const double4 sb = a.tile_box[tl];
         const double *mom = gmom + (size_t)tl * kTileMoments;
         const double *sub = gsub + (size_t)(jt / kSubTile) * kSubMoments;
@@ -980,7 +986,6 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                 // the sub-tiles' float32 loops read LDS too: short runs of
                 // scalar loads would each wait on the L2
                 lds.pos[threadIdx.x] = g32[jt + threadIdx.x];
-                lds.wgt[threadIdx.x] = gw[jt + threadIdx.x];
             }
             __syncthreads();
         }
@@ -989,9 +994,8 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             double c0[T];
 #pragma unroll
             for (int t = 0; t < T; ++t) c0[t] = c0_of(t);
-            remainder_block<T, kMixedTile, true>(SCALAR ? g32 + jt : lds.pos,
-                                                 SCALAR ? gw + jt : lds.wgt, sb, mom, c0, xi, yi,
-                                                 zi, xf, yf, zf, ux, uy, uz);
+            remainder_block<T, kMixedTile, true>(SCALAR ? g32 + jt : lds.pos, wgt_lds, sb, mom, c0,
+                                                 xi, yi, zi, xf, yf, zf, ux, uy, uz);
             continue;
         }
         // the tile again per sub-tile, each in float32 or float64
@@ -1027,7 +1031,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                                        uy, uz);
             } else {
                 remainder_block<T, kSubTile, false>(
-                    lds.pos + s * kSubTile, lds.wgt + s * kSubTile,
+                    lds.pos + s * kSubTile, wgt_lds + s * kSubTile,
                     make_double4(sm[0], sm[1], sm[2], sm[3]), sm + 4, c0, xi, yi, zi, xf, yf, zf,
                     ux, uy, uz);
             }

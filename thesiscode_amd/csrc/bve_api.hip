@@ -162,6 +162,8 @@ struct bve_ctx {
     double4 *tile_box = nullptr, *group_box = nullptr;
     double *tile_mom = nullptr;   // bounded mode: kTileMoments per 256-source tile
     double *sub_mom = nullptr;    // bounded mode: kSubMoments per 64-source sub-tile
+    unsigned short *route = nullptr;   // bounded mode: routing per (target group, source tile)
+    size_t route_cap = 0;
     double *scale = nullptr;      // bounded mode: velocity scale of the current pass
     // set between a bounded stage update, which folded max |u| into scale,
     // and the next pair pass of the same run, which then needs no probe
@@ -596,6 +598,13 @@ int ensure_mixed(bve_ctx *c, int n_src_pad, int t_stride)
             HIP_TRY(c, dev_alloc(&c->sub_mom, (size_t)(n_src_pad / kSubTile) * kSubMoments));
             c->mixed_src_cap_mom = n_src_pad;
         }
+        const size_t route_n = (size_t)(t_stride / kGroup) * (size_t)(n_src_pad / 256);
+        if (!c->route || route_n > c->route_cap) {
+            dev_free(c->route);
+            c->route_cap = 0;
+            HIP_TRY(c, dev_alloc(&c->route, route_n));
+            c->route_cap = route_n;
+        }
         if (!c->scale) HIP_TRY(c, dev_alloc(&c->scale, 1));
     }
     return BVE_OK;
@@ -617,6 +626,8 @@ MixedArgs mixed_args_of(const bve_ctx *c, const PairArgs &a, int n_targets, int 
     ma.kahan = c->precision == BVE_PRECISION_MIXED_KAHAN;
     ma.tile_mom = c->tile_mom;
     ma.sub_mom = c->sub_mom;
+    ma.route = c->route;
+    ma.n_groups = a.t_stride / kGroup;
     ma.scale = c->scale;
     ma.err_target = c->err_target;
     return ma;
@@ -1167,6 +1178,7 @@ int bve_destroy(bve_ctx *c)
     dev_free(c->group_box);
     dev_free(c->tile_mom);
     dev_free(c->sub_mom);
+    dev_free(c->route);
     dev_free(c->scale);
     if (c->queue) (void)hipFree(c->queue);
     if (c->status) (void)hipFree(c->status);

@@ -430,7 +430,6 @@ __device__ __forceinline__ void pair_f64_segments(const PairArgs &a, StagedSourc
 // requested.
 
 constexpr int kMixedTile = 256;   // sources per LDS tile and per source bounding sphere
-constexpr int kGroup = 64;        // targets per bounding sphere (one wave's lanes)
 
 // One source tile of the mixed pass in LDS.
 struct MixedLds {
@@ -818,6 +817,35 @@ __device__ __forceinline__ bool bounded_route64(const double4 &gb, const double4
            8.0 * kBoundedEps32 * tb * tb * W > tau * dmin * fmin(dmin, 1.0);
 }
 
+// Error threshold tau of a bounded pass from its velocity scale.
+__device__ __forceinline__ double bounded_tau(const MixedArgs &a)
+{
+    return 0.3 * a.err_target * *a.scale;
+}
+
+// Routing of target group g against source tile tl (bounded_route64 of the
+// group sphere against the tile and against each of its sub-tiles) ->
+// a.route[tl * n_groups + g]; one pass over every pair before the pair
+// kernel, which then reads the wave's bits instead of testing.
+__device__ __forceinline__ void bounded_route_body(const MixedArgs &a, int item, double tau)
+{
+    const int tl = item / a.n_groups, g = item - tl * a.n_groups;
+    const int jt = tl * kMixedTile, lo = g * kGroup, self_end = a.base.cls.self_end;
+    const double4 gb = a.group_box[g];
+    unsigned bits = bounded_route64(gb, a.tile_box[tl], a.tile_mom[(size_t)tl * kTileMoments + 10],
+                                    diag_tile(lo, self_end, jt, kMixedTile), tau)
+                        ? 1u
+                        : 0u;
+#pragma unroll
+    for (int s = 0; s < kMixedTile / kSubTile; ++s) {
+        const double *sm = a.sub_mom + (size_t)(jt / kSubTile + s) * kSubMoments;
+        if (bounded_route64(gb, make_double4(sm[0], sm[1], sm[2], sm[3]), sm[14],
+                            diag_tile(lo, self_end, jt + s * kSubTile, kSubTile), tau))
+            bits |= 2u << s;
+    }
+    a.route[item] = (unsigned short)bits;
+}
+
 // u += x_i x S over NS float32 records (pos, wgt) and their float64
 // expansion (sphere sb, moments mom), for N targets.  TILE: the expansion
 // is about the tile centre and the records span the tile's sub-tiles
@@ -900,10 +928,11 @@ template <int T, bool SCALAR>
 __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile, int first,
                                                    int step, int k0, int k1, double *out,
                                                    int out_stride, int out_off, BoundedLds &lds,
-                                                   double tau, const float4 *__restrict__ g32,
+                                                   const float4 *__restrict__ g32,
                                                    const float4 *__restrict__ gw,
                                                    const double *__restrict__ gmom,
-                                                   const double *__restrict__ gsub)
+                                                   const double *__restrict__ gsub,
+                                                   const unsigned short *__restrict__ groute)
 {
     // g32, gw: the float32 records a.src32, a.srcw, and gmom, gsub: the
     // moments a.tile_mom, a.sub_mom, as the kernel's no-alias arguments
@@ -967,16 +996,23 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
         const double4 sb = a.tile_box[tl];
         const double *mom = gmom + (size_t)tl * kTileMoments;
         const double *sub = gsub + (size_t)(jt / kSubTile) * kSubMoments;
-        // the routing tests run one per lane and meet in a ballot: lane t
-        // tests group t against the tile, lane 4 t + s (further down) group t
-        // against sub-tile s
-        bool r = false;
-        if (lane < T) {
-            const int g = group_of(lane);
-            r = bounded_route64(a.group_box[g / kGroup], sb, mom[10],
-                                diag_tile(g, cls.self_end, jt, kMixedTile), tau);
+        // the routing of the wave's T groups (16 bits each, adjacent: one
+        // aligned scalar load; the first group's index is a multiple of T)
+        const unsigned short *rp = groute + (size_t)tl * a.n_groups + group_of(0) / kGroup;
+        unsigned long long rw;
+        if constexpr (T == 1)
+            rw = rp[0];
+        else if constexpr (T == 2)
+            rw = *(const unsigned *)rp;
+        else
+            rw = *(const unsigned long long *)rp;
+        bool any = false;
+        unsigned sub4 = 0;   // sub-tiles in float64 for some group of the wave
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            any |= (rw >> (16 * t)) & 1;
+            sub4 |= (unsigned)(rw >> (16 * t + 1)) & 15u;
         }
-        const bool any = __ballot(r) != 0;
         // one barrier: each wave posts its need in this tile's slot, which
         // is next written two tiles on, after another barrier (and ranges
         // start with one)
@@ -1001,20 +1037,9 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                                                  xi, yi, zi, xf, yf, zf, ux, uy, uz);
             continue;
         }
-        // the tile again per sub-tile, each in float32 or float64
-        r = false;
-        if (lane < 4 * T) {
-            const int g = group_of(lane >> 2), sl = lane & 3;
-            const double *sm = sub + sl * kSubMoments;
-            r = bounded_route64(a.group_box[g / kGroup], make_double4(sm[0], sm[1], sm[2], sm[3]),
-                                sm[14], diag_tile(g, cls.self_end, jt + sl * kSubTile, kSubTile),
-                                tau);
-        }
-        // a sub-tile runs in float64 for every target of the lane when it
-        // does for one of them (the lane's targets share its loops)
-        unsigned long long sub64 = __ballot(r);
-#pragma unroll
-        for (int t = 1; t < T; ++t) sub64 |= sub64 >> 4;
+        // the tile again per sub-tile, each in float32 or float64 (for every
+        // target of the lane when for one of them: the lane's targets share
+        // its loops)
 #pragma unroll 1
         for (int s = 0; s < kMixedTile / kSubTile; ++s) {
             const int js = jt + s * kSubTile;
@@ -1022,7 +1047,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
             double c0[T];
 #pragma unroll
             for (int t = 0; t < T; ++t) c0[t] = c0_of(t);
-            if ((sub64 >> s) & 1) {
+            if ((sub4 >> s) & 1) {
                 int self[T];
                 bool diag = false;
 #pragma unroll
@@ -1050,12 +1075,6 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
     }
 }
 
-// Error threshold tau of a bounded pass from its velocity scale.
-__device__ __forceinline__ double bounded_tau(const MixedArgs &a)
-{
-    return 0.3 * a.err_target * *a.scale;
-}
-
 // Persistent body of a segmented bounded pass (source tiles in stride order).
 template <int T, bool SCALAR>
 __device__ __forceinline__ void pair_bounded_segments(const MixedArgs &a, BoundedLds &lds,
@@ -1063,15 +1082,15 @@ __device__ __forceinline__ void pair_bounded_segments(const MixedArgs &a, Bounde
                                                       const float4 *__restrict__ g32,
                                                       const float4 *__restrict__ gw,
                                                       const double *__restrict__ gmom,
-                                                      const double *__restrict__ gsub)
+                                                      const double *__restrict__ gsub,
+                                                      const unsigned short *__restrict__ groute)
 {
     constexpr int TT = kBlock * T;
-    const double tau = bounded_tau(a);
     run_segments<kMixedTile>(a.base.plan, a.base.queue, seg_slot,
                              [&](int t, int j0, int j1, int seg) {
         pair_bounded_range<T, SCALAR>(a, t, 0, a.base.plan.src_step, j0 / kMixedTile, j1 / kMixedTile,
-                              a.base.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds, tau, g32, gw,
-                              gmom, gsub);
+                              a.base.partial + (size_t)(seg + t) * 3 * TT, TT, t * TT, lds, g32, gw,
+                              gmom, gsub, groute);
     });
 }
 

@@ -55,8 +55,14 @@ struct MixedArgs {
     // the float32 offsets {fl32(c' - c), 0} of its four sub-tiles from its
     // own centre c (four float4 in the last eight doubles).
     double *sub_mom;
+    // Bounded mode: the routing of every (target group, source tile) pair
+    // of the pass, n_groups per source tile: bit 0 the whole tile in
+    // float64, bit 1 + s sub-tile s in float64 (when the tile is retried)
+    unsigned short *route;
+    int n_groups;
 };
 
+constexpr int kGroup = 64;   // targets per bounding sphere (one wave's lanes)
 constexpr int kTileMoments = 20;
 constexpr int kSubTile = 64;
 constexpr int kSubMoments = 16;

@@ -126,11 +126,19 @@ __global__ __launch_bounds__(256) void bounded_probe_kernel(const PairArgs b, in
 template <int T, bool SCALAR>
 __global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(
     MixedArgs a, const float4 *__restrict__ g32, const float4 *__restrict__ gw,
-    const double *__restrict__ gmom, const double *__restrict__ gsub)
+    const double *__restrict__ gmom, const double *__restrict__ gsub,
+    const unsigned short *__restrict__ groute)
 {
     __shared__ BoundedLds lds;
     __shared__ int seg;
-    pair_bounded_segments<T, SCALAR>(a, lds, &seg, g32, gw, gmom, gsub);
+    pair_bounded_segments<T, SCALAR>(a, lds, &seg, g32, gw, gmom, gsub, groute);
+}
+
+__global__ __launch_bounds__(256) void bounded_route_kernel(const MixedArgs a, int n_items)
+{
+    const double tau = bounded_tau(a);
+    for (int item = blockIdx.x * 256 + threadIdx.x; item < n_items; item += gridDim.x * 256)
+        bounded_route_body(a, item, tau);
 }
 
 template <int T, bool SCALAR>
@@ -138,9 +146,16 @@ static hipError_t launch_bounded_ts(const MixedArgs &a, hipStream_t s)
 {
     if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
         return hipErrorInvalidValue;
+    const int n_items = a.n_groups * (a.base.n_src_pad / kMixedTile);
+    const int route_blocks = ceil_div(n_items, 256);
+    hipLaunchKernelGGL(bounded_route_kernel, dim3(route_blocks < 4096 ? route_blocks : 4096),
+                       dim3(256), 0, s, a, n_items);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T, SCALAR>), dim3(a.base.plan.grid),
                        dim3(kBlock), 0, s, a, (const float4 *)a.src32, (const float4 *)a.srcw,
-                       (const double *)a.tile_mom, (const double *)a.sub_mom);
+                       (const double *)a.tile_mom, (const double *)a.sub_mom,
+                       (const unsigned short *)a.route);
     return hipGetLastError();
 }
 
@@ -208,7 +223,9 @@ hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
 hipError_t launch_pair_sum_bounded(const MixedArgs &a, hipStream_t s)
 {
     if (a.base.plan.tile != kMixedTile || a.base.n_src_pad % kMixedTile != 0 ||
-        a.base.t_stride % 256 != 0 || !a.tile_mom || !a.scale || !a.sub_mom)
+        a.base.t_stride % 1024 != 0 || !a.tile_mom || !a.scale || !a.sub_mom || !a.route ||
+        a.n_groups != a.base.t_stride / kGroup ||
+        a.base.plan.n_tiles * kBlock * a.base.plan.targets_per_lane > a.base.t_stride)
         return hipErrorInvalidValue;
     switch (a.base.plan.targets_per_lane) {
     case 1: return launch_bounded_t<1>(a, s);

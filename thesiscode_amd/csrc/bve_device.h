@@ -712,11 +712,11 @@ constexpr float kBoundedTmax = 0.9f;
 constexpr double kBoundedEps32 = 5.9604644775390625e-08;   // 2^-24
 
 struct BoundedLds {
-    float4 pos[kMixedTile];         // o'_j (.w unused); SCALAR passes: staged with f64
-    float4 wgt[kMixedTile];         // w_j x_j (.w unused); SCALAR passes: staged with f64
+    float4 pos[kMixedTile];         // o'_j, (w_j x_j).x; SCALAR passes: staged with f64
+    float4 wgt[kMixedTile];         // (w_j x_j).y, .z (.z, .w unused); SCALAR: even tiles
     StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
     int4 need[2];                   // per wave: float64 records needed (alternate tiles)
-    float4 wgt_alt[kMixedTile];     // SCALAR passes: w_j x_j of odd tiles (wgt: even)
+    float4 wgt_alt[kMixedTile];     // SCALAR passes: wgt of odd tiles
 };
 
 // Sources up to which a bounded pass reads its float32 records through
@@ -726,8 +726,8 @@ struct BoundedLds {
 // level 8 (393,216): 87.6 against 84.7 ms.
 constexpr int kBoundedScalarMaxSources = 131072;
 
-// Remainder sums R over NS float32 records (pos = {o', 0}, wgt = {w x, 0})
-// for N targets, from LDS or (SCALAR passes) straight from global memory at
+// Remainder sums R over NS float32 records (pos = {o', (w x).x},
+// wgt = {(w x).y, (w x).z, 0, 0}) for N targets, from LDS or (SCALAR passes) straight from global memory at
 // wave-uniform addresses, where they arrive through scalar loads into
 // SGPRs.  delta = x_i.o' + kf: kf = x_i.(c' - c) when the expansion is about
 // the tile centre c, 0 when it is about the sub-tile's own centre c'.
@@ -765,9 +765,9 @@ __device__ __forceinline__ void remainder_tile(const float4 *__restrict__ pos,
             f[3] = (dl[3] * dl[3]) * (D[2] * r23);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                Rx[t] = __builtin_fmaf(f[r], v[r].x, Rx[t]);
-                Ry[t] = __builtin_fmaf(f[r], v[r].y, Ry[t]);
-                Rz[t] = __builtin_fmaf(f[r], v[r].z, Rz[t]);
+                Rx[t] = __builtin_fmaf(f[r], s[r].w, Rx[t]);
+                Ry[t] = __builtin_fmaf(f[r], v[r].x, Ry[t]);
+                Rz[t] = __builtin_fmaf(f[r], v[r].y, Rz[t]);
             }
         }
     }
@@ -1229,11 +1229,11 @@ __device__ __forceinline__ void block_sums(double (&v)[N], double *sh)
 // (mean of the valid sources) and radius -> tile_box, the float64 moments
 // {W0 = sum w, m = sum w o, P = sum w o o^T (xx yy zz xy xz yz),
 // W = sum |w|, 0} of o = x - c -> mom[tile * kTileMoments ...], and
-// wgt = {fl32(w x), 0}.  Each wave does the same for its 64-source
-// sub-tile: centre c' and radius and the moments of o' = x - c' ->
-// sub_mom[sub-tile * kSubMoments ...], the offset {fl32(c' - c), 0} ->
-// the tile's mom entry (doubles 12 + 2 wave, 13 + 2 wave), and the float32
-// records pos = {fl32(o'), 0}.
+// wgt = {fl32((w x).y), fl32((w x).z), 0, 0}.  Each wave does the same for
+// its 64-source sub-tile: centre c' and radius and the moments of
+// o' = x - c' -> sub_mom[sub-tile * kSubMoments ...], the offset
+// {fl32(c' - c), 0} -> the tile's mom entry (doubles 12 + 2 wave,
+// 13 + 2 wave), and the float32 records pos = {fl32(o'), fl32((w x).x)}.
 __device__ __forceinline__ void bounded_source_prep_body(const Source *__restrict__ src,
                                                          int n_sources, float4 *__restrict__ src32,
                                                          float4 *__restrict__ srcw,
@@ -1253,7 +1253,7 @@ __device__ __forceinline__ void bounded_source_prep_body(const Source *__restric
     const double cx = cs[1] * inv, cy = cs[2] * inv, cz = cs[3] * inv;
     const double ox = s.x - cx, oy = s.y - cy, oz = s.z - cz;
     const double r = block_max(valid ? sqrt(ox * ox + oy * oy + oz * oz) : -1.0, sh);
-    srcw[j] = make_float4((float)(w * s.x), (float)(w * s.y), (float)(w * s.z), 0.f);
+    srcw[j] = make_float4((float)(w * s.y), (float)(w * s.z), 0.f, 0.f);
     double v[11] = {w, w * ox, w * oy, w * oz, w * ox * ox, w * oy * oy, w * oz * oz,
                     w * ox * oy, w * ox * oz, w * oy * oz, fabs(w)};
     block_sums<11>(v, sh);
@@ -1273,7 +1273,7 @@ __device__ __forceinline__ void bounded_source_prep_body(const Source *__restric
     double sr = valid ? sqrt(px * px + py * py + pz * pz) : -1.0;
 #pragma unroll
     for (int k = 32; k > 0; k >>= 1) sr = fmax(sr, __shfl_xor(sr, k, 64));
-    src32[j] = make_float4((float)px, (float)py, (float)pz, 0.f);
+    src32[j] = make_float4((float)px, (float)py, (float)pz, (float)(w * s.x));
     double u[11] = {w, w * px, w * py, w * pz, w * px * px, w * py * py, w * pz * pz,
                     w * px * py, w * px * pz, w * py * pz, fabs(w)};
     wave_sums<11>(u);

@@ -713,10 +713,10 @@ constexpr double kBoundedEps32 = 5.9604644775390625e-08;   // 2^-24
 
 struct BoundedLds {
     float4 pos[kMixedTile];         // o'_j, (w_j x_j).x; SCALAR passes: staged with f64
-    float4 wgt[kMixedTile];         // (w_j x_j).y, .z (.z, .w unused); SCALAR: even tiles
+    float2 wgt[kMixedTile];         // (w_j x_j).y, .z; SCALAR passes: even tiles
     StagedSource f64[kMixedTile];   // float64 records, staged when a target needs them
     int4 need[2];                   // per wave: float64 records needed (alternate tiles)
-    float4 wgt_alt[kMixedTile];     // SCALAR passes: wgt of odd tiles
+    float2 wgt_alt[kMixedTile];     // SCALAR passes: wgt of odd tiles
 };
 
 // Sources up to which a bounded pass reads its float32 records through
@@ -727,19 +727,20 @@ struct BoundedLds {
 constexpr int kBoundedScalarMaxSources = 131072;
 
 // Remainder sums R over NS float32 records (pos = {o', (w x).x},
-// wgt = {(w x).y, (w x).z, 0, 0}) for N targets, from LDS or (SCALAR passes) straight from global memory at
+// wgt = {(w x).y, (w x).z}) for N targets, from LDS or (SCALAR passes) straight from global memory at
 // wave-uniform addresses, where they arrive through scalar loads into
 // SGPRs.  delta = x_i.o' + kf: kf = x_i.(c' - c) when the expansion is about
 // the tile centre c, 0 when it is about the sub-tile's own centre c'.
 template <int N, int NS>
 __device__ __forceinline__ void remainder_tile(const float4 *__restrict__ pos,
-                                               const float4 *__restrict__ wgt, const float *xf,
+                                               const float2 *__restrict__ wgt, const float *xf,
                                                const float *yf, const float *zf, const float *Ef,
                                                const float *kf, float *Rx, float *Ry, float *Rz)
 {
 #pragma unroll 2
     for (int k = 0; k < NS; k += 4) {
-        float4 s[4], v[4];
+        float4 s[4];
+        float2 v[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             s[r] = pos[k + r];
@@ -853,7 +854,7 @@ __device__ __forceinline__ void bounded_route_body(const MixedArgs &a, int item,
 // expansion is about its own centre.
 template <int N, int NS, bool TILE>
 __device__ __forceinline__ void remainder_block(const float4 *__restrict__ pos,
-                                                const float4 *__restrict__ wgt, const double4 &sb,
+                                                const float2 *__restrict__ wgt, const double4 &sb,
                                                 const double *__restrict__ mom, const double *c0,
                                                 const double *xi, const double *yi,
                                                 const double *zi, const float *xf,
@@ -929,7 +930,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
                                                    int step, int k0, int k1, double *out,
                                                    int out_stride, int out_off, BoundedLds &lds,
                                                    const float4 *__restrict__ g32,
-                                                   const float4 *__restrict__ gw,
+                                                   const float2 *__restrict__ gw,
                                                    const double *__restrict__ gmom,
                                                    const double *__restrict__ gsub,
                                                    const unsigned short *__restrict__ groute)
@@ -987,7 +988,7 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
         // scalar cache alone would carry both for 16 waves per CU), staged
         // for every tile in two alternating buffers that the per-tile
         // barrier below publishes
-        float4 *wgt_lds = (SCALAR && par) ? lds.wgt_alt : lds.wgt;
+        float2 *wgt_lds = (SCALAR && par) ? lds.wgt_alt : lds.wgt;
         if (!SCALAR) {
             __syncthreads();
             lds.pos[threadIdx.x] = g32[jt + threadIdx.x];
@@ -1080,7 +1081,7 @@ template <int T, bool SCALAR>
 __device__ __forceinline__ void pair_bounded_segments(const MixedArgs &a, BoundedLds &lds,
                                                       int *seg_slot,
                                                       const float4 *__restrict__ g32,
-                                                      const float4 *__restrict__ gw,
+                                                      const float2 *__restrict__ gw,
                                                       const double *__restrict__ gmom,
                                                       const double *__restrict__ gsub,
                                                       const unsigned short *__restrict__ groute)
@@ -1229,7 +1230,8 @@ __device__ __forceinline__ void block_sums(double (&v)[N], double *sh)
 // (mean of the valid sources) and radius -> tile_box, the float64 moments
 // {W0 = sum w, m = sum w o, P = sum w o o^T (xx yy zz xy xz yz),
 // W = sum |w|, 0} of o = x - c -> mom[tile * kTileMoments ...], and
-// wgt = {fl32((w x).y), fl32((w x).z), 0, 0}.  Each wave does the same for
+// wgt = {fl32((w x).y), fl32((w x).z)} (float2 records in srcw's first
+// half).  Each wave does the same for
 // its 64-source sub-tile: centre c' and radius and the moments of
 // o' = x - c' -> sub_mom[sub-tile * kSubMoments ...], the offset
 // {fl32(c' - c), 0} -> the tile's mom entry (doubles 12 + 2 wave,
@@ -1253,7 +1255,7 @@ __device__ __forceinline__ void bounded_source_prep_body(const Source *__restric
     const double cx = cs[1] * inv, cy = cs[2] * inv, cz = cs[3] * inv;
     const double ox = s.x - cx, oy = s.y - cy, oz = s.z - cz;
     const double r = block_max(valid ? sqrt(ox * ox + oy * oy + oz * oz) : -1.0, sh);
-    srcw[j] = make_float4((float)(w * s.y), (float)(w * s.z), 0.f, 0.f);
+    ((float2 *)srcw)[j] = make_float2((float)(w * s.y), (float)(w * s.z));
     double v[11] = {w, w * ox, w * oy, w * oz, w * ox * ox, w * oy * oy, w * oz * oz,
                     w * ox * oy, w * ox * oz, w * oy * oz, fabs(w)};
     block_sums<11>(v, sh);

@@ -33,8 +33,8 @@ struct MixedArgs {
     int n_sources;        // real sources
     float4 *src32;        // per source: o = x - tile centre (float32), |o|^2 / 2 + c
                           // (bounded mode: o' = x - sub-tile centre, (w x).x)
-    float4 *srcw;         // per source: w o, w (float32; bounded mode: (w x).y, (w x).z,
-                          // 0, 0)
+    float4 *srcw;         // per source: w o, w (float32; bounded mode: float2
+                          // records (w x).y, (w x).z in the first half)
     float *srcc;          // per source: c = (1 - |x|^2) / 2
     double4 *tile_box;    // per 256-source tile: centre, radius
     double4 *group_box;   // per 64 targets: centre, radius

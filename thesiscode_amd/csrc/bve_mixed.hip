@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void bounded_probe_kernel(const PairArgs b, in
 // latency hiding more than the registers
 template <int T, bool SCALAR>
 __global__ __launch_bounds__(kBlock, 4) void pair_sum_bounded_seg_kernel(
-    MixedArgs a, const float4 *__restrict__ g32, const float4 *__restrict__ gw,
+    MixedArgs a, const float4 *__restrict__ g32, const float2 *__restrict__ gw,
     const double *__restrict__ gmom, const double *__restrict__ gsub,
     const unsigned short *__restrict__ groute)
 {
@@ -153,7 +153,7 @@ static hipError_t launch_bounded_ts(const MixedArgs &a, hipStream_t s)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T, SCALAR>), dim3(a.base.plan.grid),
-                       dim3(kBlock), 0, s, a, (const float4 *)a.src32, (const float4 *)a.srcw,
+                       dim3(kBlock), 0, s, a, (const float4 *)a.src32, (const float2 *)a.srcw,
                        (const double *)a.tile_mom, (const double *)a.sub_mom,
                        (const unsigned short *)a.route);
     return hipGetLastError();

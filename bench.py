@@ -53,7 +53,7 @@ MODES = {"fp64": 0, "fp64_fast": 2, "mixed": 1, "mixed_kahan": 3, "reference": 4
 # copy_probe), for the default plans on the level-7 Gaussian vortex
 # (profiles/pair_sum_bounded_L7_counters.md, pair_sum_f64_fast_L7_traffic_pmc.md)
 MEASURED_TRAFFIC = {
-    ("pair_sum_bounded_seg_kernel<2,true>", 7): 0.914e9,
+    ("pair_sum_bounded_seg_kernel<2,true>", 7): 0.688e9,
     ("pair_sum_f64_seg_kernel<4,512,4>", 7): 1.99e8,
 }
 

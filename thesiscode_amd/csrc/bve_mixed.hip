@@ -146,12 +146,6 @@ static hipError_t launch_bounded_ts(const MixedArgs &a, hipStream_t s)
 {
     if (a.base.plan.segments <= 0 || !a.base.queue || a.base.plan.src_step < 1)
         return hipErrorInvalidValue;
-    const int n_items = a.n_groups * (a.base.n_src_pad / kMixedTile);
-    const int route_blocks = ceil_div(n_items, 256);
-    hipLaunchKernelGGL(bounded_route_kernel, dim3(route_blocks < 4096 ? route_blocks : 4096),
-                       dim3(256), 0, s, a, n_items);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((pair_sum_bounded_seg_kernel<T, SCALAR>), dim3(a.base.plan.grid),
                        dim3(kBlock), 0, s, a, (const float4 *)a.src32, (const float2 *)a.srcw,
                        (const double *)a.tile_mom, (const double *)a.sub_mom,
@@ -194,8 +188,21 @@ hipError_t launch_scale_reset(double *scale, hipStream_t s)
     return hipGetLastError();
 }
 
+// The routing bits of every (target group, source tile) pair, from the
+// spheres, moments and velocity scale of this pass.
+static hipError_t launch_bounded_route(const MixedArgs &a, hipStream_t s)
+{
+    if (!a.route || a.n_groups != a.base.t_stride / kGroup) return hipErrorInvalidValue;
+    const int n_items = a.n_groups * (a.base.n_src_pad / kMixedTile);
+    const int blocks = ceil_div(n_items, 256);
+    hipLaunchKernelGGL(bounded_route_kernel, dim3(blocks < 4096 ? blocks : 4096), dim3(256), 0,
+                       s, a, n_items);
+    return hipGetLastError();
+}
+
 // probe = false: the scale word already holds the velocity scale (max |u|
-// of the previous RK4 stage, folded in by its stage update).
+// of the previous RK4 stage, folded in by its stage update).  Ends with the
+// routing pass, which needs the scale.
 hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
 {
     if (!a.tile_mom || !a.scale || !a.sub_mom) return hipErrorInvalidValue;
@@ -207,9 +214,11 @@ hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
     hipLaunchKernelGGL(mixed_target_prep_kernel, dim3(a.base.t_stride / 256), dim3(256), 0, s,
                        a.base, a.n_targets, a.group_box);
     e = hipGetLastError();
-    if (e != hipSuccess || !probe) return e;
+    if (e != hipSuccess) return e;
+    if (!probe) return launch_bounded_route(a, s);
     e = launch_scale_reset(a.scale, s);
-    if (e != hipSuccess || a.n_targets <= 0) return e;
+    if (e != hipSuccess) return e;
+    if (a.n_targets <= 0) return launch_bounded_route(a, s);
     // the first target of every group_stride-th group of 64: at most ~1024
     // probes, a target every ~220 at level 7 (the groups are spatially
     // compact runs, so the probes cover the sphere); ~0.05 ms at level 7
@@ -217,7 +226,9 @@ hipError_t launch_bounded_prep(const MixedArgs &a, hipStream_t s, bool probe)
     const int stride = ceil_div(groups, 1024);
     hipLaunchKernelGGL(bounded_probe_kernel, dim3(ceil_div(groups, stride)), dim3(256), 0, s,
                        a.base, a.n_targets, a.n_sources, stride, a.scale);
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_bounded_route(a, s);
 }
 
 hipError_t launch_pair_sum_bounded(const MixedArgs &a, hipStream_t s)

@@ -98,6 +98,23 @@ def test_level7_gauss_within_target(chunk):
     assert err < TARGET
 
 
+@pytest.mark.parametrize("target", [1e-8, 1e-9, 1e-11])
+def test_level6_within_each_target(target):
+    """The contract at other targets: every routing regime (whole tiles in
+    float32, sub-tiles retried in float32 or float64, float64 tiles) mixes
+    differently with the target, and the error stays within it.  Level 6
+    runs one target per lane."""
+    g = gridmod.cubed_sphere(6)
+    za, zp, _ = ic.gauss_vortex(g, lat0=np.pi / 4, lon0=0.0, width=0.05)
+    absv = zp + 2 * OMEGA * g.particles_x[:, 2]
+    with bounded_context(g, za, absv, target) as c:
+        assert c.plan()[0] == 1
+        u = c.resident_velocity()
+    err = sampled_errors(g, za, u, [ic.latlon_to_xyz(np.pi / 4, 0.0)])
+    print(f"level 6 bounded, target {target:g}: {err:.3e}")
+    assert err < target
+
+
 def test_level8_two_vortices_within_target():
     """config[4]: level-8 two vortices."""
     g = gridmod.cubed_sphere(8)

@@ -722,7 +722,8 @@ struct BoundedLds {
 // Sources up to which a bounded pass reads its float32 records through
 // scalar loads: each wave then fetches every tile itself, which pays while
 // the records (32 B per source) stay in an XCD's 4 MB L2.  Level 7 (98,304
-// sources): 5.46 against 5.66 ms per launch with the records staged in LDS;
+// sources, 32 B records, whole-tile routing): 5.46 against 5.66 ms per launch
+// with the records staged in LDS;
 // level 8 (393,216): 87.6 against 84.7 ms.
 constexpr int kBoundedScalarMaxSources = 131072;
 
@@ -954,7 +955,8 @@ __device__ __forceinline__ void pair_bounded_range(const MixedArgs &a, int ttile
         const int i = tile_base + wave * (64 * T) + t * 64 + lane;
         // targets and partial rows stream (non-temporal): the L2 keeps the
         // source records, which every workgroup re-reads (level 7: FETCH_SIZE
-        // 273 against 483 MiB per launch, 5.33 against 5.46 ms)
+        // 273 against 483 MiB per launch, 5.33 against 5.46 ms, whole-tile
+        // routing)
         xi[t] = __builtin_nontemporal_load(&a.base.tx[i]);
         yi[t] = __builtin_nontemporal_load(&a.base.ty[i]);
         zi[t] = __builtin_nontemporal_load(&a.base.tz[i]);

@@ -719,12 +719,12 @@ struct BoundedLds {
     float2 wgt_alt[kMixedTile];     // SCALAR passes: wgt of odd tiles
 };
 
-// Sources up to which a bounded pass reads its float32 records through
-// scalar loads: each wave then fetches every tile itself, which pays while
-// the records (32 B per source) stay in an XCD's 4 MB L2.  Level 7 (98,304
-// sources, 32 B records, whole-tile routing): 5.46 against 5.66 ms per launch
-// with the records staged in LDS;
-// level 8 (393,216): 87.6 against 84.7 ms.
+// Sources up to which a bounded pass reads its position records through
+// scalar loads: each wave then fetches every tile's positions itself (16 B
+// per source), which pays while the records stay in an XCD's 4 MB L2.
+// Level 7 (98,304 sources): 5.46 against 5.66 ms per launch with every
+// record staged in LDS (32 B records, whole-tile routing); level 8
+// (393,216): 80.4 against 77.1 ms with the present records.
 constexpr int kBoundedScalarMaxSources = 131072;
 
 // Remainder sums R over NS float32 records (pos = {o', (w x).x},
